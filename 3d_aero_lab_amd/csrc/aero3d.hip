@@ -1,0 +1,489 @@
+// ============================================================================
+// aero3d.hip — libaero3d.so: HIP kernels and C ABI (include/aero3d.h) of the
+// plain 3D Euler cut-cell solver on one MI355X (gfx950).
+//
+// Device layout of one solve (N = NX*NY*NZ cells, cidx order, x fastest):
+//   state   3 buffers x 5 planes x ns floats (ns = N rounded up to 64):
+//           current Q, stage-1 result S1, stage-2 result B, rotated like the
+//           reference's three ping-pong textures (src/air_cfd.js:421-425)
+//   masks   t u8[N], phi f32[N], ap f32[4][N]
+//   grid    dx, dy, dz f32 widths, xc f32 centres
+//   dt      f32[N] local time step of the last stage
+//   rsnap   f32[N] density snapshot for the residual
+//   force   list q i32, cx, cz, x, z f64 and FP64 block partials
+// ============================================================================
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/aero3d.h"
+#include "euler3d.hpp"
+
+using namespace aero3d;
+
+namespace {
+
+std::string g_create_error;   // message of the last failed aero3d_create
+
+constexpr int kBlock = 256;
+constexpr int kRedBlock = 256;
+constexpr int kRedMaxBlocks = 1024;
+
+}  // namespace
+
+struct aero3d_solver {
+  int NX = 0, NY = 0, NZ = 0, N = 0;
+  size_t ns = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  float *dx = nullptr, *dy = nullptr, *dz = nullptr;
+  std::vector<float> hxc, hdy, hdz;
+  uint8_t *t = nullptr;
+  float *phi = nullptr, *ap = nullptr;
+  float *state = nullptr;     // 3 * 5 * ns
+  float *dt = nullptr, *rsnap = nullptr;
+  int cur = 0;                // buffer index holding the current state Q
+  int it = 0, snap_it = -1;
+  double F[5] = {1, 0, 0, 0, 1 / (1.4 * 0.4)};
+  double fix[3][5];
+  double cfl = 0.7;
+  int order = 2;
+  // force list
+  int nf = 0;
+  int32_t *fq = nullptr;
+  double *fcx = nullptr, *fcz = nullptr, *fx = nullptr, *fz = nullptr;
+  double *partials = nullptr, *sums = nullptr;
+  std::string err;
+  int64_t bytes = 0;
+};
+
+namespace {
+
+int fail(aero3d_solver *s, int code, const std::string &msg) {
+  if (s) s->err = msg; else g_create_error = msg;
+  return code;
+}
+
+#define HIPCHK(s, expr)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail((s), AERO3D_E_HIP_BASE - (int)e_, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int dalloc(aero3d_solver *s, T **p, size_t n) {
+  HIPCHK(s, hipMalloc((void **)p, sizeof(T) * (n ? n : 1)));
+  s->bytes += (int64_t)(sizeof(T) * n);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) stage_kernel(CaseView cv, StageConsts kc, const float *__restrict__ qin,
+                                                       const float *__restrict__ bse, float *__restrict__ out,
+                                                       float *__restrict__ dtout, size_t ns, float wa, float wb) {
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= cv.N) return;
+  stage_cell(cv, kc, qin, bse, out, dtout, ns, wa, wb, q);
+}
+
+// Reset: every plane of buffer `dst` := F.
+__global__ void fill_kernel(float *__restrict__ dst, size_t ns, int N, float f0, float f1, float f2, float f3, float f4) {
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= N) return;
+  dst[q] = f0; dst[ns + q] = f1; dst[2 * ns + q] = f2; dst[3 * ns + q] = f3; dst[4 * ns + q] = f4;
+}
+
+__global__ void copy_kernel(float *__restrict__ dst, const float *__restrict__ src, int N) {
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q < N) dst[q] = src[q];
+}
+
+// Wavefront (64-lane) then block sum of K doubles; lane 0 of wave 0 holds the
+// block total. Fixed shape, fixed order: the result depends only on inputs.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double (*lds)[K]) {
+#pragma unroll
+  for (int c = 0; c < K; c++)
+    for (int off = 32; off > 0; off >>= 1) v[c] += __shfl_down(v[c], off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < K; c++) lds[wave][c] = v[c];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < K; c++) {
+      double a = 0;
+      for (int w = 0; w < kRedBlock / 64; w++) a += lds[w][c];
+      v[c] = a;
+    }
+}
+
+// Pressure force over the force list (cfdForces, src/air_cfd.js:443-449), FP64.
+__global__ void __launch_bounds__(kRedBlock) forces_kernel(const float *__restrict__ Q, size_t ns, const int32_t *__restrict__ fq,
+                                                           const double *__restrict__ cx, const double *__restrict__ cz,
+                                                           const double *__restrict__ x, const double *__restrict__ z, int n,
+                                                           double xcg, double zcg, double *__restrict__ partials) {
+  __shared__ double lds[kRedBlock / 64][3];
+  double v[3] = {0, 0, 0};
+  const double pinf = 1.0 / 1.4;
+  for (int s = blockIdx.x * kRedBlock + threadIdx.x; s < n; s += gridDim.x * kRedBlock) {
+    const int q = fq[s];
+    const double r = Q[q], u = Q[ns + q] / r, vv = Q[2 * ns + q] / r, w = Q[3 * ns + q] / r;
+    const double dp = (1.4 - 1) * (Q[4 * ns + q] - 0.5 * r * (u * u + vv * vv + w * w)) - pinf;
+    const double fx = -dp * cx[s], fz = -dp * cz[s];
+    v[0] += fx; v[1] += fz; v[2] += (z[s] - zcg) * fx - (x[s] - xcg) * fz;
+  }
+  block_sum<3>(v, lds);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 3; c++) partials[blockIdx.x * 3 + c] = v[c];
+}
+
+// Trefftz plane sum (src/air_cfd.js:563-568), FP64, one thread per (j, k).
+__global__ void __launch_bounds__(kRedBlock) trefftz_kernel(const float *__restrict__ Q, size_t ns, const uint8_t *__restrict__ t,
+                                                            const float *__restrict__ dy, const float *__restrict__ dz,
+                                                            int NX, int NY, int NZ, int i0, double tx, double ca, double sa,
+                                                            double *__restrict__ partials) {
+  __shared__ double lds[kRedBlock / 64][1];
+  double v[1] = {0};
+  for (int jk = blockIdx.x * kRedBlock + threadIdx.x; jk < NY * NZ; jk += gridDim.x * kRedBlock) {
+    const int j = jk % NY, k = jk / NY;
+    double u = 0, vv = 0, w = 0, ws = 0;
+    for (int di = 0; di < 2; di++) {
+      const int q = (k * NY + j) * NX + i0 + di;
+      if (t[q] != 0) continue;
+      const double ww = di ? tx : 1 - tx, r = Q[q];
+      u += ww * Q[ns + q] / r; vv += ww * Q[2 * ns + q] / r; w += ww * Q[3 * ns + q] / r; ws += ww;
+    }
+    if (ws < 1e-6) continue;
+    u /= ws; vv /= ws; w /= ws;
+    const double vn = w * ca - u * sa;
+    v[0] += (vv * vv + vn * vn) * (double)dy[j] * (double)dz[k];
+  }
+  block_sum<1>(v, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
+}
+
+// Density residual terms (src/air_cfd.js:554): sum of squared change and count
+// over every third fluid cell, FP64.
+__global__ void __launch_bounds__(kRedBlock) residual_kernel(const float *__restrict__ r, const float *__restrict__ prev,
+                                                             const uint8_t *__restrict__ t, int N, double *__restrict__ partials) {
+  __shared__ double lds[kRedBlock / 64][2];
+  double v[2] = {0, 0};
+  const int M = (N + 2) / 3;
+  for (int m = blockIdx.x * kRedBlock + threadIdx.x; m < M; m += gridDim.x * kRedBlock) {
+    const int q = 3 * m;
+    if (t[q] != 0) continue;
+    const double d = (double)r[q] - (double)prev[q];
+    v[0] += d * d; v[1] += 1;
+  }
+  block_sum<2>(v, lds);
+  if (threadIdx.x == 0) { partials[blockIdx.x * 2] = v[0]; partials[blockIdx.x * 2 + 1] = v[1]; }
+}
+
+// Final fixed-order sum of nb block partials of K components.
+template <int K>
+__global__ void __launch_bounds__(kRedBlock) finish_kernel(const double *__restrict__ partials, int nb, double *__restrict__ out) {
+  __shared__ double lds[kRedBlock / 64][K];
+  double v[K];
+#pragma unroll
+  for (int c = 0; c < K; c++) v[c] = 0;
+  for (int b = threadIdx.x; b < nb; b += kRedBlock)
+#pragma unroll
+    for (int c = 0; c < K; c++) v[c] += partials[b * K + c];
+  block_sum<K>(v, lds);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < K; c++) out[c] = v[c];
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+void prim5_host(const double *a, double *o) {
+  const double r = std::max(a[0], 1e-6), u = a[1] / r, v = a[2] / r, w = a[3] / r;
+  o[0] = r; o[1] = u; o[2] = v; o[3] = w;
+  o[4] = std::max((1.4 - 1) * (a[4] - 0.5 * r * (u * u + v * v + w * w)), 1e-7);
+}
+
+StageConsts make_consts(const aero3d_solver *s) {
+  StageConsts k;
+  double w[5];
+  prim5_host(s->F, w);
+  for (int v = 0; v < 5; v++) k.Fw[v] = (float)w[v];
+  for (int g = 0; g < 3; g++) {
+    prim5_host(s->fix[g], w);
+    for (int v = 0; v < 5; v++) { k.fixw[g][v] = (float)w[v]; k.fixq[g][v] = (float)s->fix[g][v]; }
+  }
+  k.cfl = (float)s->cfl;
+  k.order = s->order;
+  return k;
+}
+
+CaseView make_view(const aero3d_solver *s) {
+  CaseView cv;
+  cv.NX = s->NX; cv.NY = s->NY; cv.NZ = s->NZ; cv.N = s->N;
+  cv.dx = s->dx; cv.dy = s->dy; cv.dz = s->dz;
+  cv.t = s->t; cv.phi = s->phi; cv.ap = s->ap;
+  return cv;
+}
+
+inline float *buf(aero3d_solver *s, int b) { return s->state + (size_t)(b % 3) * 5 * s->ns; }
+
+int enqueue_steps(aero3d_solver *s, int n) {
+  const CaseView cv = make_view(s);
+  const StageConsts kc = make_consts(s);
+  const dim3 grid((s->N + kBlock - 1) / kBlock);
+  for (int k = 0; k < n; k++) {
+    float *Q = buf(s, s->cur), *S1 = buf(s, s->cur + 1), *B = buf(s, s->cur + 2);
+    stage_kernel<<<grid, kBlock, 0, s->stream>>>(cv, kc, Q, Q, S1, nullptr, s->ns, 0.f, 1.f);
+    stage_kernel<<<grid, kBlock, 0, s->stream>>>(cv, kc, S1, Q, B, s->dt, s->ns, 0.5f, 0.5f);
+    s->cur = (s->cur + 2) % 3;
+    s->it++;
+  }
+  HIPCHK(s, hipGetLastError());
+  return 0;
+}
+
+int reduce_blocks(int n) { int nb = (n + kRedBlock - 1) / kRedBlock; return nb < 1 ? 1 : (nb > kRedMaxBlocks ? kRedMaxBlocks : nb); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int aero3d_create(const aero3d_grid *g, const uint8_t *t, const float *phi, const float *ap0, const float *ap1,
+                  const float *ap2, const float *ap3, aero3d_solver **out) {
+  if (!out) return fail(nullptr, AERO3D_E_ARG, "aero3d_create: null output handle");
+  *out = nullptr;
+  if (!g || !g->fx || !g->fy || !g->fz || !t || !phi || !ap0 || !ap1 || !ap2 || !ap3)
+    return fail(nullptr, AERO3D_E_ARG, "aero3d_create: null argument");
+  if (g->NX < 1 || g->NY < 1 || g->NZ < 1 || (int64_t)g->NX * g->NY * g->NZ > (int64_t)1 << 30)
+    return fail(nullptr, AERO3D_E_ARG, "aero3d_create: grid size out of range");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, AERO3D_E_NODEV, "aero3d_create: no HIP device");
+  aero3d_solver *s = new aero3d_solver();
+  s->NX = g->NX; s->NY = g->NY; s->NZ = g->NZ; s->N = g->NX * g->NY * g->NZ;
+  s->ns = ((size_t)s->N + 63) / 64 * 64;
+  for (int k = 0; k < 3; k++) std::memcpy(s->fix[k], s->F, sizeof s->F);
+  auto bail = [&](int rc) { g_create_error = s->err; aero3d_destroy(s); return rc; };
+  int rc;
+  if (hipGetDevice(&s->device) != hipSuccess) return bail(fail(s, AERO3D_E_NODEV, "hipGetDevice failed"));
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(s, AERO3D_E_NODEV, "hipStreamCreate failed"));
+  // grid: FP32 widths and centres from FP64 faces, as the reference stores them
+  std::vector<float> hdx(s->NX), hdy(s->NY), hdz(s->NZ);
+  s->hxc.resize(s->NX);
+  for (int i = 0; i < s->NX; i++) { hdx[i] = (float)(g->fx[i + 1] - g->fx[i]); s->hxc[i] = (float)(0.5 * (g->fx[i] + g->fx[i + 1])); }
+  for (int j = 0; j < s->NY; j++) hdy[j] = (float)(g->fy[j + 1] - g->fy[j]);
+  for (int k = 0; k < s->NZ; k++) hdz[k] = (float)(g->fz[k + 1] - g->fz[k]);
+  s->hdy = hdy; s->hdz = hdz;
+  const size_t N = (size_t)s->N;
+  if ((rc = dalloc(s, &s->dx, s->NX)) || (rc = dalloc(s, &s->dy, s->NY)) || (rc = dalloc(s, &s->dz, s->NZ)) ||
+      (rc = dalloc(s, &s->t, N)) || (rc = dalloc(s, &s->phi, N)) || (rc = dalloc(s, &s->ap, 4 * N)) ||
+      (rc = dalloc(s, &s->state, 15 * s->ns)) || (rc = dalloc(s, &s->dt, N)) || (rc = dalloc(s, &s->rsnap, N)) ||
+      (rc = dalloc(s, &s->partials, 3 * kRedMaxBlocks)) || (rc = dalloc(s, &s->sums, 8)))
+    return bail(rc);
+  auto up = [&](void *d, const void *h, size_t b) { return hipMemcpy(d, h, b, hipMemcpyHostToDevice); };
+  if (up(s->dx, hdx.data(), 4 * s->NX) || up(s->dy, hdy.data(), 4 * s->NY) || up(s->dz, hdz.data(), 4 * s->NZ) ||
+      up(s->t, t, N) || up(s->phi, phi, 4 * N) || up(s->ap, ap0, 4 * N) || up(s->ap + N, ap1, 4 * N) ||
+      up(s->ap + 2 * N, ap2, 4 * N) || up(s->ap + 3 * N, ap3, 4 * N) ||
+      hipMemset(s->state, 0, sizeof(float) * 15 * s->ns) || hipMemset(s->dt, 0, 4 * N))
+    return bail(fail(s, AERO3D_E_HIP_BASE, "aero3d_create: upload failed"));
+  *out = s;
+  return 0;
+}
+
+int aero3d_destroy(aero3d_solver *s) {
+  if (!s) return 0;
+  if (s->stream) hipStreamSynchronize(s->stream);
+  void *ptrs[] = {s->dx, s->dy, s->dz, s->t, s->phi, s->ap, s->state, s->dt, s->rsnap, s->fq, s->fcx, s->fcz, s->fx, s->fz, s->partials, s->sums};
+  for (void *p : ptrs) if (p) hipFree(p);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+  return 0;
+}
+
+const char *aero3d_last_error(const aero3d_solver *s) { return s ? s->err.c_str() : g_create_error.c_str(); }
+
+int aero3d_set_free(aero3d_solver *s, const double F[5]) {
+  if (!s || !F) return fail(s, AERO3D_E_ARG, "set_free: null argument");
+  std::memcpy(s->F, F, sizeof s->F);
+  return 0;
+}
+int aero3d_set_fix(aero3d_solver *s, const double fix[15]) {
+  if (!s || !fix) return fail(s, AERO3D_E_ARG, "set_fix: null argument");
+  std::memcpy(s->fix, fix, sizeof s->fix);
+  return 0;
+}
+int aero3d_set_opt(aero3d_solver *s, double cfl, int order) {
+  if (!s) return AERO3D_E_ARG;
+  if (!(cfl > 0) || !(cfl < 10) || (order != 1 && order != 2)) return fail(s, AERO3D_E_ARG, "set_opt: cfl must be in (0,10), order 1 or 2");
+  s->cfl = cfl; s->order = order;
+  return 0;
+}
+int aero3d_reset(aero3d_solver *s) {
+  if (!s) return AERO3D_E_ARG;
+  s->cur = 0; s->it = 0; s->snap_it = -1;
+  fill_kernel<<<(s->N + kBlock - 1) / kBlock, kBlock, 0, s->stream>>>(buf(s, 0), s->ns, s->N, (float)s->F[0], (float)s->F[1],
+                                                                      (float)s->F[2], (float)s->F[3], (float)s->F[4]);
+  HIPCHK(s, hipGetLastError());
+  return 0;
+}
+int aero3d_step(aero3d_solver *s, int n) {
+  if (!s || n < 0) return fail(s, AERO3D_E_ARG, "step: n must be >= 0");
+  return enqueue_steps(s, n);
+}
+int aero3d_iter(const aero3d_solver *s, int *it) {
+  if (!s || !it) return AERO3D_E_ARG;
+  *it = s->it;
+  return 0;
+}
+int aero3d_sync(aero3d_solver *s) {
+  if (!s) return AERO3D_E_ARG;
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return 0;
+}
+int aero3d_read(aero3d_solver *s, float *r, float *ru, float *rv, float *rw, float *E, float *dt) {
+  if (!s) return AERO3D_E_ARG;
+  float *Q = buf(s, s->cur);
+  float *dst[5] = {r, ru, rv, rw, E};
+  const size_t b = sizeof(float) * (size_t)s->N;
+  for (int c = 0; c < 5; c++)
+    if (dst[c]) HIPCHK(s, hipMemcpyAsync(dst[c], Q + c * s->ns, b, hipMemcpyDeviceToHost, s->stream));
+  if (dt) HIPCHK(s, hipMemcpyAsync(dt, s->dt, b, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return 0;
+}
+int aero3d_write(aero3d_solver *s, const float *r, const float *ru, const float *rv, const float *rw, const float *E) {
+  if (!s || !r || !ru || !rv || !rw || !E) return fail(s, AERO3D_E_ARG, "write: null argument");
+  float *Q = buf(s, s->cur);
+  const float *src[5] = {r, ru, rv, rw, E};
+  for (int c = 0; c < 5; c++)
+    HIPCHK(s, hipMemcpyAsync(Q + c * s->ns, src[c], sizeof(float) * (size_t)s->N, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int aero3d_set_force_list(aero3d_solver *s, const int32_t *q, const double *cx, const double *cz, const double *x,
+                          const double *z, int n) {
+  if (!s || n < 0 || (n > 0 && (!q || !cx || !cz || !x || !z))) return fail(s, AERO3D_E_ARG, "set_force_list: bad argument");
+  for (int k = 0; k < n; k++)
+    if (q[k] < 0 || q[k] >= s->N) return fail(s, AERO3D_E_ARG, "set_force_list: cell index out of range");
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  void *old[] = {s->fq, s->fcx, s->fcz, s->fx, s->fz};
+  for (void *p : old) if (p) hipFree(p);
+  s->fq = nullptr; s->fcx = s->fcz = s->fx = s->fz = nullptr;
+  int rc;
+  if ((rc = dalloc(s, &s->fq, n)) || (rc = dalloc(s, &s->fcx, n)) || (rc = dalloc(s, &s->fcz, n)) ||
+      (rc = dalloc(s, &s->fx, n)) || (rc = dalloc(s, &s->fz, n)))
+    return rc;
+  if (n > 0) {
+    HIPCHK(s, hipMemcpy(s->fq, q, 4 * (size_t)n, hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->fcx, cx, 8 * (size_t)n, hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->fcz, cz, 8 * (size_t)n, hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->fx, x, 8 * (size_t)n, hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->fz, z, 8 * (size_t)n, hipMemcpyHostToDevice));
+  }
+  s->nf = n;
+  return 0;
+}
+
+int aero3d_forces(aero3d_solver *s, double alphaDeg, double M, double S, double xcg, double zcg, double cref, double out[3]) {
+  if (!s || !out) return AERO3D_E_ARG;
+  if (!s->fq) return fail(s, AERO3D_E_STATE, "forces: no force list set");
+  const int nb = reduce_blocks(s->nf);
+  forces_kernel<<<nb, kRedBlock, 0, s->stream>>>(buf(s, s->cur), s->ns, s->fq, s->fcx, s->fcz, s->fx, s->fz, s->nf, xcg, zcg, s->partials);
+  finish_kernel<3><<<1, kRedBlock, 0, s->stream>>>(s->partials, nb, s->sums);
+  HIPCHK(s, hipGetLastError());
+  double F3[3];
+  HIPCHK(s, hipMemcpyAsync(F3, s->sums, sizeof F3, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  // wind-axis rotation and half-model doubling (src/air_cfd.js:450-451)
+  const double a = alphaDeg * (M_PI / 180), q = 0.5 * M * M, k = 2 / (q * S);
+  out[0] = k * (F3[1] * std::cos(a) - F3[0] * std::sin(a));
+  out[1] = k * (F3[0] * std::cos(a) + F3[1] * std::sin(a));
+  out[2] = k * F3[2] / cref;
+  return 0;
+}
+
+int aero3d_trefftz(aero3d_solver *s, double xT, double alphaDeg, double M, double S, double *cdi) {
+  if (!s || !cdi) return AERO3D_E_ARG;
+  if (s->NX < 2) return fail(s, AERO3D_E_ARG, "trefftz: needs NX >= 2");
+  // cfdLocate on the FP32 centres (src/air_cfd.js:453)
+  const std::vector<float> &c = s->hxc;
+  const int n = s->NX;
+  int i0; double tx;
+  if (xT <= c[0]) { i0 = 0; tx = 0; }
+  else if (xT >= c[n - 1]) { i0 = n - 2; tx = 1; }
+  else {
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) { const int m = (lo + hi) >> 1; if (c[m] <= xT) lo = m; else hi = m; }
+    i0 = lo; tx = (xT - c[lo]) / ((double)c[hi] - c[lo]);
+  }
+  const double a = alphaDeg * (M_PI / 180);
+  const int nb = reduce_blocks(s->NY * s->NZ);
+  trefftz_kernel<<<nb, kRedBlock, 0, s->stream>>>(buf(s, s->cur), s->ns, s->t, s->dy, s->dz, s->NX, s->NY, s->NZ, i0, tx,
+                                                  std::cos(a), std::sin(a), s->partials);
+  finish_kernel<1><<<1, kRedBlock, 0, s->stream>>>(s->partials, nb, s->sums);
+  HIPCHK(s, hipGetLastError());
+  double sum;
+  HIPCHK(s, hipMemcpyAsync(&sum, s->sums, sizeof sum, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  const double Mn = std::max(M, 0.05);
+  *cdi = 2 * sum / (Mn * Mn * S);
+  return 0;
+}
+
+int aero3d_residual(aero3d_solver *s, double *res) {
+  if (!s || !res) return AERO3D_E_ARG;
+  const float *r = buf(s, s->cur);
+  *res = NAN;
+  const int nIt = s->it - s->snap_it;
+  if (s->snap_it >= 0 && nIt > 0) {
+    const int nb = reduce_blocks((s->N + 2) / 3);
+    residual_kernel<<<nb, kRedBlock, 0, s->stream>>>(r, s->rsnap, s->t, s->N, s->partials);
+    finish_kernel<2><<<1, kRedBlock, 0, s->stream>>>(s->partials, nb, s->sums);
+    HIPCHK(s, hipGetLastError());
+    double v[2];
+    HIPCHK(s, hipMemcpyAsync(v, s->sums, sizeof v, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    *res = std::sqrt(v[0] / std::max(1.0, v[1])) / nIt;
+  }
+  copy_kernel<<<(s->N + kBlock - 1) / kBlock, kBlock, 0, s->stream>>>(s->rsnap, r, s->N);
+  HIPCHK(s, hipGetLastError());
+  s->snap_it = s->it;
+  return 0;
+}
+
+int aero3d_time_steps(aero3d_solver *s, int n, float *ms) {
+  if (!s || !ms || n < 0) return AERO3D_E_ARG;
+  hipEvent_t e0, e1;
+  HIPCHK(s, hipEventCreate(&e0));
+  HIPCHK(s, hipEventCreate(&e1));
+  HIPCHK(s, hipEventRecord(e0, s->stream));
+  int rc = enqueue_steps(s, n);
+  if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
+  HIPCHK(s, hipEventRecord(e1, s->stream));
+  HIPCHK(s, hipEventSynchronize(e1));
+  HIPCHK(s, hipEventElapsedTime(ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return 0;
+}
+
+int aero3d_footprint(const aero3d_solver *s, int64_t *bytes) {
+  if (!s || !bytes) return AERO3D_E_ARG;
+  *bytes = s->bytes;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+/* ============================================================================
+ * aero3d.h — C ABI of the MI355X 3D Euler cut-cell solver (libaero3d.so).
+ *
+ * One handle = one solve: the library owns the device copies of the grid,
+ * the cut-cell masks and three FP32 structure-of-arrays state buffers, plus a
+ * HIP stream on which every call of the handle is ordered. Host arrays stay
+ * owned by the caller and are copied (read() copies out, unlike the live
+ * Float32Array references of the reference's CfdCPU.read).
+ *
+ * Conventions: every function returns 0 on success or a negative code
+ * (AERO3D_E_*, or -(1000 + hipError_t) for a HIP failure); the message of the
+ * last failure is returned by aero3d_last_error(). A handle must not be used
+ * by two threads at once. Entry points are listed with the reference
+ * interface each one replaces (file:line in the reference checkout).
+ * ========================================================================== */
+#ifndef AERO3D_H
+#define AERO3D_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AERO3D_OK 0
+#define AERO3D_E_ARG (-1)        /* bad argument (null pointer, size, option)   */
+#define AERO3D_E_NODEV (-2)      /* no usable HIP device                        */
+#define AERO3D_E_STATE (-3)      /* call out of order (e.g. no force list set)  */
+#define AERO3D_E_HIP_BASE (-1000)
+
+typedef struct aero3d_solver aero3d_solver;
+
+/* Grid of the half domain y >= 0: face coordinates of each axis (Float64,
+ * NX+1 / NY+1 / NZ+1 entries), as the reference's g.X.f / g.Y.f / g.Z.f
+ * (src/air_cfd.js:36). Cell widths and centres are rounded to FP32 exactly as
+ * the reference grid stores them. */
+typedef struct {
+  int NX, NY, NZ;
+  const double *fx, *fy, *fz;
+} aero3d_grid;
+
+/* new CfdCPU(g, geo) / new CfdGPU(g, geo) (src/air_cfd.js:157, 367).
+ * t: cell flags (0 fluid, 1 solid, 3 inlet sink, 4 bypass jet, 5 core jet),
+ * phi: fluid volume fraction, ap0..ap2: + face apertures, ap3: baffle slope of
+ * the + z face; all N = NX*NY*NZ entries in cidx order (x fastest,
+ * src/air_cfd.js:45). Throws (returns < 0) when no device is usable. */
+int aero3d_create(const aero3d_grid *g, const uint8_t *t, const float *phi, const float *ap0, const float *ap1,
+                  const float *ap2, const float *ap3, aero3d_solver **out);
+/* CfdGPU.dispose (src/air_cfd.js:438). Null is accepted. */
+int aero3d_destroy(aero3d_solver *s);
+/* Message of the last failure on this handle (or of the last failed create
+ * when s is null). */
+const char *aero3d_last_error(const aero3d_solver *s);
+
+/* setFree(F) (src/air_cfd.js:166): freestream conserved state [rho, rho u, rho v, rho w, E]. */
+int aero3d_set_free(aero3d_solver *s, const double F[5]);
+/* setFix(fix) (src/air_cfd.js:167): fixed states of cell flags 3, 4, 5, row-major 3 x 5. */
+int aero3d_set_fix(aero3d_solver *s, const double fix[15]);
+/* opt = {cfl, order} (src/air_cfd.js:162), read at every step. order is 1 or 2. */
+int aero3d_set_opt(aero3d_solver *s, double cfl, int order);
+/* reset() (src/air_cfd.js:168): state := F everywhere, it := 0. */
+int aero3d_reset(aero3d_solver *s);
+/* step(n) (src/air_cfd.js:252-258, 414-428): n SSP-RK2 iterations, enqueued
+ * on the handle's stream; returns without waiting. */
+int aero3d_step(aero3d_solver *s, int n);
+/* Iteration count `it`. */
+int aero3d_iter(const aero3d_solver *s, int *it);
+/* read() (src/air_cfd.js:259, 429-437): copy the current state (and the
+ * local dt of the last iteration when dt is non-null) to caller arrays of N
+ * floats; waits for the stream. Any output pointer may be null. */
+int aero3d_read(aero3d_solver *s, float *r, float *ru, float *rv, float *rw, float *E, float *dt);
+/* GPUSolver.setState (src/cfd_gpu.js:259-261) for the 3D solver: overwrite
+ * the current state from caller arrays of N floats. */
+int aero3d_write(aero3d_solver *s, const float *r, const float *ru, const float *rv, const float *rw, const float *E);
+/* Wait for all work of the handle. */
+int aero3d_sync(aero3d_solver *s);
+
+/* cfdForceList (src/air_cfd.js:143-152) uploaded once: n entries of cell
+ * index q, aperture-jump areas cx, cz and cell centre x, z. */
+int aero3d_set_force_list(aero3d_solver *s, const int32_t *q, const double *cx, const double *cz, const double *x,
+                          const double *z, int n);
+/* cfdForces(g, list, read(), alpha, M, S, xcg, zcg, cref) (src/air_cfd.js:443-452)
+ * on the device state, FP64 sums: out = {CL, CD, Cm}. Waits for the stream. */
+int aero3d_forces(aero3d_solver *s, double alphaDeg, double M, double S, double xcg, double zcg, double cref,
+                  double out[3]);
+/* Trefftz-plane induced drag of CFD3.readback (src/air_cfd.js:560-570) at
+ * plane xT, FP64 sums. Waits for the stream. */
+int aero3d_trefftz(aero3d_solver *s, double xT, double alphaDeg, double M, double S, double *cdi);
+/* Density residual of CFD3.readback (src/air_cfd.js:553-554): RMS over every
+ * third fluid cell of rho minus the density snapshot, divided by the
+ * iterations since the snapshot; then the snapshot is retaken. The first call
+ * after create/reset only takes the snapshot and returns *res = NaN. */
+int aero3d_residual(aero3d_solver *s, double *res);
+
+/* Benchmark helper: run n iterations on the handle's stream bracketed by HIP
+ * events recorded on that stream; *ms = elapsed device time. */
+int aero3d_time_steps(aero3d_solver *s, int n, float *ms);
+/* Device-memory footprint of the handle in bytes. */
+int aero3d_footprint(const aero3d_solver *s, int64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
