@@ -22,7 +22,7 @@ i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 EXPORTS = ["aero3d_create", "aero3d_destroy", "aero3d_last_error", "aero3d_set_free", "aero3d_set_fix",
            "aero3d_set_opt", "aero3d_reset", "aero3d_step", "aero3d_iter", "aero3d_read", "aero3d_write",
            "aero3d_sync", "aero3d_set_force_list", "aero3d_forces", "aero3d_trefftz", "aero3d_residual",
-           "aero3d_time_steps", "aero3d_footprint"]
+           "aero3d_time_steps", "aero3d_set_tile", "aero3d_get_tile", "aero3d_autotune", "aero3d_footprint"]
 
 
 class Grid(C.Structure):
@@ -60,6 +60,9 @@ def lib():
         L.aero3d_residual.argtypes = [vp, C.POINTER(C.c_double)]
         L.aero3d_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_float)]
         L.aero3d_footprint.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.aero3d_set_tile.argtypes = [vp, C.c_int, C.c_int]
+        L.aero3d_get_tile.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.aero3d_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
         _lib = L
     return _lib
 
@@ -171,6 +174,21 @@ class CfdHIP:
         ms = C.c_float()
         self._chk(lib().aero3d_time_steps(self.h, int(n), C.byref(ms)))
         return ms.value
+
+    def set_tile(self, ty, zchunk):
+        self._chk(lib().aero3d_set_tile(self.h, ty, zchunk))
+
+    def get_tile(self):
+        a, b = C.c_int(), C.c_int()
+        self._chk(lib().aero3d_get_tile(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, iters=10):
+        """Pick the fastest stage-kernel shape for this grid; returns (ty, zchunk, ms_per_iter)."""
+        self._sync_opt()
+        a, b, ms = C.c_int(), C.c_int(), C.c_float()
+        self._chk(lib().aero3d_autotune(self.h, iters, C.byref(a), C.byref(b), C.byref(ms)))
+        return a.value, b.value, ms.value
 
     def footprint(self):
         v = C.c_int64()

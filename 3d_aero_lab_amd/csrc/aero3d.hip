@@ -22,6 +22,7 @@
 
 #include "../../include/aero3d.h"
 #include "euler3d.hpp"
+#include "euler3d_tile.hpp"
 
 using namespace aero3d;
 
@@ -47,6 +48,7 @@ struct aero3d_solver {
   float *state = nullptr;     // 3 * 5 * ns
   float *dt = nullptr, *rsnap = nullptr;
   int cur = 0;                // buffer index holding the current state Q
+  int tile_ty = 8, tile_zc = 8; // stage kernel: 0 = per-cell kernel, else 64 x tile_ty tiles, z chunks of tile_zc
   int it = 0, snap_it = -1;
   double F[5] = {1, 0, 0, 0, 1 / (1.4 * 0.4)};
   double fix[3][5];
@@ -91,6 +93,26 @@ __global__ void __launch_bounds__(kBlock) stage_kernel(CaseView cv, StageConsts 
   const int q = blockIdx.x * kBlock + threadIdx.x;
   if (q >= cv.N) return;
   stage_cell(cv, kc, qin, bse, out, dtout, ns, wa, wb, q);
+}
+
+// Hardware dispatches workgroup b to XCD b % 8; give each XCD a contiguous
+// range of logical tiles so x-y neighbours (which read each other's halos)
+// share an L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int per = nb / 8, rem = nb % 8, x = b % 8, l = b / 8;
+  return x < rem ? x * (per + 1) + l : rem * (per + 1) + (x - rem) * per + l;
+}
+
+template <int TY>
+__global__ void __launch_bounds__(64 * TY) tile_kernel(CaseView cv, StageConsts kc, const float *__restrict__ qin,
+                                                      const float *__restrict__ bse, float *__restrict__ out,
+                                                      float *__restrict__ dtout, size_t ns, float wa, float wb, int ntx, int nty,
+                                                      int zc) {
+  __shared__ TileSmem<64, TY> sm;
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int bx = lb % ntx, by = (lb / ntx) % nty, bz = lb / (ntx * nty);
+  const int k0 = bz * zc, k1 = min(cv.NZ, k0 + zc);
+  tile_stage<64, TY>(cv, kc, qin, bse, out, dtout, ns, wa, wb, bx * 64, by * TY, k0, k1, sm);
 }
 
 // Reset: every plane of buffer `dst` := F.
@@ -237,14 +259,27 @@ CaseView make_view(const aero3d_solver *s) {
 
 inline float *buf(aero3d_solver *s, int b) { return s->state + (size_t)(b % 3) * 5 * s->ns; }
 
+void launch_stage(aero3d_solver *s, const CaseView &cv, const StageConsts &kc, const float *qin, const float *bse, float *out,
+                  float *dtout, float wa, float wb) {
+  if (s->tile_ty == 0) {
+    stage_kernel<<<(s->N + kBlock - 1) / kBlock, kBlock, 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb);
+    return;
+  }
+  const int ntx = (s->NX + 63) / 64, nty = (s->NY + s->tile_ty - 1) / s->tile_ty, nzc = (s->NZ + s->tile_zc - 1) / s->tile_zc;
+  const int nb = ntx * nty * nzc;
+  if (s->tile_ty == 4)
+    tile_kernel<4><<<nb, dim3(64, 4), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty, s->tile_zc);
+  else
+    tile_kernel<8><<<nb, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty, s->tile_zc);
+}
+
 int enqueue_steps(aero3d_solver *s, int n) {
   const CaseView cv = make_view(s);
   const StageConsts kc = make_consts(s);
-  const dim3 grid((s->N + kBlock - 1) / kBlock);
   for (int k = 0; k < n; k++) {
     float *Q = buf(s, s->cur), *S1 = buf(s, s->cur + 1), *B = buf(s, s->cur + 2);
-    stage_kernel<<<grid, kBlock, 0, s->stream>>>(cv, kc, Q, Q, S1, nullptr, s->ns, 0.f, 1.f);
-    stage_kernel<<<grid, kBlock, 0, s->stream>>>(cv, kc, S1, Q, B, s->dt, s->ns, 0.5f, 0.5f);
+    launch_stage(s, cv, kc, Q, Q, S1, nullptr, 0.f, 1.f);
+    launch_stage(s, cv, kc, S1, Q, B, k == n - 1 ? s->dt : nullptr, 0.5f, 0.5f);
     s->cur = (s->cur + 2) % 3;
     s->it++;
   }
@@ -477,6 +512,54 @@ int aero3d_time_steps(aero3d_solver *s, int n, float *ms) {
   HIPCHK(s, hipEventElapsedTime(ms, e0, e1));
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  return 0;
+}
+
+int aero3d_set_tile(aero3d_solver *s, int ty, int zchunk) {
+  if (!s) return AERO3D_E_ARG;
+  if (!(ty == 0 || ty == 4 || ty == 8) || (ty != 0 && zchunk < 1))
+    return fail(s, AERO3D_E_ARG, "set_tile: ty must be 0, 4 or 8 and zchunk >= 1");
+  s->tile_ty = ty;
+  s->tile_zc = zchunk < 1 ? 1 : zchunk;
+  return 0;
+}
+int aero3d_get_tile(const aero3d_solver *s, int *ty, int *zchunk) {
+  if (!s || !ty || !zchunk) return AERO3D_E_ARG;
+  *ty = s->tile_ty; *zchunk = s->tile_zc;
+  return 0;
+}
+
+int aero3d_autotune(aero3d_solver *s, int iters, int *ty, int *zchunk, float *ms_best) {
+  if (!s || iters < 1) return AERO3D_E_ARG;
+  // candidates: both tile heights, z chunks from short (many blocks) to long (little preamble)
+  const int tys[] = {4, 8};
+  const int zcs[] = {2, 3, 4, 6, 8, 12, 16, 24, 32};
+  // time on a scratch copy of the state: save the current state, restore after
+  const size_t b = sizeof(float) * 5 * s->ns;
+  float *save = nullptr;
+  HIPCHK(s, hipMalloc((void **)&save, b));
+  HIPCHK(s, hipMemcpyAsync(save, buf(s, s->cur), b, hipMemcpyDeviceToDevice, s->stream));
+  const int cur0 = s->cur, it0 = s->it;
+  int bty = s->tile_ty, bzc = s->tile_zc;
+  float best = 1e30f;
+  for (int t : tys)
+    for (int z : zcs) {
+      if (z > s->NZ && z != zcs[0]) continue;
+      s->tile_ty = t; s->tile_zc = z;
+      float ms;
+      int rc = aero3d_time_steps(s, 1, &ms);          // warm
+      if (!rc) rc = aero3d_time_steps(s, iters, &ms);
+      if (rc) { hipFree(save); return rc; }
+      if (ms < best) { best = ms; bty = t; bzc = z; }
+    }
+  s->tile_ty = bty; s->tile_zc = bzc;
+  s->cur = cur0; s->it = it0;
+  HIPCHK(s, hipMemcpyAsync(buf(s, s->cur), save, b, hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  hipFree(save);
+  if (ty) *ty = bty;
+  if (zchunk) *zchunk = bzc;
+  if (ms_best) *ms_best = best / iters;
   return 0;
 }
 

@@ -1,0 +1,371 @@
+// ============================================================================
+// euler3d_tile.hpp — tiled, face-once SSP-RK2 stage of the 3D Euler cut-cell
+// scheme for gfx950 (one wavefront = one 64-cell x row of a tile).
+//
+// Same arithmetic as the reference's per-cell residual (CfdCPU.cellResidual,
+// src/air_cfd.js:198-236) but every face flux is evaluated once and shared by
+// the two cells of the face: the face state L = P_i + s_i/2, R = P_{i+1} -
+// s_{i+1}/2 uses slopes that depend only on the cell they belong to (three
+// cells of that cell's own stencil and its two faces), so the flux of face
+// i+1/2 seen from cell i (sg = +1) and from cell i+1 (sg = -1) is the same
+// number (src/air_cfd.js:213-223).
+//
+// Work decomposition of one block (TX = 64 x TY threads):
+//   x-y tile of TX x TY columns, marching planes k0..k1-1 of a z chunk.
+//   LDS per plane: primitives and flags of the tile plus a 2-cell halo ring,
+//   x / y apertures, x / y slopes, x / y face fluxes.
+//   Registers per column: primitives of planes k-1..k+2, z slopes of k and
+//   k+1, the z flux of face k-1/2 carried from the previous plane.
+// Per plane: load (own plane k+2 into the queue, halo of plane k into LDS),
+// slopes, fluxes, cell update; three barriers.
+// Divisions use the hardware reciprocal and square root (1 ulp).
+// ============================================================================
+#pragma once
+#include "euler3d.hpp"
+
+namespace aero3d {
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// Primitives (rho, u, v, w, p) of conserved a[] (cfdPrim5, src/air_cfd.js:169).
+__device__ __forceinline__ void prim_fast(float a0, float a1, float a2, float a3, float a4, float *o) {
+  const float r = fmaxf(a0, 1e-6f), ir = frcp(r), u = a1 * ir, v = a2 * ir, w = a3 * ir;
+  o[0] = r; o[1] = u; o[2] = v; o[3] = w;
+  o[4] = fmaxf(kGm1 * (a4 - 0.5f * r * (u * u + v * v + w * w)), 1e-7f);
+}
+
+// HLLC flux along axis d (src/air_cfd.js:179-195) with reciprocals.
+template <int d>
+__device__ __forceinline__ void hllc_fast(const float *L, const float *R, float *out) {
+  const float rL = L[0], rR = R[0], pL = L[4], pR = R[4], uL = L[1 + d], uR = R[1 + d];
+  const float irL = frcp(rL), irR = frcp(rR);
+  const float aL = fsqrt(kGamma * pL * irL), aR = fsqrt(kGamma * pR * irR);
+  const float EL = pL * 2.5f + 0.5f * rL * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]);
+  const float ER = pR * 2.5f + 0.5f * rR * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]);
+  const float SL = fminf(uL - aL, uR - aR), SR = fmaxf(uL + aL, uR + aR);
+  if (SL >= 0.f || SR <= 0.f) {
+    const bool left = SL >= 0.f;
+    const float k0 = left ? L[0] : R[0], k1 = left ? L[1] : R[1], k2 = left ? L[2] : R[2], k3 = left ? L[3] : R[3];
+    const float k4 = left ? pL : pR, uK = left ? uL : uR, E = left ? EL : ER;
+    out[0] = k0 * uK; out[1] = k0 * k1 * uK; out[2] = k0 * k2 * uK; out[3] = k0 * k3 * uK;
+    out[1 + d] += k4;
+    out[4] = uK * (E + k4);
+    return;
+  }
+  const float SM = (pR - pL + rL * uL * (SL - uL) - rR * uR * (SR - uR)) * frcp(rL * (SL - uL) - rR * (SR - uR));
+  const bool left = SM >= 0.f;
+  const float k1 = left ? L[1] : R[1], k2 = left ? L[2] : R[2], k3 = left ? L[3] : R[3];
+  const float S = left ? SL : SR, uK = left ? uL : uR, rK = left ? rL : rR, pK = left ? pL : pR, EK = left ? EL : ER;
+  const float irK = left ? irL : irR;
+  const float SmU = S - uK;
+  const float f = rK * SmU * frcp(S - SM);
+  float Fm[4] = {rK * uK, rK * k1 * uK, rK * k2 * uK, rK * k3 * uK};
+  Fm[1 + d] += pK;
+  const float Um[4] = {rK, rK * k1, rK * k2, rK * k3};
+  float Us[4] = {f, f * k1, f * k2, f * k3};
+  Us[1 + d] = f * SM;
+  const float Es = f * (EK * irK + (SM - uK) * (SM + pK * frcp(rK * SmU)));
+#pragma unroll
+  for (int c = 0; c < 4; c++) out[c] = Fm[c] + S * (Us[c] - Um[c]);
+  out[4] = uK * (EK + pK) + S * (Es - EK);
+}
+
+// Face flux of axis d between left cell (PL, TL, slope sL) and right cell
+// (PR, TR, slope sR) with aperture a; zero when the face is closed or touches
+// no fluid cell (no cell would use it).
+template <int d>
+__device__ __forceinline__ void face_flux(const float *PL, const float *sL, int TL, const float *PR, const float *sR, int TR,
+                                          float a, float *F) {
+  if (!(a > 0.f) || (TL != 0 && TR != 0)) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) F[v] = 0.f;
+    return;
+  }
+  float L[5], R[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) { L[v] = PL[v] + 0.5f * sL[v]; R[v] = PR[v] - 0.5f * sR[v]; }
+  if (L[0] <= 0.f || L[4] <= 0.f || R[0] <= 0.f || R[4] <= 0.f) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) { L[v] = PL[v]; R[v] = PR[v]; }
+  }
+  hllc_fast<d>(L, R, F);
+}
+
+// minmod slope of cell c from its neighbours (src/air_cfd.js:213-216):
+// zero unless the three cells are fluid and both faces of c fully open.
+__device__ __forceinline__ void slope5(const float *Pm, const float *P0, const float *Pp, bool ok, float *s) {
+#pragma unroll
+  for (int v = 0; v < 5; v++) s[v] = ok ? mmod(P0[v] - Pm[v], Pp[v] - P0[v]) : 0.f;
+}
+
+template <int TX, int TY>
+struct TileSmem {
+  static constexpr int PW = TX + 4, PH = TY + 4;
+  float P[5][PH][PW];        // primitives, tile + 2-cell halo (corners unused)
+  int8_t T[PH][PW];          // flags: 0..5 cells, -1 freestream ghost, -2 mirror ghost
+  float A0[TY][TX + 3];      // ap0 of cells x = -2..TX (x faces)
+  float A1[TY + 3][TX];      // ap1 of rows y = -2..TY (y faces)
+  float Sx[5][TY][TX + 2];   // x slopes of cells -1..TX
+  float Sy[5][TY + 2][TX];   // y slopes of rows -1..TY
+  float Fx[5][TY][TX + 1];   // x fluxes of faces -1/2..TX-1/2
+  float Fy[5][TY + 1][TX];   // y fluxes of faces -1/2..TY-1/2
+};
+
+// Classify and load the primitives of cell (i, j, k) of one case:
+// freestream ghost outside the box, mirror below y = 0, fixed state for
+// flags >= 3, else the primitives of qin (src/air_cfd.js:205-210).
+__device__ __forceinline__ int load_prim(const CaseView &cv, const StageConsts &kc, const float *__restrict__ qin, size_t ns,
+                                         int i, int j, int k, float *P) {
+  if (i < 0 || i >= cv.NX || j >= cv.NY || k < 0 || k >= cv.NZ) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) P[v] = kc.Fw[v];
+    return -1;
+  }
+  const bool mirror = j < 0;
+  const int jj = mirror ? -1 - j : j;
+  const int q = (k * cv.NY + jj) * cv.NX + i;
+  const int tq = cv.t[q];
+  if (tq >= 3) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) P[v] = kc.fixw[tq - 3][v];
+  } else {
+    prim_fast(qin[q], qin[ns + q], qin[2 * ns + q], qin[3 * ns + q], qin[4 * ns + q], P);
+  }
+  if (mirror) { P[2] = -P[2]; return -2; }
+  return tq;
+}
+
+// One stage over the z chunk [k0, k1) of tile (i0, j0) of case cv.
+// Writes out[] for every cell of the chunk; dtout (if non-null) gets dt.
+template <int TX, int TY>
+__device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts &kc, const float *__restrict__ qin,
+                                           const float *__restrict__ bse, float *__restrict__ out, float *__restrict__ dtout,
+                                           size_t ns, float wa, float wb, int i0, int j0, int k0, int k1,
+                                           TileSmem<TX, TY> &sm) {
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+  const int NX = cv.NX, NY = cv.NY, NZ = cv.NZ, NXY = NX * NY;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool own = i < NX && j < NY;
+  const int qc = own ? j * NX + i : 0;                 // column base index (plane 0)
+  const bool o2 = kc.order == 2;
+  const float ihx = own ? frcp(cv.dx[i]) : 0.f, ihy = own ? frcp(cv.dy[j]) : 0.f;
+  const float *__restrict__ ap0 = cv.ap, *__restrict__ ap1 = cv.ap + cv.N, *__restrict__ ap2 = cv.ap + 2 * (size_t)cv.N,
+                            *__restrict__ ap3 = cv.ap + 3 * (size_t)cv.N;
+  auto apz = [&](int k) -> float { return (k < 0 || k >= NZ || !own) ? 1.f : ap2[k * NXY + qc]; };
+
+  // ---- z queue preamble: planes k0-2 .. k0+1
+  float Pz[4][5];          // primitives of planes k-1, k, k+1, k+2 (own column)
+  int Tz[4];
+  float Pm2[5];
+  // columns outside the box (partial tiles) carry freestream ghosts
+  const int Tm2 = load_prim(cv, kc, qin, ns, own ? i : -1, j, k0 - 2, Pm2);
+#pragma unroll
+  for (int m = 0; m < 3; m++) Tz[m] = load_prim(cv, kc, qin, ns, own ? i : -1, j, k0 - 1 + m, Pz[m]);
+  float Am1 = apz(k0 - 2), A0z = apz(k0 - 1), A1z = apz(k0);   // faces k0-3/2, k0-1/2, k0+1/2
+  float sz0[5], sz1[5], Fzm[5];
+  slope5(Pm2, Pz[0], Pz[1], o2 && Tm2 == 0 && Tz[0] == 0 && Tz[1] == 0 && Am1 == 1.f && A0z == 1.f, sz0);   // s(k0-1)
+  slope5(Pz[0], Pz[1], Pz[2], o2 && Tz[0] == 0 && Tz[1] == 0 && Tz[2] == 0 && A0z == 1.f && A1z == 1.f, sz1); // s(k0)
+  face_flux<2>(Pz[0], sz0, Tz[0], Pz[1], sz1, Tz[1], A0z, Fzm);                                             // F(k0-1/2)
+#pragma unroll
+  for (int v = 0; v < 5; v++) sz0[v] = sz1[v];                                                              // sz0 = s(k)
+  // queue: Pz[0] = k-1, Pz[1] = k, Pz[2] = k+1; Pz[3] loaded each plane
+  float sUpPrev = (own && k0 > 0) ? ap3[(k0 - 1) * NXY + qc] : 0.f;
+
+  for (int k = k0; k < k1; k++) {
+    // ---------------- load phase
+    Tz[3] = load_prim(cv, kc, qin, ns, own ? i : -1, j, k + 2, Pz[3]);
+    const float A2z = apz(k + 1);                   // face k+3/2
+    // tile interior of plane k from the queue
+#pragma unroll
+    for (int v = 0; v < 5; v++) sm.P[v][ty + 2][tx + 2] = Pz[1][v];
+    sm.T[ty + 2][tx + 2] = (int8_t)Tz[1];
+    // halo ring (no corners): 4 rows of TX, 4 columns of TY
+    for (int h = tid; h < 4 * TX + 4 * TY; h += TX * TY) {
+      int hx, hy;
+      if (h < 4 * TX) { const int r = h / TX; hx = h % TX; hy = r < 2 ? r - 2 : TY + r - 2; }
+      else { const int c = (h - 4 * TX) / TY; hy = (h - 4 * TX) % TY; hx = c < 2 ? c - 2 : TX + c - 2; }
+      float P[5];
+      const int T = load_prim(cv, kc, qin, ns, i0 + hx, j0 + hy, k, P);
+#pragma unroll
+      for (int v = 0; v < 5; v++) sm.P[v][hy + 2][hx + 2] = P[v];
+      sm.T[hy + 2][hx + 2] = (int8_t)T;
+    }
+    // apertures: own x / y faces, plus the two faces left of / below the tile
+    sm.A0[ty][tx + 2] = own ? ap0[k * NXY + qc] : 1.f;
+    sm.A1[ty + 2][tx] = own ? ap1[k * NXY + qc] : 1.f;
+    if (tid < 2 * TY) {
+      const int r = tid >> 1, ii = (tid & 1) - 2, gi = i0 + ii, gj = j0 + r;
+      sm.A0[r][ii + 2] = (gi < 0 || gi >= NX || gj >= NY) ? 1.f : ap0[(k * NY + gj) * NX + gi];
+    } else if (tid >= 64 && tid < 64 + 2 * TX) {
+      const int u = tid - 64, c = u >> 1, jj = (u & 1) - 2, gi = i0 + c, gj = j0 + jj;
+      sm.A1[jj + 2][c] = (gj < 0 || gj >= NY || gi >= NX) ? 1.f : ap1[(k * NY + gj) * NX + gi];
+    }
+    __syncthreads();
+
+    // ---------------- slopes: own x and y, then the tile-edge extras
+    {
+      const int xi = tx + 2, yj = ty + 2;
+      float P0[5], Pm[5], Pp[5], s[5];
+#pragma unroll
+      for (int v = 0; v < 5; v++) { P0[v] = sm.P[v][yj][xi]; Pm[v] = sm.P[v][yj][xi - 1]; Pp[v] = sm.P[v][yj][xi + 1]; }
+      bool ok = o2 && sm.T[yj][xi] == 0 && sm.T[yj][xi - 1] == 0 && sm.T[yj][xi + 1] == 0 && sm.A0[ty][tx + 1] == 1.f && sm.A0[ty][tx + 2] == 1.f;
+      slope5(Pm, P0, Pp, ok, s);
+#pragma unroll
+      for (int v = 0; v < 5; v++) sm.Sx[v][ty][tx + 1] = s[v];
+#pragma unroll
+      for (int v = 0; v < 5; v++) { Pm[v] = sm.P[v][yj - 1][xi]; Pp[v] = sm.P[v][yj + 1][xi]; }
+      ok = o2 && sm.T[yj][xi] == 0 && sm.T[yj - 1][xi] == 0 && sm.T[yj + 1][xi] == 0 && sm.A1[ty + 1][tx] == 1.f && sm.A1[ty + 2][tx] == 1.f;
+      slope5(Pm, P0, Pp, ok, s);
+#pragma unroll
+      for (int v = 0; v < 5; v++) sm.Sy[v][ty + 1][tx] = s[v];
+    }
+    if (tid < 2 * TY + 2 * TX) {
+      float P0[5], Pm[5], Pp[5], s[5];
+      if (tid < 2 * TY) {                               // x slope of cell -1 or TX in row r
+        const int r = tid >> 1, cx = (tid & 1) ? TX : -1, xi = cx + 2, yj = r + 2;
+#pragma unroll
+        for (int v = 0; v < 5; v++) { P0[v] = sm.P[v][yj][xi]; Pm[v] = sm.P[v][yj][xi - 1]; Pp[v] = sm.P[v][yj][xi + 1]; }
+        // faces of cell cx: ap0 of cells cx-1 and cx; cell TX's + face lies outside the tile
+        const int gi = i0 + cx, gj = j0 + r;
+        const float am = sm.A0[r][cx + 1];
+        const float apf = (cx == -1) ? sm.A0[r][cx + 2] : ((gi < 0 || gi >= NX || gj >= NY) ? 1.f : ap0[(k * NY + gj) * NX + gi]);
+        const bool ok = o2 && sm.T[yj][xi] == 0 && sm.T[yj][xi - 1] == 0 && sm.T[yj][xi + 1] == 0 && am == 1.f && apf == 1.f;
+        slope5(Pm, P0, Pp, ok, s);
+#pragma unroll
+        for (int v = 0; v < 5; v++) sm.Sx[v][r][cx + 1] = s[v];
+      } else {                                          // y slope of row -1 or TY in column c
+        const int u = tid - 2 * TY, c = u >> 1, ry = (u & 1) ? TY : -1, xi = c + 2, yj = ry + 2;
+#pragma unroll
+        for (int v = 0; v < 5; v++) { P0[v] = sm.P[v][yj][xi]; Pm[v] = sm.P[v][yj - 1][xi]; Pp[v] = sm.P[v][yj + 1][xi]; }
+        const int gi = i0 + c, gj = j0 + ry;
+        const float am = sm.A1[ry + 1][c];
+        const float apf = (ry == -1) ? sm.A1[ry + 2][c] : ((gj < 0 || gj >= NY || gi >= NX) ? 1.f : ap1[(k * NY + gj) * NX + gi]);
+        const bool ok = o2 && sm.T[yj][xi] == 0 && sm.T[yj - 1][xi] == 0 && sm.T[yj + 1][xi] == 0 && am == 1.f && apf == 1.f;
+        slope5(Pm, P0, Pp, ok, s);
+#pragma unroll
+        for (int v = 0; v < 5; v++) sm.Sy[v][ry + 1][c] = s[v];
+      }
+    }
+    // z slope of plane k+1 (own column): cells k, k+1, k+2, faces k+1/2, k+3/2
+    float sz1n[5];
+    slope5(Pz[1], Pz[2], Pz[3], o2 && Tz[1] == 0 && Tz[2] == 0 && Tz[3] == 0 && A1z == 1.f && A2z == 1.f, sz1n);
+    __syncthreads();
+
+    // ---------------- fluxes: own + x, + y faces, tile-edge - faces, own + z face
+    {
+      const int xi = tx + 2, yj = ty + 2;
+      float PL[5], PR[5], sL[5], sR[5], F[5];
+#pragma unroll
+      for (int v = 0; v < 5; v++) { PL[v] = sm.P[v][yj][xi]; PR[v] = sm.P[v][yj][xi + 1]; sL[v] = sm.Sx[v][ty][tx + 1]; sR[v] = sm.Sx[v][ty][tx + 2]; }
+      face_flux<0>(PL, sL, sm.T[yj][xi], PR, sR, sm.T[yj][xi + 1], sm.A0[ty][tx + 2], F);
+#pragma unroll
+      for (int v = 0; v < 5; v++) sm.Fx[v][ty][tx + 1] = F[v];
+#pragma unroll
+      for (int v = 0; v < 5; v++) { PR[v] = sm.P[v][yj + 1][xi]; sL[v] = sm.Sy[v][ty + 1][tx]; sR[v] = sm.Sy[v][ty + 2][tx]; }
+      face_flux<1>(PL, sL, sm.T[yj][xi], PR, sR, sm.T[yj + 1][xi], sm.A1[ty + 2][tx], F);
+#pragma unroll
+      for (int v = 0; v < 5; v++) sm.Fy[v][ty + 1][tx] = F[v];
+    }
+    if (tid < TX) {                                    // y face below row 0, column tid
+      const int c = tid, xi = c + 2;
+      float PL[5], PR[5], sL[5], sR[5], F[5];
+#pragma unroll
+      for (int v = 0; v < 5; v++) { PL[v] = sm.P[v][1][xi]; PR[v] = sm.P[v][2][xi]; sL[v] = sm.Sy[v][0][c]; sR[v] = sm.Sy[v][1][c]; }
+      face_flux<1>(PL, sL, sm.T[1][xi], PR, sR, sm.T[2][xi], sm.A1[1][c], F);
+#pragma unroll
+      for (int v = 0; v < 5; v++) sm.Fy[v][0][c] = F[v];
+    } else if (tid >= TX && tid < TX + TY) {           // x face left of column 0, row r
+      const int r = tid - TX, yj = r + 2;
+      float PL[5], PR[5], sL[5], sR[5], F[5];
+#pragma unroll
+      for (int v = 0; v < 5; v++) { PL[v] = sm.P[v][yj][1]; PR[v] = sm.P[v][yj][2]; sL[v] = sm.Sx[v][r][0]; sR[v] = sm.Sx[v][r][1]; }
+      face_flux<0>(PL, sL, sm.T[yj][1], PR, sR, sm.T[yj][2], sm.A0[r][1], F);
+#pragma unroll
+      for (int v = 0; v < 5; v++) sm.Fx[v][r][0] = F[v];
+    }
+    float Fzp[5];
+    face_flux<2>(Pz[1], sz0, Tz[1], Pz[2], sz1n, Tz[2], A1z, Fzp);
+    // own apertures into registers (LDS apertures are rewritten by the next plane's load)
+    const float ax_m = sm.A0[ty][tx + 1], ax_p = sm.A0[ty][tx + 2], ay_m = sm.A1[ty + 1][tx], ay_p = sm.A1[ty + 2][tx];
+    __syncthreads();
+
+    // ---------------- cell update
+    if (own) {
+      const int q = k * NXY + qc;
+      const int tq = Tz[1];
+      if (tq != 0) {
+        if (tq >= 3) {
+#pragma unroll
+          for (int c = 0; c < 5; c++) out[c * ns + q] = kc.fixq[tq - 3][c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 5; c++) out[c * ns + q] = bse[c * ns + q];
+        }
+        if (dtout) dtout[q] = 0.f;
+      } else {
+        const float ihz = frcp(cv.dz[k]);
+        const float *C = Pz[1];
+        const float pC = C[4];
+        float R[5];
+        // x: + face, - face, wall pressure
+#pragma unroll
+        for (int c = 0; c < 5; c++) R[c] = -(ax_p * sm.Fx[c][ty][tx + 1]) * ihx;
+#pragma unroll
+        for (int c = 0; c < 5; c++) R[c] += (ax_m * sm.Fx[c][ty][tx]) * ihx;
+        R[1] += pC * (ax_p - ax_m) * ihx;
+        // y
+#pragma unroll
+        for (int c = 0; c < 5; c++) R[c] -= (ay_p * sm.Fy[c][ty + 1][tx]) * ihy;
+#pragma unroll
+        for (int c = 0; c < 5; c++) R[c] += (ay_m * sm.Fy[c][ty][tx]) * ihy;
+        R[2] += pC * (ay_p - ay_m) * ihy;
+        // z
+#pragma unroll
+        for (int c = 0; c < 5; c++) R[c] -= (A1z * Fzp[c]) * ihz;
+#pragma unroll
+        for (int c = 0; c < 5; c++) R[c] += (A0z * Fzm[c]) * ihz;
+        R[3] += pC * (A1z - A0z) * ihz;
+        // baffle transpiration (src/air_cfd.js:226-233)
+        const float sUp = ap3[q], sDn = sUpPrev;
+        if (sUp != 0.f || sDn != 0.f) {
+          const float r = C[0], u = C[1], v = C[2], w = C[3];
+          const float Hh = (pC * 2.5f + 0.5f * r * (u * u + v * v + w * w) + pC) * frcp(r);
+          const float m = (sDn != 0.f ? (1.f - A0z) * r * u * sDn : 0.f) - (sUp != 0.f ? (1.f - A1z) * r * u * sUp : 0.f);
+          const float mh = m * ihz;
+          R[0] += mh; R[1] += mh * u; R[2] += mh * v; R[3] += mh * w; R[4] += mh * Hh;
+        }
+        const float ph = cv.phi[q], iph = frcp(ph);
+        float Qc[5], Bq[5], Bw[5];
+#pragma unroll
+        for (int c = 0; c < 5; c++) { Qc[c] = qin[c * ns + q]; Bq[c] = bse[c * ns + q]; }
+        prim_fast(Bq[0], Bq[1], Bq[2], Bq[3], Bq[4], Bw);
+        const float a = fsqrt(kGamma * Bw[4] * frcp(Bw[0]));
+        const float dt = kc.cfl * fmaxf(ph, 0.25f) *
+                         frcp((fabsf(Bw[1]) + a) * ihx + (fabsf(Bw[2]) + a) * ihy + (fabsf(Bw[3]) + a) * ihz);
+        const float dtp = dt * iph;
+        float n[5];
+#pragma unroll
+        for (int c = 0; c < 5; c++) n[c] = wa * Bq[c] + wb * (Qc[c] + dtp * R[c]);
+        n[0] = fmaxf(n[0], 1e-4f);
+        const float ke = 0.5f * (n[1] * n[1] + n[2] * n[2] + n[3] * n[3]) * frcp(n[0]);
+        if (kGm1 * (n[4] - ke) < 1e-5f) n[4] = 2.5e-5f + ke;
+#pragma unroll
+        for (int c = 0; c < 5; c++) out[c * ns + q] = n[c];
+        if (dtout) dtout[q] = dt;
+      }
+      sUpPrev = ap3[q];
+    }
+    // ---------------- shift the z queue
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      Pz[0][v] = Pz[1][v]; Pz[1][v] = Pz[2][v]; Pz[2][v] = Pz[3][v];
+      sz0[v] = sz1n[v]; Fzm[v] = Fzp[v];
+    }
+    Tz[0] = Tz[1]; Tz[1] = Tz[2]; Tz[2] = Tz[3];
+    A0z = A1z; A1z = A2z;
+    (void)Am1;
+  }
+}
+
+}  // namespace aero3d

@@ -40,17 +40,33 @@ def cpu_baseline(grid, geo, F, fix, budget_s=15.0):
             "sample": f"{n} SSP-RK2 iterations of the same 1.0M-cell transport (after 1 untimed), OpenMP over cells"}
 
 
+def copy_probe(nbytes=1 << 30):
+    """A plain device-to-device copy of `nbytes` (reads nbytes, writes nbytes)."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--tile", default="auto", help="stage-kernel shape 'ty,zchunk' or 'auto' (autotuner)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain device copy of known size (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only; the solve would shard by z-slabs with a 2-cell halo "
               "(see DESIGN.md) but no multi-GPU path is built", file=sys.stderr)
         sys.exit(2)
+    if a.copy_probe:
+        return copy_probe()
     import geometry
     import aero3d
     grid, geo, fl, meta = geometry.build_case(WORKLOAD)
@@ -64,6 +80,11 @@ def main():
     s.set_free(F); s.set_fix(fix); s.reset()
     s.set_force_list(fl)
     print(f"[bench] grid {meta['NX']}x{meta['NY']}x{meta['NZ']} = {meta['N']} cells, warmup {a.warmup}", file=sys.stderr, flush=True)
+    if a.tile == "auto":
+        ty, zc, _ = s.autotune(10)
+    else:
+        ty, zc = (int(x) for x in a.tile.split(","))
+        s.set_tile(ty, zc)
     s.step(a.warmup)
     s.sync()
     t0 = time.perf_counter()
@@ -76,14 +97,22 @@ def main():
     achieved = BYTES_PER_CELL_UPDATE / 2 * N / launch_s / 1e9
     r = meta["ref"]
     f = s.forces(alpha, M, r["S"], r["xcg"], 0, r["c"])
+    traffic = None
+    try:     # HBM bytes per launch measured by rocprofv3 counters for this kernel shape (profiles/pmc_traffic.json)
+        t = json.load(open(os.path.join(ROOT, "profiles", "pmc_traffic.json")))["plain_transport_1m"]
+        if t["tile"] == [64, ty, zc]:
+            traffic = t["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        pass
     out = {
         "metric": "cell-updates/s", "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms / a.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "plain 3D Euler, transport body, 1.0M-cell stretched half domain, M0.78, alpha 2, CFL 0.5",
-                   "cells": N, "grid": [meta["NX"], meta["NY"], meta["NZ"]], "parallelism": "single GPU"},
+                   "cells": N, "grid": [meta["NX"], meta["NY"], meta["NZ"]], "parallelism": "single GPU",
+                   "tile": [64, ty, zc]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "wall_s": wall, "it": s.it, "CL": f["CL"], "CD": f["CD"],
     }
     if not a.no_cpu:

@@ -1,0 +1,26 @@
+// Test-only stand-in for <hip/hip_runtime.h>: lets g++ compile the device
+// headers of 3d_aero_lab_amd/csrc as host code so that a CPU run (one OS thread
+// per GPU thread, a barrier per __syncthreads) can check indexing under
+// AddressSanitizer. Never used by the product build.
+#pragma once
+#include <barrier>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+
+#define __device__
+#define __host__
+#define __global__
+#define __forceinline__ inline
+#define __launch_bounds__(x)
+#define __shared__
+
+struct emul_dim3 { int x = 0, y = 0, z = 0; };
+extern thread_local emul_dim3 threadIdx;
+extern thread_local emul_dim3 blockIdx;
+extern thread_local std::barrier<> *emul_block_barrier;
+inline void __syncthreads() { emul_block_barrier->arrive_and_wait(); }
+inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
+inline float __builtin_amdgcn_sqrtf(float x) { return std::sqrt(x); }
+#include <math.h>
+inline int min(int a, int b) { return a < b ? a : b; }

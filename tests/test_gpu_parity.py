@@ -138,3 +138,33 @@ def test_bad_option_is_an_error():
     s.opt["order"] = 3
     with pytest.raises(aero3d.Aero3DError, match="order"):
         s.step(1)
+
+
+@pytest.mark.parametrize("ty,zc", [(0, 1), (4, 3), (4, 75), (8, 1), (8, 8), (8, 80)])
+def test_kernel_shapes_agree(ty, zc):
+    """Every stage-kernel shape (per-cell kernel, tiles of 64 x 4 / 64 x 8 with
+    short to whole-domain z chunks) reproduces the oracle."""
+    g = Golden("transport60k")
+    F, fix = g.s["F"], g.s["fix"]
+    say("shape", ty, zc)
+    s = make_hip(g, F, fix, 0.5)
+    s.set_tile(ty, zc)
+    s.step(2)
+    d = rel_diff(s.read(), g.state("it2"), g.geo()["t"])
+    say("shape", ty, zc, d)
+    assert max(d.values()) < STATE_TOL, d
+
+
+def test_autotune_keeps_state():
+    g = Golden("sphere")
+    F = g.s["F"]
+    s = make_hip(g, F, [F, F, F], 0.5)
+    s.step(10)
+    Q = s.read()
+    ty, zc, ms = s.autotune(3)
+    say("autotune sphere", ty, zc, ms)
+    assert ty in (4, 8) and zc >= 1 and ms > 0
+    assert s.it == 10 and np.array_equal(s.read(), Q)
+    s.step(190)
+    d = rel_diff(s.read(), g.state("it200"), g.geo()["t"])
+    assert max(d.values()) < STATE_TOL, d
