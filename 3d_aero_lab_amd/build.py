@@ -47,7 +47,7 @@ def build_addon():
     if not os.path.exists(os.path.join(inc, "node_api.h")):
         return None
     tmp = ADDON + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", inc, "-I", os.path.join(ROOT, "include"),
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-DNODE_GYP_MODULE_NAME=aero3d", "-I", inc, "-I", os.path.join(ROOT, "include"),
           "-o", tmp, os.path.join(CSRC, "aero3d_napi.cc"), "-L", LIB, "-laero3d", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, ADDON)
     return ADDON

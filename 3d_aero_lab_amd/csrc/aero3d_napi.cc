@@ -1,0 +1,302 @@
+// ============================================================================
+// aero3d_napi.cc — Node-API addon (aero3d.node) over the C ABI of
+// libaero3d.so. Plain Node-API (N-API 8, no node-addon-api), typed arrays
+// passed by pointer, every non-zero return code thrown as a JS Error with the
+// library's message. The JS solver object CfdHIP (js/hip_solver.js) wraps it.
+// ============================================================================
+#include <node_api.h>
+
+#include <string>
+#include <vector>
+
+#include "aero3d.h"
+
+namespace {
+
+#define NAPI_OK(call)                                                        \
+  do {                                                                       \
+    if ((call) != napi_ok) {                                                 \
+      napi_throw_error(env, nullptr, "aero3d addon: Node-API call failed");  \
+      return nullptr;                                                        \
+    }                                                                        \
+  } while (0)
+
+napi_value throw_rc(napi_env env, aero3d_solver *s, int rc) {
+  std::string msg = "libaero3d error " + std::to_string(rc) + ": " + aero3d_last_error(s);
+  napi_throw_error(env, nullptr, msg.c_str());
+  return nullptr;
+}
+
+bool args(napi_env env, napi_callback_info info, size_t want, napi_value *argv) {
+  size_t argc = want;
+  if (napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr) != napi_ok || argc < want) {
+    napi_throw_type_error(env, nullptr, "aero3d addon: missing arguments");
+    return false;
+  }
+  return true;
+}
+
+// Typed-array data pointer and element count; type-checked.
+template <class T>
+bool typed(napi_env env, napi_value v, napi_typedarray_type want, T **data, size_t *n) {
+  bool is = false;
+  napi_is_typedarray(env, v, &is);
+  napi_typedarray_type ty;
+  void *p = nullptr;
+  napi_value ab;
+  size_t off = 0, len = 0;
+  if (!is || napi_get_typedarray_info(env, v, &ty, &len, &p, &ab, &off) != napi_ok || ty != want) {
+    napi_throw_type_error(env, nullptr, "aero3d addon: wrong typed-array type");
+    return false;
+  }
+  *data = static_cast<T *>(p);
+  if (n) *n = len;
+  return true;
+}
+
+aero3d_solver *handle(napi_env env, napi_value v) {
+  void *p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, nullptr, "aero3d addon: not a solver handle (disposed?)");
+    return nullptr;
+  }
+  return static_cast<aero3d_solver *>(p);
+}
+
+double num(napi_env env, napi_value v) { double d = 0; napi_get_value_double(env, v, &d); return d; }
+
+napi_value undef(napi_env env) { napi_value u; napi_get_undefined(env, &u); return u; }
+
+// create(fx, fy, fz, t, phi, ap0, ap1, ap2, ap3) -> external handle
+napi_value Create(napi_env env, napi_callback_info info) {
+  napi_value a[9];
+  if (!args(env, info, 9, a)) return nullptr;
+  double *fx, *fy, *fz; size_t nx, ny, nz, n;
+  uint8_t *t; float *phi, *ap[4];
+  if (!typed(env, a[0], napi_float64_array, &fx, &nx) || !typed(env, a[1], napi_float64_array, &fy, &ny) ||
+      !typed(env, a[2], napi_float64_array, &fz, &nz) || !typed(env, a[3], napi_uint8_array, &t, &n) ||
+      !typed(env, a[4], napi_float32_array, &phi, nullptr))
+    return nullptr;
+  for (int d = 0; d < 4; d++) if (!typed(env, a[5 + d], napi_float32_array, &ap[d], nullptr)) return nullptr;
+  if (nx < 2 || ny < 2 || nz < 2 || n != (nx - 1) * (ny - 1) * (nz - 1)) {
+    napi_throw_range_error(env, nullptr, "aero3d addon: mask length does not match the grid");
+    return nullptr;
+  }
+  aero3d_grid g{(int)nx - 1, (int)ny - 1, (int)nz - 1, fx, fy, fz};
+  aero3d_solver *s = nullptr;
+  int rc = aero3d_create(&g, t, phi, ap[0], ap[1], ap[2], ap[3], &s);
+  if (rc) return throw_rc(env, nullptr, rc);
+  napi_value ext;
+  NAPI_OK(napi_create_external(env, s, nullptr, nullptr, &ext));
+  return ext;
+}
+
+napi_value Dispose(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!args(env, info, 1, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  if (!s) return nullptr;
+  aero3d_destroy(s);
+  return undef(env);
+}
+
+napi_value SetFree(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!args(env, info, 2, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  double *F; size_t n;
+  if (!s || !typed(env, a[1], napi_float64_array, &F, &n)) return nullptr;
+  if (n != 5) { napi_throw_range_error(env, nullptr, "setFree: 5 values"); return nullptr; }
+  int rc = aero3d_set_free(s, F);
+  return rc ? throw_rc(env, s, rc) : undef(env);
+}
+
+napi_value SetFix(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!args(env, info, 2, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  double *F; size_t n;
+  if (!s || !typed(env, a[1], napi_float64_array, &F, &n)) return nullptr;
+  if (n != 15) { napi_throw_range_error(env, nullptr, "setFix: 3 x 5 values"); return nullptr; }
+  int rc = aero3d_set_fix(s, F);
+  return rc ? throw_rc(env, s, rc) : undef(env);
+}
+
+napi_value SetOpt(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!args(env, info, 3, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  if (!s) return nullptr;
+  int rc = aero3d_set_opt(s, num(env, a[1]), (int)num(env, a[2]));
+  return rc ? throw_rc(env, s, rc) : undef(env);
+}
+
+napi_value Reset(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!args(env, info, 1, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  if (!s) return nullptr;
+  int rc = aero3d_reset(s);
+  return rc ? throw_rc(env, s, rc) : undef(env);
+}
+
+napi_value Step(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!args(env, info, 2, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  if (!s) return nullptr;
+  int rc = aero3d_step(s, (int)num(env, a[1]));
+  return rc ? throw_rc(env, s, rc) : undef(env);
+}
+
+napi_value Iter(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!args(env, info, 1, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  if (!s) return nullptr;
+  int it = 0;
+  int rc = aero3d_iter(s, &it);
+  if (rc) return throw_rc(env, s, rc);
+  napi_value v;
+  NAPI_OK(napi_create_int32(env, it, &v));
+  return v;
+}
+
+// read(h, r, ru, rv, rw, E, dt|null): fills caller Float32Arrays
+napi_value Read(napi_env env, napi_callback_info info) {
+  napi_value a[7];
+  if (!args(env, info, 7, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  if (!s) return nullptr;
+  float *p[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  for (int c = 0; c < 6; c++) {
+    napi_valuetype vt;
+    napi_typeof(env, a[1 + c], &vt);
+    if (vt == napi_null || vt == napi_undefined) continue;
+    if (!typed(env, a[1 + c], napi_float32_array, &p[c], nullptr)) return nullptr;
+  }
+  int rc = aero3d_read(s, p[0], p[1], p[2], p[3], p[4], p[5]);
+  return rc ? throw_rc(env, s, rc) : undef(env);
+}
+
+napi_value Write(napi_env env, napi_callback_info info) {
+  napi_value a[6];
+  if (!args(env, info, 6, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  if (!s) return nullptr;
+  float *p[5];
+  for (int c = 0; c < 5; c++) if (!typed(env, a[1 + c], napi_float32_array, &p[c], nullptr)) return nullptr;
+  int rc = aero3d_write(s, p[0], p[1], p[2], p[3], p[4]);
+  return rc ? throw_rc(env, s, rc) : undef(env);
+}
+
+napi_value Sync(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!args(env, info, 1, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  if (!s) return nullptr;
+  int rc = aero3d_sync(s);
+  return rc ? throw_rc(env, s, rc) : undef(env);
+}
+
+// setForceList(h, q Int32Array, cx, cz, x, z Float64Array)
+napi_value SetForceList(napi_env env, napi_callback_info info) {
+  napi_value a[6];
+  if (!args(env, info, 6, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  int32_t *q; size_t n; double *d[4];
+  if (!s || !typed(env, a[1], napi_int32_array, &q, &n)) return nullptr;
+  for (int c = 0; c < 4; c++) if (!typed(env, a[2 + c], napi_float64_array, &d[c], nullptr)) return nullptr;
+  int rc = aero3d_set_force_list(s, q, d[0], d[1], d[2], d[3], (int)n);
+  return rc ? throw_rc(env, s, rc) : undef(env);
+}
+
+// forces(h, alpha, M, S, xcg, zcg, cref) -> {CL, CD, Cm}
+napi_value Forces(napi_env env, napi_callback_info info) {
+  napi_value a[7];
+  if (!args(env, info, 7, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  if (!s) return nullptr;
+  double out[3];
+  int rc = aero3d_forces(s, num(env, a[1]), num(env, a[2]), num(env, a[3]), num(env, a[4]), num(env, a[5]), num(env, a[6]), out);
+  if (rc) return throw_rc(env, s, rc);
+  napi_value o, v;
+  NAPI_OK(napi_create_object(env, &o));
+  const char *k[3] = {"CL", "CD", "Cm"};
+  for (int c = 0; c < 3; c++) { NAPI_OK(napi_create_double(env, out[c], &v)); NAPI_OK(napi_set_named_property(env, o, k[c], v)); }
+  return o;
+}
+
+napi_value Trefftz(napi_env env, napi_callback_info info) {
+  napi_value a[5];
+  if (!args(env, info, 5, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  if (!s) return nullptr;
+  double cdi;
+  int rc = aero3d_trefftz(s, num(env, a[1]), num(env, a[2]), num(env, a[3]), num(env, a[4]), &cdi);
+  if (rc) return throw_rc(env, s, rc);
+  napi_value v;
+  NAPI_OK(napi_create_double(env, cdi, &v));
+  return v;
+}
+
+napi_value Residual(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!args(env, info, 1, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  if (!s) return nullptr;
+  double r;
+  int rc = aero3d_residual(s, &r);
+  if (rc) return throw_rc(env, s, rc);
+  napi_value v;
+  NAPI_OK(napi_create_double(env, r, &v));
+  return v;
+}
+
+napi_value TimeSteps(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!args(env, info, 2, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  if (!s) return nullptr;
+  float ms;
+  int rc = aero3d_time_steps(s, (int)num(env, a[1]), &ms);
+  if (rc) return throw_rc(env, s, rc);
+  napi_value v;
+  NAPI_OK(napi_create_double(env, ms, &v));
+  return v;
+}
+
+napi_value Autotune(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!args(env, info, 2, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  if (!s) return nullptr;
+  int ty, zc;
+  float ms;
+  int rc = aero3d_autotune(s, (int)num(env, a[1]), &ty, &zc, &ms);
+  if (rc) return throw_rc(env, s, rc);
+  napi_value o, v;
+  NAPI_OK(napi_create_object(env, &o));
+  NAPI_OK(napi_create_int32(env, ty, &v)); NAPI_OK(napi_set_named_property(env, o, "ty", v));
+  NAPI_OK(napi_create_int32(env, zc, &v)); NAPI_OK(napi_set_named_property(env, o, "zchunk", v));
+  NAPI_OK(napi_create_double(env, ms, &v)); NAPI_OK(napi_set_named_property(env, o, "msPerIter", v));
+  return o;
+}
+
+napi_value Init(napi_env env, napi_value exports) {
+  struct { const char *name; napi_callback cb; } fns[] = {
+      {"create", Create}, {"dispose", Dispose}, {"setFree", SetFree}, {"setFix", SetFix}, {"setOpt", SetOpt},
+      {"reset", Reset}, {"step", Step}, {"iter", Iter}, {"read", Read}, {"write", Write}, {"sync", Sync},
+      {"setForceList", SetForceList}, {"forces", Forces}, {"trefftz", Trefftz}, {"residual", Residual},
+      {"timeSteps", TimeSteps}, {"autotune", Autotune}};
+  for (auto &f : fns) {
+    napi_value fn;
+    NAPI_OK(napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));
+    NAPI_OK(napi_set_named_property(env, exports, f.name, fn));
+  }
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)
