@@ -194,3 +194,223 @@ class CfdHIP:
         v = C.c_int64()
         self._chk(lib().aero3d_footprint(self.h, C.byref(v)))
         return v.value
+
+
+# ---------------------------------------------------------------------------
+# batched and served forms
+# ---------------------------------------------------------------------------
+class Request(C.Structure):
+    _fields_ = [("geometry", C.c_int), ("F", C.c_double * 5), ("fix", C.c_double * 15), ("cfl", C.c_double),
+                ("order", C.c_int), ("alpha", C.c_double), ("M", C.c_double), ("S", C.c_double), ("xcg", C.c_double),
+                ("zcg", C.c_double), ("cref", C.c_double), ("res_drop", C.c_double), ("max_iters", C.c_int),
+                ("arrival", C.c_double), ("tag", C.c_int64)]
+
+
+class Result(C.Structure):
+    _fields_ = [("tag", C.c_int64), ("iters", C.c_int), ("converged", C.c_int), ("res0", C.c_double), ("res", C.c_double),
+                ("CL", C.c_double), ("CD", C.c_double), ("Cm", C.c_double), ("arrival", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+BATCH_EXPORTS = ["aero3d_batch_create", "aero3d_batch_destroy", "aero3d_batch_last_error", "aero3d_batch_add_geometry",
+                 "aero3d_batch_remove_geometry", "aero3d_batch_case_bytes", "aero3d_batch_add", "aero3d_batch_remove",
+                 "aero3d_batch_set_zchunk", "aero3d_batch_step", "aero3d_batch_sync", "aero3d_batch_iter", "aero3d_batch_read",
+                 "aero3d_batch_residuals", "aero3d_batch_forces", "aero3d_batch_footprint", "aero3d_batch_time_steps",
+                 "aero3d_server_create", "aero3d_server_destroy", "aero3d_server_batch", "aero3d_server_clock",
+                 "aero3d_server_submit", "aero3d_server_tick", "aero3d_server_poll", "aero3d_server_stats"]
+EXPORTS = EXPORTS + BATCH_EXPORTS
+
+_blib = None
+
+
+def blib():
+    global _blib
+    if _blib is None:
+        L = lib()
+        vp, ci, cd, pi, pd = C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_double)
+        p64 = C.POINTER(C.c_int64)
+        L.aero3d_batch_create.argtypes = [C.c_int64, ci, C.POINTER(vp)]
+        L.aero3d_batch_destroy.argtypes = [vp]
+        L.aero3d_batch_last_error.argtypes = [vp]
+        L.aero3d_batch_last_error.restype = C.c_char_p
+        L.aero3d_batch_add_geometry.argtypes = [vp, C.POINTER(Grid), u8p, f32p, f32p, f32p, f32p, f32p, i32p, f64p, f64p, f64p, f64p, ci, pi]
+        L.aero3d_batch_remove_geometry.argtypes = [vp, ci]
+        L.aero3d_batch_case_bytes.argtypes = [vp, ci, p64]
+        L.aero3d_batch_add.argtypes = [vp, ci, f64p, f64p, cd, ci, pi]
+        L.aero3d_batch_remove.argtypes = [vp, ci]
+        L.aero3d_batch_set_zchunk.argtypes = [vp, ci]
+        L.aero3d_batch_step.argtypes = [vp, ci]
+        L.aero3d_batch_sync.argtypes = [vp]
+        L.aero3d_batch_iter.argtypes = [vp, ci, pi]
+        L.aero3d_batch_read.argtypes = [vp, ci] + [f32p] * 5
+        L.aero3d_batch_residuals.argtypes = [vp, i32p, ci, f64p]
+        L.aero3d_batch_forces.argtypes = [vp, ci] + [cd] * 6 + [f64p]
+        L.aero3d_batch_footprint.argtypes = [vp, p64, p64]
+        L.aero3d_batch_time_steps.argtypes = [vp, ci, C.POINTER(C.c_float)]
+        L.aero3d_server_create.argtypes = [C.c_int64, ci, ci, C.POINTER(vp)]
+        L.aero3d_server_destroy.argtypes = [vp]
+        L.aero3d_server_batch.argtypes = [vp]
+        L.aero3d_server_batch.restype = vp
+        L.aero3d_server_clock.argtypes = [vp, pd]
+        L.aero3d_server_submit.argtypes = [vp, C.POINTER(Request)]
+        L.aero3d_server_tick.argtypes = [vp, pi, pi]
+        L.aero3d_server_poll.argtypes = [vp, C.POINTER(Result), ci, pi]
+        L.aero3d_server_stats.argtypes = [vp, pd, p64]
+        _blib = L
+    return _blib
+
+
+def _grid_struct(grid):
+    fx, fy, fz = (np.ascontiguousarray(grid[k], dtype=np.float64) for k in ("fx", "fy", "fz"))
+    g = Grid(len(fx) - 1, len(fy) - 1, len(fz) - 1, fx.ctypes.data_as(C.POINTER(C.c_double)),
+             fy.ctypes.data_as(C.POINTER(C.c_double)), fz.ctypes.data_as(C.POINTER(C.c_double)))
+    return g, (fx, fy, fz)
+
+
+class Batch:
+    """Many solves stepped together (aero3d_batch_*)."""
+
+    def __init__(self, arena_bytes=8 << 30, max_slots=256, handle=None):
+        L = blib()
+        if handle is not None:
+            self.h, self._own = handle, False
+        else:
+            h = C.c_void_p()
+            rc = L.aero3d_batch_create(int(arena_bytes), max_slots, C.byref(h))
+            if rc:
+                raise Aero3DError(f"aero3d_batch_create failed ({rc})")
+            self.h, self._own = h, True
+        self.N = {}
+
+    def _chk(self, rc):
+        if rc:
+            raise Aero3DError(f"libaero3d batch error {rc}: {blib().aero3d_batch_last_error(self.h).decode()}")
+
+    def dispose(self):
+        if getattr(self, "h", None) and self._own:
+            blib().aero3d_batch_destroy(self.h)
+        self.h = None
+
+    __del__ = dispose
+
+    def add_geometry(self, grid, geo, fl=None):
+        g, keep = _grid_struct(grid)
+        a = lambda x, dt: np.ascontiguousarray(x, dtype=dt)
+        if fl is None:
+            fl = {"q": np.zeros(0, np.int32), **{k: np.zeros(0) for k in ("cx", "cz", "x", "z")}}
+        gid = C.c_int()
+        self._chk(blib().aero3d_batch_add_geometry(self.h, C.byref(g), a(geo["t"], np.uint8), a(geo["phi"], np.float32),
+                                                   *[a(geo["ap"][d], np.float32) for d in range(4)], a(fl["q"], np.int32),
+                                                   *[a(fl[k], np.float64) for k in ("cx", "cz", "x", "z")], len(fl["q"]), C.byref(gid)))
+        self.N[gid.value] = g.NX * g.NY * g.NZ
+        return gid.value
+
+    def case_bytes(self, gid):
+        v = C.c_int64()
+        self._chk(blib().aero3d_batch_case_bytes(self.h, gid, C.byref(v)))
+        return v.value
+
+    def add(self, gid, F, fix, cfl, order=2):
+        s = C.c_int()
+        self._chk(blib().aero3d_batch_add(self.h, gid, np.asarray(F, np.float64), np.ascontiguousarray(np.asarray(fix, np.float64).reshape(15)),
+                                          cfl, order, C.byref(s)))
+        self.N[("slot", s.value)] = self.N[gid]
+        return s.value
+
+    def remove(self, slot):
+        self._chk(blib().aero3d_batch_remove(self.h, slot))
+
+    def set_zchunk(self, zc):
+        self._chk(blib().aero3d_batch_set_zchunk(self.h, zc))
+
+    def step(self, n):
+        self._chk(blib().aero3d_batch_step(self.h, n))
+
+    def sync(self):
+        self._chk(blib().aero3d_batch_sync(self.h))
+
+    def iter(self, slot):
+        v = C.c_int()
+        self._chk(blib().aero3d_batch_iter(self.h, slot, C.byref(v)))
+        return v.value
+
+    def read(self, slot):
+        N = self.N[("slot", slot)]
+        out = [np.empty(N, np.float32) for _ in range(5)]
+        self._chk(blib().aero3d_batch_read(self.h, slot, *out))
+        return np.stack(out)
+
+    def residuals(self, slots):
+        sl = np.ascontiguousarray(slots, np.int32)
+        out = np.zeros(len(sl), np.float64)
+        self._chk(blib().aero3d_batch_residuals(self.h, sl, len(sl), out))
+        return out
+
+    def forces(self, slot, alpha, M, S, xcg, zcg, cref):
+        out = np.zeros(3, np.float64)
+        self._chk(blib().aero3d_batch_forces(self.h, slot, alpha, M, S, xcg, zcg, cref, out))
+        return {"CL": out[0], "CD": out[1], "Cm": out[2]}
+
+    def footprint(self):
+        u, c = C.c_int64(), C.c_int64()
+        self._chk(blib().aero3d_batch_footprint(self.h, C.byref(u), C.byref(c)))
+        return u.value, c.value
+
+    def time_steps(self, n):
+        ms = C.c_float()
+        self._chk(blib().aero3d_batch_time_steps(self.h, n, C.byref(ms)))
+        return ms.value
+
+
+class Server:
+    """Continuous-batching server of solve requests (aero3d_server_*)."""
+
+    def __init__(self, arena_bytes=8 << 30, max_slots=256, check_every=50):
+        h = C.c_void_p()
+        rc = blib().aero3d_server_create(int(arena_bytes), max_slots, check_every, C.byref(h))
+        if rc:
+            raise Aero3DError(f"aero3d_server_create failed ({rc})")
+        self.h = h
+        self.batch = Batch(handle=C.c_void_p(blib().aero3d_server_batch(h)))
+
+    def dispose(self):
+        if getattr(self, "h", None):
+            blib().aero3d_server_destroy(self.h)
+            self.h = None
+
+    __del__ = dispose
+
+    def _chk(self, rc):
+        if rc:
+            raise Aero3DError(f"libaero3d server error {rc}: {blib().aero3d_batch_last_error(self.batch.h).decode()}")
+
+    def clock(self):
+        v = C.c_double()
+        self._chk(blib().aero3d_server_clock(self.h, C.byref(v)))
+        return v.value
+
+    def submit(self, gid, F, fix, cfl, order, force_ref, res_drop, max_iters, arrival, tag):
+        r = Request()
+        r.geometry = gid
+        r.F[:] = list(F)
+        r.fix[:] = list(np.asarray(fix, np.float64).reshape(15))
+        r.cfl, r.order = cfl, order
+        r.alpha, r.M, r.S, r.xcg, r.zcg, r.cref = force_ref
+        r.res_drop, r.max_iters, r.arrival, r.tag = res_drop, max_iters, arrival, tag
+        self._chk(blib().aero3d_server_submit(self.h, C.byref(r)))
+
+    def tick(self):
+        a, p = C.c_int(), C.c_int()
+        self._chk(blib().aero3d_server_tick(self.h, C.byref(a), C.byref(p)))
+        return a.value, p.value
+
+    def poll(self, max_n=1024):
+        buf = (Result * max_n)()
+        n = C.c_int()
+        self._chk(blib().aero3d_server_poll(self.h, buf, max_n, C.byref(n)))
+        return [{f: getattr(buf[i], f) for f, _ in Result._fields_} for i in range(n.value)]
+
+    def stats(self):
+        b, c = C.c_double(), C.c_int64()
+        self._chk(blib().aero3d_server_stats(self.h, C.byref(b), C.byref(c)))
+        return b.value, c.value

@@ -44,3 +44,27 @@ def load_dir(d):
     fl = {"q": rd("fl_q", np.int32), "cx": rd("fl_cx", np.float64), "cz": rd("fl_cz", np.float64),
           "x": rd("fl_x", np.float64), "z": rd("fl_z", np.float64)}
     return grid, geo, fl, meta
+
+
+def build_cases(specs, timeout=1800):
+    """Build several cases with one node process -> list of (grid, geo, force_list, meta)."""
+    d = tempfile.mkdtemp(prefix="aero3d_geoms_")
+    try:
+        subprocess.run(["node", CLI, "geom-many", d, *specs], check=True, capture_output=True, timeout=timeout)
+        return [load_dir(os.path.join(d, str(k))) for k in range(len(specs))]
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def mulberry32(seed):
+    """Seeded uniform generator (same sequence as bodies.js mulberry32)."""
+    a = seed & 0xFFFFFFFF
+
+    def rnd():
+        nonlocal a
+        a = (a + 0x6D2B79F5) & 0xFFFFFFFF
+        t = a
+        t = ((t ^ (t >> 15)) * (t | 1)) & 0xFFFFFFFF
+        t ^= (t + (((t ^ (t >> 7)) * (t | 61)) & 0xFFFFFFFF)) & 0xFFFFFFFF
+        return ((t ^ (t >> 14)) & 0xFFFFFFFF) / 4294967296.0
+    return rnd

@@ -48,6 +48,21 @@ function sphereCase(R = SPHERE.R) {
   return { kind: 'sphere', g, geo, body, ref: { S: Math.PI * R * R, xcg: 0, zcg: 0, c: 1 } };
 }
 
+/* The sphere case refined or coarsened to about `cells` cells: every cell
+ * count scales by f = cbrt(cells / 36864), the core spacing by 1/f and the
+ * stretch ratios by their f-th root, so the physical box stays the same. */
+function sphereCaseN(cells, R = SPHERE.R) {
+  const f = Math.cbrt(cells / 36864), h = 1 / f;
+  const n = (c) => Math.max(1, Math.round(c * f));
+  const fx = stretchedFaces(-6, 6, h, n(18), n(18), Math.pow(1.1, 1 / f));
+  const fy = stretchedFaces(0, 6, h, 0, n(26), Math.pow(1.06, 1 / f));
+  const fz = stretchedFaces(-6, 6, h, n(6), n(6), Math.pow(1.14, 1 / f));
+  const g = A.gridFromFaces(fx, fy, fz, h, 'sphere');
+  const body = B.sphereBody(R);
+  const geo = A.cfdGeomBody(g, body);
+  return { kind: 'sphere', g, geo, body, ref: { S: Math.PI * R * R, xcg: 0, zcg: 0, c: 1 } };
+}
+
 /* Aircraft of the stock family on a grid of about `cells` cells, tail
  * deflection delta (deg). `over` overrides the parameter table. */
 function aircraftCase(type, cells, deltaDeg = 0, over = null) {
@@ -71,4 +86,4 @@ function nacelleStates(sink, bypass, core) {
 }
 const TRANSPORT_FIX = nacelleStates([1.05, 0.45, 1.1], [0.95, 1.0, 1.3], [0.5, 1.6, 1.2]);
 
-module.exports = { engineSpec, stretchedFaces, SPHERE, sphereCase, aircraftCase, transportCase, nacelleStates, TRANSPORT_FIX };
+module.exports = { engineSpec, stretchedFaces, SPHERE, sphereCase, sphereCaseN, aircraftCase, transportCase, nacelleStates, TRANSPORT_FIX };

@@ -13,7 +13,7 @@ const K = require('./cases.js');
 
 function buildCase(spec) {
   const [kind, cells, delta] = spec.split(':');
-  if (kind === 'sphere') return K.sphereCase();
+  if (kind === 'sphere') return cells ? K.sphereCaseN(+cells) : K.sphereCase();
   if (kind === 'transport') { const C = K.transportCase(+cells, +(delta || 0)); C.fix = K.TRANSPORT_FIX; return C; }
   return K.aircraftCase(kind, +cells, +(delta || 0));
 }
@@ -31,7 +31,12 @@ function writeGeom(C, dir) {
   return meta;
 }
 
-if (require.main === module) {
+if (require.main === module && process.argv[2] === 'geom-many') {
+  // node cli.js geom-many <outroot> <spec> ...: one directory per spec, <outroot>/<index>
+  const root = process.argv[3], specs = process.argv.slice(4);
+  specs.forEach((sp, k) => { const t0 = Date.now(); const m = writeGeom(buildCase(sp), path.join(root, String(k))); m.build_s = (Date.now() - t0) / 1000; m.spec = sp; fs.writeFileSync(path.join(root, String(k), 'meta.json'), JSON.stringify(m)); });
+  console.log(JSON.stringify({ n: specs.length }));
+} else if (require.main === module) {
   const [cmd, spec, out] = process.argv.slice(2);
   if (cmd !== 'geom' || !spec || !out) { console.error('usage: node cli.js geom <case> <outdir>'); process.exit(2); }
   const t0 = Date.now();

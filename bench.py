@@ -12,6 +12,7 @@ timed region. Prints one JSON line.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -51,12 +52,167 @@ def copy_probe(nbytes=1 << 30):
     print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}), flush=True)
 
 
+def freestream(M, alpha):
+    import math
+    a = alpha * math.pi / 180
+    return [1, M * math.cos(a), 0, M * math.sin(a), 1 / (1.4 * 0.4) + 0.5 * M * M]
+
+
+def case_setup(meta, M, alpha):
+    """Freestream, fixed states and force-coefficient reference of one built case."""
+    F = freestream(M, alpha)
+    fix = meta["fix"] if meta.get("fix") else [F, F, F]
+    r = meta["ref"]
+    return F, fix, (alpha, M, r["S"], r["xcg"], 0.0, r["c"])
+
+
+def batched_workload(seed=20261203, n=64):
+    """64 cases: sphere or transport (p = 1/2), cells ~ U[30k, 1.0M], M ~ U[0.3, 0.85],
+    alpha ~ U[-2, 8] deg, tail deflection ~ U[-5, 5] deg, all from one mulberry32 seed."""
+    import geometry
+    rnd = geometry.mulberry32(seed)
+    out = []
+    for _ in range(n):
+        body = "sphere" if rnd() < 0.5 else "transport"
+        cells = int(30000 + rnd() * 970000)
+        M, alpha, delta = 0.3 + 0.55 * rnd(), -2 + 10 * rnd(), -5 + 10 * rnd()
+        out.append((f"sphere:{cells}" if body == "sphere" else f"transport:{cells}:{delta:.6f}", M, alpha))
+    return out
+
+
+def run_converged_plain(built, conds, res_drop, max_iters, check_every, tile):
+    """Each case alone on the plain path, one after another, to res/res0 <= res_drop or max_iters."""
+    import aero3d
+    t0 = time.perf_counter()
+    iters = []
+    for (grid, geo, fl, meta), (M, alpha) in zip(built, conds):
+        F, fix, _ = case_setup(meta, M, alpha)
+        s = aero3d.CfdHIP(grid, geo)
+        s.opt["cfl"] = 0.5
+        s.set_tile(*tile)
+        s.set_free(F); s.set_fix(fix); s.reset()
+        s.residual()
+        res0 = None
+        while True:
+            s.step(check_every)
+            r = s.residual()
+            if res0 is None and r > 0:
+                res0 = r
+            if (res0 and r / res0 <= res_drop) or s.it >= max_iters or not math.isfinite(r):
+                break
+        iters.append(s.it)
+        s.dispose()
+    return time.perf_counter() - t0, iters
+
+
+def run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes, max_slots=256):
+    """Submit requests (geometry index, M, alpha, arrival s) to the continuous-batching server as their
+    arrival times come; tick until all are done. Returns results and the wall time."""
+    import aero3d
+    srv = aero3d.Server(arena_bytes=arena_bytes, max_slots=max_slots, check_every=check_every)
+    srv.batch.set_zchunk(zc)
+    gids = [srv.batch.add_geometry(g, geo, fl) for (g, geo, fl, meta) in built]
+    t_start = srv.clock()
+    pending = sorted(range(len(reqs)), key=lambda k: reqs[k][3])
+    done, nxt = [], 0
+    while len(done) < len(reqs):
+        now = srv.clock() - t_start
+        while nxt < len(pending) and reqs[pending[nxt]][3] <= now:
+            k = pending[nxt]
+            gi, M, alpha, arr = reqs[k]
+            F, fix, fref = case_setup(built[gi][3], M, alpha)
+            srv.submit(gids[gi], F, fix, 0.5, 2, fref, res_drop, max_iters, t_start + arr, k)
+            nxt += 1
+        a, p = srv.tick()
+        done += srv.poll()
+        if a == 0 and p == 0 and nxt < len(pending):
+            time.sleep(max(0.0, reqs[pending[nxt]][3] - (srv.clock() - t_start)))
+    wall = srv.clock() - t_start
+    busy, cu = srv.stats()
+    srv.dispose()
+    return done, wall, busy, cu
+
+
+def batched_leg(res_drop=1e-4, max_iters=2000, check_every=50):
+    """Converged solves/s of the 64-case batch (one persistent launch per stage, graph-replayed) against
+    the same cases solved one at a time on the plain path."""
+    import aero3d
+    import geometry
+    wl = batched_workload()
+    print(f"[bench] batched: building {len(wl)} geometries", file=sys.stderr, flush=True)
+    built = geometry.build_cases([w[0] for w in wl])
+    conds = [(w[1], w[2]) for w in wl]
+    cells = sum(b[3]["N"] for b in built)
+    # autotune the batch z chunk on the first eight cases (throwaway batch)
+    tb = aero3d.Batch(arena_bytes=4 << 30, max_slots=16)
+    for (g, geo, fl, meta), (M, alpha) in list(zip(built, conds))[:8]:
+        F, fix, _ = case_setup(meta, M, alpha)
+        tb.add(tb.add_geometry(g, geo), F, fix, 0.5)
+    best = None
+    for zc in (2, 3, 4, 6, 8):
+        tb.set_zchunk(zc)
+        tb.time_steps(1)
+        ms = tb.time_steps(6)
+        if best is None or ms < best[1]:
+            best = (zc, ms)
+    tb.dispose()
+    zc = best[0]
+    print(f"[bench] batched: {cells} cells, zchunk {zc}; server run", file=sys.stderr, flush=True)
+    reqs = [(k, M, alpha, 0.0) for k, (M, alpha) in enumerate(conds)]
+    done, wall_b, busy, cu = run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes=int(cells * 110 + (2 << 30)))
+    print("[bench] batched: plain path, one case at a time", file=sys.stderr, flush=True)
+    wall_p, iters_p = run_converged_plain(built, conds, res_drop, max_iters, check_every, (8, 3))
+    it_b = {d["tag"]: d["iters"] for d in done}
+    same = all(it_b[k] == iters_p[k] for k in range(len(conds)))
+    return {"cases": len(conds), "cells": cells, "solves_per_s": len(conds) / wall_b, "wall_s": wall_b,
+            "cell_updates_per_s": cu / wall_b, "plain_solves_per_s": len(conds) / wall_p, "plain_wall_s": wall_p,
+            "speedup_vs_plain": wall_p / wall_b, "converged": sum(d["converged"] for d in done),
+            "iters_mean": sum(it_b.values()) / len(it_b), "same_iterations_as_plain": same, "zchunk": zc,
+            "criterion": f"res/res0 <= {res_drop} (density residual every {check_every} iterations) or {max_iters} iterations"}
+
+
+def serve_workload(seed=20261204, n=10000):
+    """Poisson arrivals (mean 20 req/s); body sphere / transport / swept-delta single fuselage (1/3 each);
+    level 30k / 120k / 500k cells (0.5 / 0.3 / 0.2); M ~ U[0.3, 0.85], alpha ~ U[-2, 8] deg."""
+    import geometry
+    rnd = geometry.mulberry32(seed)
+    t, out = 0.0, []
+    for _ in range(n):
+        t += -math.log(1 - rnd()) * 0.05
+        b = int(rnd() * 3)
+        u = rnd()
+        lv = 0 if u < 0.5 else (1 if u < 0.8 else 2)
+        M, alpha = 0.3 + 0.55 * rnd(), -2 + 10 * rnd()
+        out.append((3 * b + lv, M, alpha, t))
+    return out
+
+
+SERVE_SPECS = ["sphere:30000", "sphere:120000", "sphere:500000", "transport:30000", "transport:120000",
+               "transport:500000", "interceptor:30000", "interceptor:120000", "interceptor:500000"]
+
+
+def serve_leg(n_requests=200, res_drop=1e-4, max_iters=2000, check_every=50):
+    import geometry
+    reqs = serve_workload()[:n_requests]
+    print(f"[bench] serve: {len(reqs)} requests over {reqs[-1][3]:.1f} s of arrivals", file=sys.stderr, flush=True)
+    built = geometry.build_cases(SERVE_SPECS)
+    done, wall, busy, cu = run_server(built, reqs, res_drop, max_iters, check_every, 3, arena_bytes=64 << 30)
+    lat = sorted(d["t_done"] - d["arrival"] for d in done)
+    pct = lambda p: lat[min(len(lat) - 1, int(round(p / 100 * (len(lat) - 1))))]
+    return {"requests": len(done), "sample": f"first {n_requests} of the 10k-request stream (seed 20261204)",
+            "solves_per_s": len(done) / wall, "wall_s": wall, "p50_s": pct(50), "p95_s": pct(95), "p99_s": pct(99),
+            "gpu_busy_frac": busy / wall, "cell_updates_per_s": cu / wall, "converged": sum(d["converged"] for d in done),
+            "criterion": f"res/res0 <= {res_drop} or {max_iters} iterations, residual every {check_every}"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--legs", default="plain,batched,serve", help="comma list of plain, batched, serve")
+    ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--tile", default="auto", help="stage-kernel shape 'ty,zchunk' or 'auto' (autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of known size (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -115,6 +271,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "wall_s": wall, "it": s.it, "CL": f["CL"], "CD": f["CD"],
     }
+    s.dispose()
+    legs = a.legs.split(",")
+    if "batched" in legs:
+        out["batched"] = batched_leg()
+    if "serve" in legs:
+        out["served"] = serve_leg(a.serve_requests)
     if not a.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(grid, geo, F, fix)

@@ -109,6 +109,91 @@ int aero3d_autotune(aero3d_solver *s, int iters, int *ty, int *zchunk, float *ms
 /* Device-memory footprint of the handle in bytes. */
 int aero3d_footprint(const aero3d_solver *s, int64_t *bytes);
 
+/* ---------------------------------------------------------------------------
+ * Batched form (no reference counterpart: the reference steps one solve per
+ * WebGL context, src/air_cfd.js:414-428). Many independent solves of uneven
+ * grid sizes share one device arena; every iteration is one persistent
+ * launch per RK stage over all cases, replayed from a hipGraph.
+ * ------------------------------------------------------------------------- */
+typedef struct aero3d_batch aero3d_batch;
+/* Arena of `arena_bytes` device bytes for geometries and case states, at most
+ * `max_slots` resident cases. */
+int aero3d_batch_create(int64_t arena_bytes, int max_slots, aero3d_batch **out);
+int aero3d_batch_destroy(aero3d_batch *b);
+const char *aero3d_batch_last_error(const aero3d_batch *b);
+/* Register a grid + cut-cell masks + force list (the inputs of new CfdCPU(g,
+ * geo) and cfdForceList, src/air_cfd.js:143-157), shared read-only by every
+ * case that uses it. *gid identifies it. */
+int aero3d_batch_add_geometry(aero3d_batch *b, const aero3d_grid *g, const uint8_t *t, const float *phi, const float *ap0,
+                              const float *ap1, const float *ap2, const float *ap3, const int32_t *fq, const double *fcx,
+                              const double *fcz, const double *fx, const double *fz, int nf, int *gid);
+int aero3d_batch_remove_geometry(aero3d_batch *b, int gid);
+/* Device bytes one case on geometry gid occupies (state, snapshot, partials). */
+int aero3d_batch_case_bytes(const aero3d_batch *b, int gid, int64_t *bytes);
+/* Add a case on geometry gid: setFree(F), setFix(fix), opt {cfl, order},
+ * reset() (src/air_cfd.js:162-168). Fails with AERO3D_E_STATE when no slot or
+ * arena room is left. */
+int aero3d_batch_add(aero3d_batch *b, int gid, const double F[5], const double fix[15], double cfl, int order, int *slot);
+int aero3d_batch_remove(aero3d_batch *b, int slot);
+/* Stage-kernel z chunk of every case (tiles are 64 x 8 columns). */
+int aero3d_batch_set_zchunk(aero3d_batch *b, int zchunk);
+/* n SSP-RK2 iterations of every resident case (src/air_cfd.js:252-258), enqueued. */
+int aero3d_batch_step(aero3d_batch *b, int n);
+int aero3d_batch_sync(aero3d_batch *b);
+int aero3d_batch_iter(const aero3d_batch *b, int slot, int *it);
+/* read() of one case (src/air_cfd.js:259); waits. */
+int aero3d_batch_read(aero3d_batch *b, int slot, float *r, float *ru, float *rv, float *rw, float *E);
+/* Density residual of each listed case (CFD3.readback, src/air_cfd.js:553-554)
+ * since its previous call; NaN on a case's first call. Waits. */
+int aero3d_batch_residuals(aero3d_batch *b, const int *slots, int n, double *res);
+/* cfdForces of one case on the device (src/air_cfd.js:443-452). Waits. */
+int aero3d_batch_forces(aero3d_batch *b, int slot, double alphaDeg, double M, double S, double xcg, double zcg, double cref,
+                        double out[3]);
+int aero3d_batch_footprint(const aero3d_batch *b, int64_t *used, int64_t *capacity);
+/* n iterations of every resident case bracketed by HIP events on the batch stream. */
+int aero3d_batch_time_steps(aero3d_batch *b, int n, float *ms);
+
+/* ---------------------------------------------------------------------------
+ * Served form (no reference counterpart: the reference runs one solve at a
+ * time from its UI loop, src/air_cfd.js:539-578). Requests queue FIFO and are
+ * admitted into a batch as slots and arena memory free up (admission by HBM
+ * footprint, continuous batching); each tick steps every resident case by the
+ * check interval, measures its density residual on the device and retires it
+ * once res / res0 <= res_drop (the readback residual, src/air_cfd.js:553-554)
+ * or at max_iters, with its force coefficients.
+ * ------------------------------------------------------------------------- */
+typedef struct aero3d_server aero3d_server;
+typedef struct {
+  int geometry;                  /* gid registered on aero3d_server_batch(s)        */
+  double F[5], fix[15];          /* freestream and fixed states                      */
+  double cfl;
+  int order;
+  double alpha, M, S, xcg, zcg, cref;   /* force-coefficient reference (cfdForces)  */
+  double res_drop;               /* convergence: res / res0 <= res_drop              */
+  int max_iters;                 /* iteration cap                                    */
+  double arrival;                /* arrival time on the server clock (s)             */
+  int64_t tag;                   /* caller's id, returned in the result              */
+} aero3d_request;
+typedef struct {
+  int64_t tag;
+  int iters, converged;
+  double res0, res, CL, CD, Cm;
+  double arrival, t_admit, t_done;  /* server clock (s); latency = t_done - arrival */
+} aero3d_result;
+int aero3d_server_create(int64_t arena_bytes, int max_slots, int check_every, aero3d_server **out);
+int aero3d_server_destroy(aero3d_server *s);
+/* The server's batch: register geometries on it with aero3d_batch_add_geometry. */
+aero3d_batch *aero3d_server_batch(aero3d_server *s);
+/* Seconds since the server was created (steady clock). */
+int aero3d_server_clock(const aero3d_server *s, double *now);
+int aero3d_server_submit(aero3d_server *s, const aero3d_request *r);
+/* Admit, step check_every iterations, retire; returns resident / queued counts. */
+int aero3d_server_tick(aero3d_server *s, int *n_active, int *n_pending);
+/* Move up to max finished results to out; *n = number moved. */
+int aero3d_server_poll(aero3d_server *s, aero3d_result *out, int max, int *n);
+/* Wall time spent in ticks with resident work, and cell-updates performed. */
+int aero3d_server_stats(const aero3d_server *s, double *busy_s, int64_t *cell_updates);
+
 #ifdef __cplusplus
 }
 #endif
