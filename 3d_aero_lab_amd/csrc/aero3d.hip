@@ -14,13 +14,14 @@
 // ============================================================================
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
-#include "../../include/aero3d.h"
+#include "aero3d.h"
 #include "euler3d.hpp"
 #include "euler3d_tile.hpp"
 #include "reduce.hpp"
@@ -47,7 +48,9 @@ struct aero3d_solver {
   float *state = nullptr;     // 3 * 5 * ns
   float *dt = nullptr, *rsnap = nullptr;
   int cur = 0;                // buffer index holding the current state Q
-  int tile_ty = 8, tile_zc = 8; // stage kernel: 0 = per-cell kernel, else 64 x tile_ty tiles, z chunks of tile_zc
+  int tile_ty = 8, tile_zc = 0; // stage kernel: ty 0 = per-cell kernel, else 64 x ty tiles; zc > 0 fixed z chunks,
+                                //   zc <= 0 balanced launch of (max(1, -zc)) x resident blocks
+  int resident4 = 0, resident8 = 0; // resident blocks of the balanced kernels on this device
   int it = 0, snap_it = -1;
   double F[5] = {1, 0, 0, 0, 1 / (1.4 * 0.4)};
   double fix[3][5];
@@ -106,6 +109,27 @@ __global__ void __launch_bounds__(64 * TY) tile_kernel(CaseView cv, StageConsts 
   tile_stage<64, TY>(cv, kc, qin, bse, out, dtout, ns, wa, wb, bx * 64, by * TY, k0, k1, sm);
 }
 
+// Balanced launch: exactly G blocks (the device's resident capacity times a
+// small multiple). Block r (XCD-grouped) marches an equal contiguous share of
+// the planes of one tile: tile t owns blocks [ceil(t G / nt), ceil((t+1) G / nt)),
+// and those blocks split its NZ planes evenly. No second partial round of
+// blocks, one z-queue preamble per block.
+template <int TY>
+__global__ void __launch_bounds__(64 * TY) tile_kernel_bal(CaseView cv, StageConsts kc, const float *__restrict__ qin,
+                                                          const float *__restrict__ bse, float *__restrict__ out,
+                                                          float *__restrict__ dtout, size_t ns, float wa, float wb, int ntx, int nty) {
+  __shared__ TileSmem<64, TY> sm;
+  const int G = gridDim.x, nt = ntx * nty;
+  const int r = xcd_remap(blockIdx.x, G);
+  const int t = (int)(((long long)r * nt) / G);
+  const int first = (int)(((long long)t * G + nt - 1) / nt), next = (int)(((long long)(t + 1) * G + nt - 1) / nt);
+  const int c = next - first, l = r - first;
+  const int k0 = (int)(((long long)l * cv.NZ) / c), k1 = (int)(((long long)(l + 1) * cv.NZ) / c);
+  if (k1 <= k0) return;            // block-uniform: more blocks than planes for this tile
+  const int bx = t % ntx, by = t / ntx;
+  tile_stage<64, TY>(cv, kc, qin, bse, out, dtout, ns, wa, wb, bx * 64, by * TY, k0, k1, sm);
+}
+
 // Reset: every plane of buffer `dst` := F.
 __global__ void fill_kernel(float *__restrict__ dst, size_t ns, int N, float f0, float f1, float f2, float f3, float f4) {
   const int q = blockIdx.x * kBlock + threadIdx.x;
@@ -157,7 +181,20 @@ void launch_stage(aero3d_solver *s, const CaseView &cv, const StageConsts &kc, c
     stage_kernel<<<(s->N + kBlock - 1) / kBlock, kBlock, 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb);
     return;
   }
-  const int ntx = (s->NX + 63) / 64, nty = (s->NY + s->tile_ty - 1) / s->tile_ty, nzc = (s->NZ + s->tile_zc - 1) / s->tile_zc;
+  const int ntx = (s->NX + 63) / 64, nty = (s->NY + s->tile_ty - 1) / s->tile_ty;
+  if (s->tile_zc <= 0) {            // balanced launch, G = multiple of the resident block count
+    const int mult = s->tile_zc == 0 ? 1 : -s->tile_zc;
+    const long long nt = (long long)ntx * nty;
+    long long G = (long long)mult * (s->tile_ty == 4 ? s->resident4 : s->resident8);
+    G = std::min(G, nt * s->NZ);
+    G = std::max(G, nt);
+    if (s->tile_ty == 4)
+      tile_kernel_bal<4><<<(int)G, dim3(64, 4), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
+    else
+      tile_kernel_bal<8><<<(int)G, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
+    return;
+  }
+  const int nzc = (s->NZ + s->tile_zc - 1) / s->tile_zc;
   const int nb = ntx * nty * nzc;
   if (s->tile_ty == 4)
     tile_kernel<4><<<nb, dim3(64, 4), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty, s->tile_zc);
@@ -205,6 +242,14 @@ int aero3d_create(const aero3d_grid *g, const uint8_t *t, const float *phi, cons
   int rc;
   if (hipGetDevice(&s->device) != hipSuccess) return bail(fail(s, AERO3D_E_NODEV, "hipGetDevice failed"));
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(s, AERO3D_E_NODEV, "hipStreamCreate failed"));
+  {
+    int ncu = 0, o4 = 0, o8 = 0;
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o4, tile_kernel_bal<4>, 256, 0) != hipSuccess) o4 = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o8, tile_kernel_bal<8>, 512, 0) != hipSuccess) o8 = 1;
+    s->resident4 = std::max(1, ncu * std::max(1, o4));
+    s->resident8 = std::max(1, ncu * std::max(1, o8));
+  }
   // grid: FP32 widths and centres from FP64 faces, as the reference stores them
   std::vector<float> hdx(s->NX), hdy(s->NY), hdz(s->NZ);
   s->hxc.resize(s->NX);
@@ -408,10 +453,10 @@ int aero3d_time_steps(aero3d_solver *s, int n, float *ms) {
 
 int aero3d_set_tile(aero3d_solver *s, int ty, int zchunk) {
   if (!s) return AERO3D_E_ARG;
-  if (!(ty == 0 || ty == 4 || ty == 8) || (ty != 0 && zchunk < 1))
-    return fail(s, AERO3D_E_ARG, "set_tile: ty must be 0, 4 or 8 and zchunk >= 1");
+  if (!(ty == 0 || ty == 4 || ty == 8) || zchunk < -8)
+    return fail(s, AERO3D_E_ARG, "set_tile: ty must be 0, 4 or 8 and zchunk >= -8");
   s->tile_ty = ty;
-  s->tile_zc = zchunk < 1 ? 1 : zchunk;
+  s->tile_zc = zchunk;
   return 0;
 }
 int aero3d_get_tile(const aero3d_solver *s, int *ty, int *zchunk) {
@@ -424,7 +469,7 @@ int aero3d_autotune(aero3d_solver *s, int iters, int *ty, int *zchunk, float *ms
   if (!s || iters < 1) return AERO3D_E_ARG;
   // candidates: both tile heights, z chunks from short (many blocks) to long (little preamble)
   const int tys[] = {4, 8};
-  const int zcs[] = {2, 3, 4, 6, 8, 12, 16, 24, 32};
+  const int zcs[] = {0, -2, -3, 2, 3, 4, 6, 8};
   // time on a scratch copy of the state: save the current state, restore after
   const size_t b = sizeof(float) * 5 * s->ns;
   float *save = nullptr;
@@ -435,7 +480,7 @@ int aero3d_autotune(aero3d_solver *s, int iters, int *ty, int *zchunk, float *ms
   float best = 1e30f;
   for (int t : tys)
     for (int z : zcs) {
-      if (z > s->NZ && z != zcs[0]) continue;
+      if (z > s->NZ) continue;
       s->tile_ty = t; s->tile_zc = z;
       float ms;
       int rc = aero3d_time_steps(s, 1, &ms);          // warm

@@ -32,7 +32,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/aero3d.h"
+#include "aero3d.h"
 #include "euler3d.hpp"
 #include "euler3d_tile.hpp"
 #include "reduce.hpp"

@@ -96,13 +96,16 @@ int aero3d_residual(aero3d_solver *s, double *res);
 int aero3d_time_steps(aero3d_solver *s, int n, float *ms);
 /* Stage-kernel shape (the tuning knob of the stage kernel, no reference
  * counterpart: the reference's shader has one fragment per cell,
- * src/air_cfd.js:405-413). ty = 4 or 8: tiles of 64 x ty columns marching z
- * chunks of `zchunk` planes; ty = 0: one thread per cell, each face flux
- * evaluated by both of its cells as the reference does. Default 8 x 8. */
+ * src/air_cfd.js:405-413). ty = 4 or 8: tiles of 64 x ty columns marching z;
+ * zchunk > 0: fixed chunks of zchunk planes, one block each; zchunk <= 0:
+ * balanced launch of max(1, -zchunk) x (resident blocks of the device), each
+ * block marching an equal share of one tile's planes. ty = 0: one thread per
+ * cell, each face flux evaluated by both of its cells as the reference does.
+ * Default ty 8, zchunk 0. */
 int aero3d_set_tile(aero3d_solver *s, int ty, int zchunk);
 int aero3d_get_tile(const aero3d_solver *s, int *ty, int *zchunk);
 /* Autotuner of the stage-kernel shape: times every candidate (ty in {4, 8},
- * zchunk in {2 .. 32}) over `iters` iterations on this handle's grid, keeps
+ * zchunk in {0, -2, -3, 2, 3, 4, 6, 8}) over `iters` iterations on this handle's grid, keeps
  * the fastest, and restores the state and iteration count. Outputs may be
  * null; *ms_best is the best time per iteration. */
 int aero3d_autotune(aero3d_solver *s, int iters, int *ty, int *zchunk, float *ms_best);
