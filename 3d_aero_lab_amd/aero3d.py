@@ -414,3 +414,135 @@ class Server:
         b, c = C.c_double(), C.c_int64()
         self._chk(blib().aero3d_server_stats(self.h, C.byref(b), C.byref(c)))
         return b.value, c.value
+
+
+# ---------------------------------------------------------------------------
+# axisymmetric engine solver (include/axi.h)
+# ---------------------------------------------------------------------------
+class AxiGrid(C.Structure):
+    _fields_ = [("W", C.c_int), ("H", C.c_int), ("xf", C.POINTER(C.c_double)), ("rf", C.POINTER(C.c_double))]
+
+
+class AxiZones(C.Structure):
+    _fields_ = [("solid", C.c_void_p), ("comp", C.c_void_p)] + [(k, C.c_void_p) for k in
+                                                                 ("phi", "ae", "an", "wgt", "lmix", "tgt", "rate", "yrate", "mfd")]
+
+
+AXI_EXPORTS = ["axi_create", "axi_destroy", "axi_last_error", "axi_set_opt", "axi_set_comps", "axi_set_zones", "axi_set_fs",
+               "axi_set_state", "axi_step", "axi_iter", "axi_read", "axi_sync", "axi_time_steps"]
+_alib = None
+
+
+def alib():
+    global _alib
+    if _alib is None:
+        L = lib()
+        vp = C.c_void_p
+        L.axi_create.argtypes = [C.POINTER(AxiGrid), C.POINTER(AxiZones), f64p, f64p, f64p, C.POINTER(vp)]
+        L.axi_destroy.argtypes = [vp]
+        L.axi_last_error.argtypes = [vp]
+        L.axi_last_error.restype = C.c_char_p
+        L.axi_set_opt.argtypes = [vp, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double]
+        L.axi_set_comps.argtypes = [vp, f64p]
+        L.axi_set_zones.argtypes = [vp, C.POINTER(AxiZones)]
+        L.axi_set_fs.argtypes = [vp, f64p]
+        L.axi_set_state.argtypes = [vp, f64p]
+        L.axi_step.argtypes = [vp, C.c_int]
+        L.axi_iter.argtypes = [vp, C.POINTER(C.c_int)]
+        L.axi_read.argtypes = [vp] + [C.c_void_p] * 4
+        L.axi_sync.argtypes = [vp]
+        L.axi_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_float)]
+        _alib = L
+    return _alib
+
+
+class AxiHIP:
+    """Axisymmetric solver object (GPUSolver protocol, src/cfd_gpu.js:207-323)."""
+
+    ZF = ("phi", "ae", "an", "wgt", "lmix", "tgt", "rate", "yrate", "mfd")
+
+    def __init__(self, xf, rf, zones, comps, fs, Q0, opt=None):
+        L = alib()
+        self._xf = np.ascontiguousarray(xf, np.float64)
+        self._rf = np.ascontiguousarray(rf, np.float64)
+        self.W, self.H = len(self._xf) - 1, len(self._rf) - 1
+        self.N = self.W * self.H
+        g = AxiGrid(self.W, self.H, self._xf.ctypes.data_as(C.POINTER(C.c_double)), self._rf.ctypes.data_as(C.POINTER(C.c_double)))
+        z = self._zones(zones)
+        Q = np.ascontiguousarray(np.asarray(Q0, np.float64).reshape(-1))
+        if Q.size != 5 * self.N:
+            raise ValueError("Q0 must hold 5 planes of W*H values")
+        h = C.c_void_p()
+        rc = L.axi_create(C.byref(g), C.byref(z), np.ascontiguousarray(comps, np.float64),
+                          np.array([fs["r"], fs["u"], fs["p"]], np.float64), Q, C.byref(h))
+        if rc:
+            raise Aero3DError(f"axi_create failed ({rc}): {L.axi_last_error(None).decode()}")
+        self.h = h
+        self.opt = dict({"cfl": 0.6, "order": 2, "visc": True, "hf": 0.99 * 43e6, "global": False, "dtRatio": 0}, **(opt or {}))
+        self._opt_set = None
+
+    def _zones(self, zones):
+        self._zkeep = {"solid": np.ascontiguousarray(zones["solid"], np.uint8), "comp": np.ascontiguousarray(zones["comp"], np.uint8)}
+        for k in self.ZF:
+            self._zkeep[k] = np.ascontiguousarray(zones[k], np.float32)
+        for k, a in self._zkeep.items():
+            if a.shape != (self.N,):
+                raise ValueError(f"zones.{k} has {a.shape}")
+        return AxiZones(*[self._zkeep[k].ctypes.data for k in ("solid", "comp") + self.ZF])
+
+    def _chk(self, rc):
+        if rc:
+            raise Aero3DError(f"libaero3d axi error {rc}: {alib().axi_last_error(self.h).decode()}")
+
+    def dispose(self):
+        if getattr(self, "h", None):
+            alib().axi_destroy(self.h)
+            self.h = None
+
+    __del__ = dispose
+
+    def _sync_opt(self):
+        o = self.opt
+        key = (float(o["cfl"]), int(o["order"]), int(bool(o["visc"])), float(o["hf"] or 0), int(bool(o["global"])), float(o["dtRatio"] or 0))
+        if key != self._opt_set:
+            self._chk(alib().axi_set_opt(self.h, *key))
+            self._opt_set = key
+
+    def set_comps(self, c):
+        self._chk(alib().axi_set_comps(self.h, np.ascontiguousarray(c, np.float64)))
+
+    def set_zones(self, zones):
+        z = self._zones(zones)
+        self._chk(alib().axi_set_zones(self.h, C.byref(z)))
+
+    def set_fs(self, fs):
+        self._chk(alib().axi_set_fs(self.h, np.array([fs["r"], fs["u"], fs["p"]], np.float64)))
+
+    def set_state(self, Q):
+        self._chk(alib().axi_set_state(self.h, np.ascontiguousarray(np.asarray(Q, np.float64).reshape(-1))))
+
+    def step(self, n=1):
+        self._sync_opt()
+        self._chk(alib().axi_step(self.h, n))
+
+    def read(self):
+        Q = np.empty(5 * self.N, np.float32)
+        src = np.empty(self.N, np.float32)
+        mfx = np.empty(self.N, np.float32)
+        dt = np.empty(self.N, np.float32)
+        self._chk(alib().axi_read(self.h, Q.ctypes.data, src.ctypes.data, mfx.ctypes.data, dt.ctypes.data))
+        return Q.reshape(5, self.N), src, mfx, dt
+
+    @property
+    def iter(self):
+        v = C.c_int()
+        self._chk(alib().axi_iter(self.h, C.byref(v)))
+        return v.value
+
+    def time_steps(self, n):
+        self._sync_opt()
+        ms = C.c_float()
+        self._chk(alib().axi_time_steps(self.h, n, C.byref(ms)))
+        return ms.value
+
+EXPORTS = EXPORTS + AXI_EXPORTS

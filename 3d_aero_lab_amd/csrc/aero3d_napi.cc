@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "aero3d.h"
+#include "axi.h"
 
 namespace {
 
@@ -283,12 +284,167 @@ napi_value Autotune(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// ---------------------------------------------------------------------------
+// axisymmetric solver (axi.h)
+// ---------------------------------------------------------------------------
+napi_value axi_throw(napi_env env, axi_solver *s, int rc) {
+  std::string msg = "libaero3d axi error " + std::to_string(rc) + ": " + axi_last_error(s);
+  napi_throw_error(env, nullptr, msg.c_str());
+  return nullptr;
+}
+axi_solver *axi_handle(napi_env env, napi_value v) {
+  void *p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, nullptr, "aero3d addon: not an axisymmetric solver handle (disposed?)");
+    return nullptr;
+  }
+  return static_cast<axi_solver *>(p);
+}
+bool get_prop(napi_env env, napi_value o, const char *k, napi_value *v) {
+  bool has = false;
+  if (napi_has_named_property(env, o, k, &has) != napi_ok || !has || napi_get_named_property(env, o, k, v) != napi_ok) {
+    std::string m = std::string("aero3d addon: zones.") + k + " missing";
+    napi_throw_type_error(env, nullptr, m.c_str());
+    return false;
+  }
+  return true;
+}
+// zones object -> axi_zones (solid/comp Uint8Array, the rest Float32Array)
+bool zones_of(napi_env env, napi_value o, size_t N, axi_zones *z) {
+  napi_value v;
+  size_t n;
+  uint8_t *u8;
+  if (!get_prop(env, o, "solid", &v) || !typed(env, v, napi_uint8_array, &u8, &n)) return false;
+  z->solid = u8;
+  if (n != N) { napi_throw_range_error(env, nullptr, "zones: length"); return false; }
+  if (!get_prop(env, o, "comp", &v) || !typed(env, v, napi_uint8_array, &u8, &n)) return false;
+  z->comp = u8;
+  const char *fk[9] = {"phi", "ae", "an", "wgt", "lmix", "tgt", "rate", "yrate", "mfd"};
+  const float **fp[9] = {&z->phi, &z->ae, &z->an, &z->wgt, &z->lmix, &z->tgt, &z->rate, &z->yrate, &z->mfd};
+  for (int i = 0; i < 9; i++) {
+    float *f;
+    if (!get_prop(env, o, fk[i], &v) || !typed(env, v, napi_float32_array, &f, &n)) return false;
+    if (n != N) { napi_throw_range_error(env, nullptr, "zones: length"); return false; }
+    *fp[i] = f;
+  }
+  return true;
+}
+// axiCreate(xf, rf, zones, comps Float64Array(32), fs Float64Array(3), Q0 Float64Array(5N))
+napi_value AxiCreate(napi_env env, napi_callback_info info) {
+  napi_value a[6];
+  if (!args(env, info, 6, a)) return nullptr;
+  double *xf, *rf, *comps, *fs, *Q0;
+  size_t nx, nr, nc, nf, nq;
+  if (!typed(env, a[0], napi_float64_array, &xf, &nx) || !typed(env, a[1], napi_float64_array, &rf, &nr) ||
+      !typed(env, a[3], napi_float64_array, &comps, &nc) || !typed(env, a[4], napi_float64_array, &fs, &nf) ||
+      !typed(env, a[5], napi_float64_array, &Q0, &nq))
+    return nullptr;
+  const size_t N = (nx - 1) * (nr - 1);
+  if (nx < 3 || nr < 3 || nc != 32 || nf != 3 || nq != 5 * N) { napi_throw_range_error(env, nullptr, "axiCreate: sizes"); return nullptr; }
+  axi_zones z;
+  if (!zones_of(env, a[2], N, &z)) return nullptr;
+  axi_grid g{(int)nx - 1, (int)nr - 1, xf, rf};
+  axi_solver *s = nullptr;
+  int rc = axi_create(&g, &z, comps, fs, Q0, &s);
+  if (rc) return axi_throw(env, nullptr, rc);
+  napi_value ext;
+  NAPI_OK(napi_create_external(env, s, nullptr, nullptr, &ext));
+  return ext;
+}
+napi_value AxiDispose(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!args(env, info, 1, a)) return nullptr;
+  axi_solver *s = axi_handle(env, a[0]);
+  if (!s) return nullptr;
+  axi_destroy(s);
+  return undef(env);
+}
+// axiSetOpt(h, cfl, order, visc, hf, global, dtRatio)
+napi_value AxiSetOpt(napi_env env, napi_callback_info info) {
+  napi_value a[7];
+  if (!args(env, info, 7, a)) return nullptr;
+  axi_solver *s = axi_handle(env, a[0]);
+  if (!s) return nullptr;
+  int rc = axi_set_opt(s, num(env, a[1]), (int)num(env, a[2]), (int)num(env, a[3]), num(env, a[4]), (int)num(env, a[5]), num(env, a[6]));
+  return rc ? axi_throw(env, s, rc) : undef(env);
+}
+napi_value AxiSetComps(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!args(env, info, 2, a)) return nullptr;
+  axi_solver *s = axi_handle(env, a[0]);
+  double *c; size_t n;
+  if (!s || !typed(env, a[1], napi_float64_array, &c, &n)) return nullptr;
+  if (n != 32) { napi_throw_range_error(env, nullptr, "setComps: 32 values"); return nullptr; }
+  int rc = axi_set_comps(s, c);
+  return rc ? axi_throw(env, s, rc) : undef(env);
+}
+napi_value AxiSetZones(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!args(env, info, 3, a)) return nullptr;
+  axi_solver *s = axi_handle(env, a[0]);
+  if (!s) return nullptr;
+  axi_zones z;
+  if (!zones_of(env, a[1], (size_t)num(env, a[2]), &z)) return nullptr;
+  int rc = axi_set_zones(s, &z);
+  return rc ? axi_throw(env, s, rc) : undef(env);
+}
+napi_value AxiSetFs(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!args(env, info, 2, a)) return nullptr;
+  axi_solver *s = axi_handle(env, a[0]);
+  double *f; size_t n;
+  if (!s || !typed(env, a[1], napi_float64_array, &f, &n)) return nullptr;
+  if (n != 3) { napi_throw_range_error(env, nullptr, "setFs: r, u, p"); return nullptr; }
+  int rc = axi_set_fs(s, f);
+  return rc ? axi_throw(env, s, rc) : undef(env);
+}
+napi_value AxiSetState(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!args(env, info, 2, a)) return nullptr;
+  axi_solver *s = axi_handle(env, a[0]);
+  double *q; size_t n;
+  if (!s || !typed(env, a[1], napi_float64_array, &q, &n)) return nullptr;
+  int rc = axi_set_state(s, q);
+  return rc ? axi_throw(env, s, rc) : undef(env);
+}
+napi_value AxiStep(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!args(env, info, 2, a)) return nullptr;
+  axi_solver *s = axi_handle(env, a[0]);
+  if (!s) return nullptr;
+  int rc = axi_step(s, (int)num(env, a[1]));
+  return rc ? axi_throw(env, s, rc) : undef(env);
+}
+// axiRead(h, Q Float32Array(5N), src, mfx)
+napi_value AxiRead(napi_env env, napi_callback_info info) {
+  napi_value a[4];
+  if (!args(env, info, 4, a)) return nullptr;
+  axi_solver *s = axi_handle(env, a[0]);
+  float *q, *sr, *mf;
+  if (!s || !typed(env, a[1], napi_float32_array, &q, nullptr) || !typed(env, a[2], napi_float32_array, &sr, nullptr) ||
+      !typed(env, a[3], napi_float32_array, &mf, nullptr))
+    return nullptr;
+  int rc = axi_read(s, q, sr, mf, nullptr);
+  return rc ? axi_throw(env, s, rc) : undef(env);
+}
+napi_value AxiSync(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!args(env, info, 1, a)) return nullptr;
+  axi_solver *s = axi_handle(env, a[0]);
+  if (!s) return nullptr;
+  int rc = axi_sync(s);
+  return rc ? axi_throw(env, s, rc) : undef(env);
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct { const char *name; napi_callback cb; } fns[] = {
       {"create", Create}, {"dispose", Dispose}, {"setFree", SetFree}, {"setFix", SetFix}, {"setOpt", SetOpt},
       {"reset", Reset}, {"step", Step}, {"iter", Iter}, {"read", Read}, {"write", Write}, {"sync", Sync},
       {"setForceList", SetForceList}, {"forces", Forces}, {"trefftz", Trefftz}, {"residual", Residual},
-      {"timeSteps", TimeSteps}, {"autotune", Autotune}};
+      {"timeSteps", TimeSteps}, {"autotune", Autotune},
+      {"axiCreate", AxiCreate}, {"axiDispose", AxiDispose}, {"axiSetOpt", AxiSetOpt}, {"axiSetComps", AxiSetComps},
+      {"axiSetZones", AxiSetZones}, {"axiSetFs", AxiSetFs}, {"axiSetState", AxiSetState}, {"axiStep", AxiStep},
+      {"axiRead", AxiRead}, {"axiSync", AxiSync}};
   for (auto &f : fns) {
     napi_value fn;
     NAPI_OK(napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));

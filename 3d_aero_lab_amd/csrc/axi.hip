@@ -1,0 +1,231 @@
+// ============================================================================
+// axi.hip — axisymmetric engine solver on gfx950 (C ABI: include/axi.h).
+//
+// Scheme of the reference CPUSolver (src/cfd_cpu.js:7-210), FP32:
+//   face kernel  one thread per face; x faces ((W+1) x H) then r faces
+//                (W x (H+1)); primitives of the 2-4 cells a face reads,
+//                MUSCL-minmod on full cells (phi > 0.999), HLLC with per-side
+//                gamma, Riemann-invariant farfield at x = lo, x = hi, r = top;
+//                r faces rotate (normal v, tangential u) and add the
+//                mixing-length shear, heat and species fluxes; each face is
+//                evaluated once and stored (5 planes) with its eddy viscosity.
+//   cell kernel  one thread per cell; flux divergence with r-weighted faces,
+//                wall / geometric pressure source, blade-row work and the
+//                relaxed heat source, local dt (stage 1), RK update, floors.
+// Per SSP-RK2 iteration: face(Q), cell(Q -> Q1, dt), face(Q1), cell(Q1, Q -> Q).
+// Device layout: Q and Q1 as 5 planes of N floats, face fluxes Fx 5 x
+// (W+1)H, Fr 5 x W(H+1), face eddy viscosity W(H+1), dt / src / mfx N floats.
+// ============================================================================
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "axi.h"
+
+#include "axi_kernels.hpp"
+
+using namespace axik;
+
+namespace {
+
+std::string g_axi_err;
+
+}  // namespace
+
+struct axi_solver {
+  int W = 0, H = 0, N = 0, it = 0;
+  hipStream_t stream = nullptr;
+  AxiConsts c;
+  float *rc = nullptr, *dx = nullptr, *dr = nullptr, *rf = nullptr;
+  uint8_t *solid = nullptr, *comp = nullptr;
+  float *zf = nullptr;            // 9 zone planes: phi ae an wgt lmix tgt rate yrate mfd
+  float *Q = nullptr, *Q1 = nullptr, *Fx = nullptr, *Fr = nullptr, *nutF = nullptr, *dt = nullptr, *src = nullptr, *mfx = nullptr;
+  float *partial = nullptr, *cap = nullptr;
+  int global = 0;
+  double dtRatio = 0;
+  std::string err;
+};
+
+namespace {
+
+int afail(axi_solver *s, int code, const std::string &m) { if (s) s->err = m; else g_axi_err = m; return code; }
+#define ACHK(s, expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return afail((s), -1000 - (int)e_, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+AxiView view(const axi_solver *s) {
+  AxiView v;
+  v.W = s->W; v.H = s->H; v.N = s->N;
+  v.rc = s->rc; v.dx = s->dx; v.dr = s->dr; v.rf = s->rf;
+  v.solid = s->solid; v.comp = s->comp;
+  const size_t N = s->N;
+  v.phi = s->zf; v.ae = s->zf + N; v.an = s->zf + 2 * N; v.wgt = s->zf + 3 * N; v.lmix = s->zf + 4 * N;
+  v.tgt = s->zf + 5 * N; v.rate = s->zf + 6 * N; v.yrate = s->zf + 7 * N; v.mfd = s->zf + 8 * N;
+  return v;
+}
+
+int upload_zones(axi_solver *s, const axi_zones *z) {
+  if (!z || !z->solid || !z->comp || !z->phi || !z->ae || !z->an || !z->wgt || !z->lmix || !z->tgt || !z->rate || !z->yrate || !z->mfd)
+    return afail(s, -1, "zones: null array");
+  const size_t N = s->N;
+  ACHK(s, hipMemcpy(s->solid, z->solid, N, hipMemcpyHostToDevice));
+  ACHK(s, hipMemcpy(s->comp, z->comp, N, hipMemcpyHostToDevice));
+  const float *planes[9] = {z->phi, z->ae, z->an, z->wgt, z->lmix, z->tgt, z->rate, z->yrate, z->mfd};
+  for (int p = 0; p < 9; p++) ACHK(s, hipMemcpy(s->zf + p * N, planes[p], 4 * N, hipMemcpyHostToDevice));
+  for (size_t k = 0; k < N; k++)
+    if (z->comp[k] >= 8) return afail(s, -1, "zones: component id >= 8");
+  return 0;
+}
+
+int steps(axi_solver *s, int n) {
+  const AxiView v = view(s);
+  const int NF = (s->W + 1) * s->H + s->W * (s->H + 1);
+  const dim3 gf((NF + kB - 1) / kB), gc((s->N + kB - 1) / kB);
+  const bool capped = s->global || s->dtRatio != 0;
+  const int nbm = std::min(1024, (s->N + kB - 1) / kB);
+  for (int k = 0; k < n; k++) {
+    face_kernel<<<gf, kB, 0, s->stream>>>(v, s->c, s->Q, s->Fx, s->Fr, s->nutF);
+    if (capped) {
+      cell_kernel<<<gc, kB, 0, s->stream>>>(v, s->c, s->Q, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 2, s->cap);
+      dtmin_kernel<<<nbm, kB, 0, s->stream>>>(s->dt, s->solid, s->N, s->partial);
+      dtcap_kernel<<<1, 64, 0, s->stream>>>(s->partial, nbm, s->global ? 1.f : (float)s->dtRatio, s->cap);
+      cell_kernel<<<gc, kB, 0, s->stream>>>(v, s->c, s->Q, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 3, s->cap);
+    } else {
+      cell_kernel<<<gc, kB, 0, s->stream>>>(v, s->c, s->Q, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 0, s->cap);
+    }
+    face_kernel<<<gf, kB, 0, s->stream>>>(v, s->c, s->Q1, s->Fx, s->Fr, s->nutF);
+    cell_kernel<<<gc, kB, 0, s->stream>>>(v, s->c, s->Q1, s->Q, s->Q, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 1, s->cap);
+    s->it++;
+  }
+  ACHK(s, hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int axi_create(const axi_grid *g, const axi_zones *z, const double comps[32], const double fs[3], const double *Q0, axi_solver **out) {
+  if (!out) return afail(nullptr, -1, "axi_create: null output");
+  *out = nullptr;
+  if (!g || !g->xf || !g->rf || !z || !comps || !fs || !Q0 || g->W < 2 || g->H < 2) return afail(nullptr, -1, "axi_create: bad argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return afail(nullptr, -2, "axi_create: no HIP device");
+  axi_solver *s = new axi_solver();
+  s->W = g->W; s->H = g->H; s->N = g->W * g->H;
+  const size_t N = s->N, NXF = (size_t)(s->W + 1) * s->H, NRF = (size_t)s->W * (s->H + 1);
+  auto bail = [&](int rc) { g_axi_err = s->err; axi_destroy(s); return rc; };
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(afail(s, -2, "stream"));
+  std::memset(&s->c, 0, sizeof s->c);
+  s->c.cfl = 0.6f; s->c.order = 2; s->c.visc = 1; s->c.hf = 0.99f * 43e6f;
+  for (int q = 0; q < 32; q++) s->c.comps[q] = (float)comps[q];
+  s->c.fs_r = (float)fs[0]; s->c.fs_u = (float)fs[1]; s->c.fs_p = (float)fs[2];
+  std::vector<float> rc(s->H), dx(s->W), dr(s->H), rf(s->H + 1);
+  for (int i = 0; i < s->W; i++) dx[i] = (float)(g->xf[i + 1] - g->xf[i]);
+  for (int j = 0; j < s->H; j++) { rc[j] = (float)(0.5 * (g->rf[j] + g->rf[j + 1])); dr[j] = (float)(g->rf[j + 1] - g->rf[j]); }
+  for (int j = 0; j <= s->H; j++) rf[j] = (float)g->rf[j];
+  auto al = [&](auto **p, size_t n) { return hipMalloc((void **)p, sizeof(**p) * (n ? n : 1)); };
+  if (al(&s->rc, s->H) || al(&s->dx, s->W) || al(&s->dr, s->H) || al(&s->rf, s->H + 1) || al(&s->solid, N) || al(&s->comp, N) ||
+      al(&s->zf, 9 * N) || al(&s->Q, 5 * N) || al(&s->Q1, 5 * N) || al(&s->Fx, 5 * NXF) || al(&s->Fr, 5 * NRF) || al(&s->nutF, NRF) ||
+      al(&s->dt, N) || al(&s->src, N) || al(&s->mfx, N) || al(&s->partial, 1024) || al(&s->cap, 1))
+    return bail(afail(s, -1002, "axi_create: hipMalloc failed"));
+  if (hipMemcpy(s->rc, rc.data(), 4 * rc.size(), hipMemcpyHostToDevice) || hipMemcpy(s->dx, dx.data(), 4 * dx.size(), hipMemcpyHostToDevice) ||
+      hipMemcpy(s->dr, dr.data(), 4 * dr.size(), hipMemcpyHostToDevice) || hipMemcpy(s->rf, rf.data(), 4 * rf.size(), hipMemcpyHostToDevice) ||
+      hipMemset(s->src, 0, 4 * N) || hipMemset(s->mfx, 0, 4 * N) || hipMemset(s->dt, 0, 4 * N))
+    return bail(afail(s, -1001, "axi_create: upload failed"));
+  int rc_;
+  if ((rc_ = upload_zones(s, z))) return bail(rc_);
+  if ((rc_ = axi_set_state(s, Q0))) return bail(rc_);
+  *out = s;
+  return 0;
+}
+
+int axi_destroy(axi_solver *s) {
+  if (!s) return 0;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  void *p[] = {s->rc, s->dx, s->dr, s->rf, s->solid, s->comp, s->zf, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, s->partial, s->cap};
+  for (void *x : p) if (x) (void)hipFree(x);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return 0;
+}
+
+const char *axi_last_error(const axi_solver *s) { return s ? s->err.c_str() : g_axi_err.c_str(); }
+
+int axi_set_opt(axi_solver *s, double cfl, int order, int visc, double hf, int global, double dtRatio) {
+  if (!s) return -1;
+  if (!(cfl > 0 && cfl < 10) || (order != 1 && order != 2) || dtRatio < 0) return afail(s, -1, "set_opt: cfl in (0,10), order 1|2, dtRatio >= 0");
+  s->c.cfl = (float)cfl; s->c.order = order; s->c.visc = visc ? 1 : 0; s->c.hf = hf > 0 ? (float)hf : 0.99f * 43e6f;
+  s->global = global ? 1 : 0; s->dtRatio = dtRatio;
+  return 0;
+}
+int axi_set_comps(axi_solver *s, const double comps[32]) {
+  if (!s || !comps) return -1;
+  for (int q = 0; q < 32; q++) s->c.comps[q] = (float)comps[q];
+  return 0;
+}
+int axi_set_zones(axi_solver *s, const axi_zones *z) {
+  if (!s) return -1;
+  ACHK(s, hipStreamSynchronize(s->stream));
+  return upload_zones(s, z);
+}
+int axi_set_fs(axi_solver *s, const double fs[3]) {
+  if (!s || !fs) return -1;
+  s->c.fs_r = (float)fs[0]; s->c.fs_u = (float)fs[1]; s->c.fs_p = (float)fs[2];
+  return 0;
+}
+int axi_set_state(axi_solver *s, const double *Q) {
+  if (!s || !Q) return -1;
+  std::vector<float> f(5 * (size_t)s->N);
+  for (size_t k = 0; k < f.size(); k++) f[k] = (float)Q[k];
+  ACHK(s, hipStreamSynchronize(s->stream));
+  ACHK(s, hipMemcpy(s->Q, f.data(), 4 * f.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+int axi_step(axi_solver *s, int n) {
+  if (!s || n < 0) return afail(s, -1, "step: n >= 0");
+  return steps(s, n);
+}
+int axi_iter(const axi_solver *s, int *it) {
+  if (!s || !it) return -1;
+  *it = s->it;
+  return 0;
+}
+int axi_read(axi_solver *s, float *Q, float *src, float *mfx, float *dt) {
+  if (!s) return -1;
+  const size_t b = 4 * (size_t)s->N;
+  if (Q) ACHK(s, hipMemcpyAsync(Q, s->Q, 5 * b, hipMemcpyDeviceToHost, s->stream));
+  if (src) ACHK(s, hipMemcpyAsync(src, s->src, b, hipMemcpyDeviceToHost, s->stream));
+  if (mfx) ACHK(s, hipMemcpyAsync(mfx, s->mfx, b, hipMemcpyDeviceToHost, s->stream));
+  if (dt) ACHK(s, hipMemcpyAsync(dt, s->dt, b, hipMemcpyDeviceToHost, s->stream));
+  ACHK(s, hipStreamSynchronize(s->stream));
+  return 0;
+}
+int axi_sync(axi_solver *s) {
+  if (!s) return -1;
+  ACHK(s, hipStreamSynchronize(s->stream));
+  return 0;
+}
+int axi_time_steps(axi_solver *s, int n, float *ms) {
+  if (!s || !ms) return -1;
+  hipEvent_t e0, e1;
+  ACHK(s, hipEventCreate(&e0));
+  ACHK(s, hipEventCreate(&e1));
+  ACHK(s, hipEventRecord(e0, s->stream));
+  int rc = steps(s, n);
+  if (rc) return rc;
+  ACHK(s, hipEventRecord(e1, s->stream));
+  ACHK(s, hipEventSynchronize(e1));
+  ACHK(s, hipEventElapsedTime(ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,276 @@
+// ============================================================================
+// axi_kernels.hpp — device code of the axisymmetric solver (face and cell
+// kernels, dt minimum); included by axi.hip. Scheme: CPUSolver,
+// src/cfd_cpu.js:7-210, in FP32.
+// ============================================================================
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace axik {
+
+constexpr int kB = 256;
+constexpr float kR = 287.05f;
+
+struct AxiConsts {
+  float comps[32];
+  float fs_r, fs_u, fs_p;
+  float cfl, hf;
+  int order, visc;
+};
+
+struct AxiView {
+  int W, H, N;
+  const float *rc, *dx, *dr, *rf;
+  const uint8_t *solid, *comp;
+  const float *phi, *ae, *an, *wgt, *lmix, *tgt, *rate, *yrate, *mfd;
+};
+
+__device__ __forceinline__ float clampf(float v, float a, float b) { return v < a ? a : v > b ? b : v; }
+__device__ __forceinline__ float minmod(float a, float b) { return a * b <= 0.f ? 0.f : (fabsf(a) < fabsf(b) ? a : b); }
+
+struct Prim { float r, u, v, p, Y, g; };
+
+// CPUSolver.prim for one cell (src/cfd_cpu.js:67-74)
+__device__ __forceinline__ Prim prim(const float *__restrict__ Q, int N, int k) {
+  Prim P;
+  P.r = fmaxf(Q[k], 1e-4f);
+  P.u = Q[N + k] / P.r; P.v = Q[2 * N + k] / P.r;
+  P.Y = clampf(Q[4 * N + k] / P.r, 0.f, 1.f);
+  P.g = 1.4f - 0.07f * P.Y;
+  P.p = fmaxf(1.f, (P.g - 1.f) * (Q[3 * N + k] - 0.5f * P.r * (P.u * P.u + P.v * P.v)));
+  return P;
+}
+
+// hllc (src/cfd_cpu.js:7-31); states (rho, un, ut, p, Y) with gamma each side
+__device__ void hllc(float rL, float uL, float vL, float pL, float YL, float gL, float rR, float uR, float vR, float pR, float YR,
+                     float gR, float *F) {
+  const float cL = sqrtf(gL * pL / rL), cR = sqrtf(gR * pR / rR);
+  const float SL = fminf(uL - cL, uR - cR), SR = fmaxf(uL + cL, uR + cR);
+  const float EL = pL / (gL - 1.f) + 0.5f * rL * (uL * uL + vL * vL), ER = pR / (gR - 1.f) + 0.5f * rR * (uR * uR + vR * vR);
+  if (SL >= 0.f) { F[0] = rL * uL; F[1] = rL * uL * uL + pL; F[2] = rL * uL * vL; F[3] = uL * (EL + pL); F[4] = rL * uL * YL; return; }
+  if (SR <= 0.f) { F[0] = rR * uR; F[1] = rR * uR * uR + pR; F[2] = rR * uR * vR; F[3] = uR * (ER + pR); F[4] = rR * uR * YR; return; }
+  const float SM = (pR - pL + rL * uL * (SL - uL) - rR * uR * (SR - uR)) / (rL * (SL - uL) - rR * (SR - uR));
+  const bool left = SM >= 0.f;
+  const float rK = left ? rL : rR, uK = left ? uL : uR, vK = left ? vL : vR, pK = left ? pL : pR, YK = left ? YL : YR;
+  const float EK = left ? EL : ER, S = left ? SL : SR;
+  const float f = rK * (S - uK) / (S - SM);
+  const float Es = f * (EK / rK + (SM - uK) * (SM + pK / (rK * (S - uK))));
+  F[0] = rK * uK + S * (f - rK);
+  F[1] = rK * uK * uK + pK + S * (f * SM - rK * uK);
+  F[2] = rK * uK * vK + S * (f * vK - rK * vK);
+  F[3] = uK * (EK + pK) + S * (Es - EK);
+  F[4] = rK * uK * YK + S * (f * YK - rK * YK);
+}
+
+// farfield ghost (src/cfd_cpu.js:35-48), outward normal (nx, nr); G = rho, u, v, p, Y
+__device__ void farfield(const AxiConsts &c, const Prim &P, float nx, float nr, float *G) {
+  const float un = P.u * nx + P.v * nr, cs = sqrtf(P.g * P.p / P.r);
+  const float g0 = 1.4f, c0 = sqrtf(g0 * c.fs_p / c.fs_r), un0 = c.fs_u * nx;
+  if (un >= cs) { G[0] = P.r; G[1] = P.u; G[2] = P.v; G[3] = P.p; G[4] = P.Y; return; }
+  if (un0 <= -c0) { G[0] = c.fs_r; G[1] = c.fs_u; G[2] = 0.f; G[3] = c.fs_p; G[4] = 0.f; return; }
+  const float Rp = un + 2.f * cs / (P.g - 1.f), Rm = un0 - 2.f * c0 / (g0 - 1.f);
+  const float unb = 0.5f * (Rp + Rm);
+  float gb, s, ut, vt, Yb;
+  if (unb > 0.f) { gb = P.g; s = P.p / powf(P.r, P.g); ut = P.u - un * nx; vt = P.v - un * nr; Yb = P.Y; }
+  else { gb = g0; s = c.fs_p / powf(c.fs_r, g0); ut = c.fs_u - un0 * nx; vt = -un0 * nr; Yb = 0.f; }
+  const float cb = fmaxf(1.f, 0.25f * (gb - 1.f) * (Rp - Rm));
+  const float rb = powf(cb * cb / (gb * s), 1.f / (gb - 1.f));
+  G[0] = rb; G[1] = ut + unb * nx; G[2] = vt + unb * nr; G[3] = rb * cb * cb / gb; G[4] = Yb;
+}
+
+__device__ __forceinline__ void sel(const Prim &P, int dir, float *o) {
+  o[0] = P.r; o[1] = dir ? P.v : P.u; o[2] = dir ? P.u : P.v; o[3] = P.p; o[4] = P.Y;
+}
+
+// MUSCL face states (recon, src/cfd_cpu.js:86-94)
+__device__ void recon(const Prim &A, const Prim *Am, const Prim &B, const Prim *Bp, int dir, bool second, float *L, float *R) {
+  float A5[5], B5[5], T5[5];
+  sel(A, dir, A5); sel(B, dir, B5);
+  for (int q = 0; q < 5; q++) { L[q] = A5[q]; R[q] = B5[q]; }
+  if (!second) return;
+  if (Am) { sel(*Am, dir, T5); for (int q = 0; q < 5; q++) L[q] = A5[q] + 0.5f * minmod(A5[q] - T5[q], B5[q] - A5[q]); }
+  if (Bp) { sel(*Bp, dir, T5); for (int q = 0; q < 5; q++) R[q] = B5[q] - 0.5f * minmod(B5[q] - A5[q], T5[q] - B5[q]); }
+  if (L[0] <= 0.f || L[3] <= 0.f) for (int q = 0; q < 5; q++) L[q] = A5[q];
+  if (R[0] <= 0.f || R[3] <= 0.f) for (int q = 0; q < 5; q++) R[q] = B5[q];
+}
+
+// One thread per face (src/cfd_cpu.js:97-143).
+__global__ void __launch_bounds__(kB) face_kernel(AxiView v, AxiConsts c, const float *__restrict__ Q, float *__restrict__ Fx,
+                                                  float *__restrict__ Fr, float *__restrict__ nutF) {
+  const int W = v.W, H = v.H, N = v.N;
+  const int NXF = (W + 1) * H, NRF = W * (H + 1);
+  const int t = blockIdx.x * kB + threadIdx.x;
+  const bool second = c.order == 2;
+  float F[5], G[5], L[5], R[5];
+  auto full = [&](int k) { return v.phi[k] > 0.999f; };
+  if (t < NXF) {
+    const int j = t / (W + 1), f = t % (W + 1), a = f - 1, b = f, ka = j * W + a, kb = j * W + b;
+    float al;
+    if (a < 0) al = v.solid[kb] ? 0.f : 1.f; else if (b >= W) al = v.solid[ka] ? 0.f : 1.f; else al = v.ae[ka];
+    if (al <= 0.f) { for (int q = 0; q < 5; q++) Fx[q * NXF + t] = 0.f; return; }
+    if (a < 0) {
+      const Prim Pb = prim(Q, N, kb);
+      farfield(c, Pb, -1.f, 0.f, G);
+      hllc(G[0], G[1], G[2], G[3], G[4], 1.4f - 0.07f * G[4], Pb.r, Pb.u, Pb.v, Pb.p, Pb.Y, Pb.g, F);
+    } else if (b >= W) {
+      const Prim Pa = prim(Q, N, ka);
+      farfield(c, Pa, 1.f, 0.f, G);
+      hllc(Pa.r, Pa.u, Pa.v, Pa.p, Pa.Y, Pa.g, G[0], G[1], G[2], G[3], G[4], 1.4f - 0.07f * G[4], F);
+    } else {
+      const Prim Pa = prim(Q, N, ka), Pb = prim(Q, N, kb);
+      const bool fa = full(ka) && full(kb);
+      Prim Pm, Pp;
+      const bool hm = second && fa && a - 1 >= 0 && full(ka - 1), hp = second && fa && b + 1 < W && full(kb + 1);
+      if (hm) Pm = prim(Q, N, ka - 1);
+      if (hp) Pp = prim(Q, N, kb + 1);
+      recon(Pa, hm ? &Pm : nullptr, Pb, hp ? &Pp : nullptr, 0, second, L, R);
+      hllc(L[0], L[1], L[2], L[3], L[4], 1.4f - 0.07f * L[4], R[0], R[1], R[2], R[3], R[4], 1.4f - 0.07f * R[4], F);
+    }
+    for (int q = 0; q < 5; q++) Fx[q * NXF + t] = al * F[q];
+    return;
+  }
+  const int u = t - NXF;
+  if (u >= NRF) return;
+  const int f = u / W, i = u % W, a = f - 1, b = f, ka = a * W + i, kb = b * W + i;
+  float G2[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  float nut = 0.f;
+  const float al = a < 0 ? 0.f : b >= H ? (v.solid[ka] ? 0.f : 1.f) : v.an[ka];
+  if (al <= 0.f) { for (int q = 0; q < 5; q++) Fr[q * NRF + u] = 0.f; nutF[u] = 0.f; return; }
+  if (b >= H) {
+    const Prim Pa = prim(Q, N, ka);
+    farfield(c, Pa, 0.f, 1.f, G);
+    hllc(Pa.r, Pa.v, Pa.u, Pa.p, Pa.Y, Pa.g, G[0], G[2], G[1], G[3], G[4], 1.4f - 0.07f * G[4], F);
+  } else {
+    const Prim Pa = prim(Q, N, ka), Pb = prim(Q, N, kb);
+    const bool fa = full(ka) && full(kb);
+    Prim Pm, Pp;
+    const bool hm = second && fa && a - 1 >= 0 && full(ka - W), hp = second && fa && b + 1 < H && full(kb + W);
+    if (hm) Pm = prim(Q, N, ka - W);
+    if (hp) Pp = prim(Q, N, kb + W);
+    recon(Pa, hm ? &Pm : nullptr, Pb, hp ? &Pp : nullptr, 1, second, L, R);
+    hllc(L[0], L[1], L[2], L[3], L[4], 1.4f - 0.07f * L[4], R[0], R[1], R[2], R[3], R[4], 1.4f - 0.07f * R[4], F);
+    if (c.visc) {            // mixing-length shear, heat and species fluxes (src/cfd_cpu.js:129-140)
+      const float drc = v.rc[b] - v.rc[a];
+      const float dudr = (Pb.u - Pa.u) / drc;
+      const float l = 0.5f * (v.lmix[ka] + v.lmix[kb]);
+      nut = l * l * fabsf(dudr);
+      const float rho = 0.5f * (Pa.r + Pb.r), mu = rho * nut;
+      const float Ta = Pa.p / (Pa.r * kR), Tb = Pb.p / (Pb.r * kR);
+      const float ga = 0.5f * (Pa.g + Pb.g), cp = ga * kR / (ga - 1.f);
+      const float tau = mu * dudr;
+      G2[1] = tau;
+      G2[3] = 0.5f * (Pa.u + Pb.u) * tau + mu * cp / 0.9f * (Tb - Ta) / drc;
+      G2[4] = mu / 0.9f * (Pb.Y - Pa.Y) / drc;
+    }
+  }
+  Fr[u] = al * F[0];
+  Fr[NRF + u] = al * (F[2] - G2[1]);
+  Fr[2 * NRF + u] = al * F[1];
+  Fr[3 * NRF + u] = al * (F[3] - G2[3]);
+  Fr[4 * NRF + u] = al * (F[4] - G2[4]);
+  nutF[u] = nut;
+}
+
+// One thread per cell (src/cfd_cpu.js:145-185, 187-207). mode 0: stage 1
+// (dt from the state, Qout = Qin + dt R); mode 1: stage 2 (Qout = (Qb + Qin +
+// dt R) / 2); mode 2: stage 1, dt only (for the global / ratio cap); mode 3:
+// stage 1 update with the stored dt capped at *cap.
+__global__ void __launch_bounds__(kB) cell_kernel(AxiView v, AxiConsts c, const float *__restrict__ Qin, const float *Qb,
+                                                  float *Qout, const float *__restrict__ Fx, const float *__restrict__ Fr,
+                                                  const float *__restrict__ nutF, float *__restrict__ dt, float *__restrict__ src,
+                                                  float *__restrict__ mfxo, int mode, const float *__restrict__ cap) {
+  const int W = v.W, N = v.N;
+  const int k = blockIdx.x * kB + threadIdx.x;
+  if (k >= N) return;
+  const int NXF = (W + 1) * v.H, NRF = W * (v.H + 1);
+  const int i = k % W, j = k / W;
+  if (v.solid[k]) {
+    if (mode == 0 || mode == 3)
+      for (int q = 0; q < 5; q++) Qout[q * N + k] = Qin[q * N + k];
+    return;
+  }
+  const Prim P = prim(Qin, N, k);
+  const float dx = v.dx[i], dr = v.dr[j];
+  if (mode == 0 || mode == 2) {
+    const float nu = fmaxf(nutF[j * W + i], nutF[(j + 1) * W + i]);
+    const float cs = sqrtf(P.g * P.p / P.r);
+    dt[k] = c.cfl * fmaxf(v.phi[k], 0.25f) / ((fabsf(P.u) + cs) / dx + (fabsf(P.v) + cs) / dr + 2.2f * nu / (dr * dr));
+    if (mode == 2) return;
+  }
+  const int fx0 = j * (W + 1) + i, fx1 = fx0 + 1, fr0 = j * W + i, fr1 = fr0 + W;
+  const float ax0 = v.rf[j], ax1 = v.rf[j + 1], vol = v.rc[j] * dr, ph = v.phi[k];
+  float R[5];
+  for (int q = 0; q < 5; q++)
+    R[q] = (-(Fx[q * NXF + fx1] - Fx[q * NXF + fx0]) / dx - (Fr[q * NRF + fr1] * ax1 - Fr[q * NRF + fr0] * ax0) / vol) / ph;
+  const float aw = i > 0 ? v.ae[k - 1] : 1.f, as = j > 0 ? v.an[k - W] : 0.f;
+  R[1] += P.p * (v.ae[k] - aw) / (dx * ph);
+  R[2] += P.p * (v.an[k] * ax1 - as * ax0) / (vol * ph);
+  const float mfx = 0.5f * (Fx[fx0] + Fx[fx1]);
+  float sk = 0.f;
+  const int id = v.comp[k];
+  if (id) {
+    const float rho = Qin[k], g = P.g, cp = g * kR / (g - 1.f);
+    const float T = P.p / (rho * kR), Tt = T + 0.5f * (P.u * P.u + P.v * P.v) / cp;
+    const float ty = c.comps[id * 4 + 2], mf = mfx / ph;
+    if (ty == 1.f || ty == 2.f) {     // blade row: tau^(wgt dx) on the upstream face enthalpy
+      const float e = c.comps[id * 4 + 1], fac = ty == 1.f ? 1.f : clampf(mf / (0.3f * v.mfd[k]), 0.f, 1.f);
+      const float mw = Fx[fx0], HU = mw > 1e-9f ? Fx[3 * NXF + fx0] / mw : cp * Tt;
+      const float w = (HU > 0.f ? HU : cp * Tt) * expm1f(c.comps[id * 4] * v.wgt[k] * fac * dx) / dx;
+      R[1] += ty == 1.f ? e * rho * w : rho * w / e;
+      R[3] += mf * w;
+      sk = mf * w;
+    } else if (v.rate[k] > 0.f) {      // heat source relaxed to the target total temperature
+      const float K = c.comps[id * 4 + 1] != 0.f ? c.comps[id * 4 + 1] : 1.f;
+      const float tgt = c.comps[id * 4] + (v.tgt[k] - c.comps[id * 4]) * K;
+      const float m0 = fmaxf(mf, 0.f);
+      const float q = m0 * cp * fmaxf(0.f, tgt - Tt) * v.rate[k];
+      R[3] += q;
+      const float mfuel = q / c.hf;
+      R[0] += mfuel;
+      sk = mfuel;
+      R[4] += m0 * (1.f - P.Y) * v.yrate[k] + mfuel;
+    }
+  }
+  src[k] = sk; mfxo[k] = mfx;
+  float d = dt[k];
+  if (mode == 3) d = fminf(d, *cap);
+  if (mode == 3) dt[k] = d;
+  float n[5];
+  if (mode == 1) for (int q = 0; q < 5; q++) n[q] = 0.5f * Qb[q * N + k] + 0.5f * (Qin[q * N + k] + d * R[q]);
+  else for (int q = 0; q < 5; q++) n[q] = Qin[q * N + k] + d * R[q];
+  // floors (CPUSolver.fix, src/cfd_cpu.js:187-192)
+  const float r = fmaxf(n[0], 1e-3f);
+  n[0] = r;
+  const float Y = clampf(n[4] / r, 0.f, 1.f);
+  n[4] = Y * r;
+  const float g = 1.4f - 0.07f * Y, ke = 0.5f * (n[1] * n[1] + n[2] * n[2]) / r;
+  if ((g - 1.f) * (n[3] - ke) < 50.f) n[3] = 50.f / (g - 1.f) + ke;
+  for (int q = 0; q < 5; q++) Qout[q * N + k] = n[q];
+}
+
+// Minimum dt over fluid cells (CPUSolver.step global / dtRatio cap, src/cfd_cpu.js:198-202).
+__global__ void __launch_bounds__(kB) dtmin_kernel(const float *__restrict__ dt, const uint8_t *__restrict__ solid, int N,
+                                                   float *__restrict__ partial) {
+  __shared__ float lds[kB / 64];
+  float m = INFINITY;
+  for (int k = blockIdx.x * kB + threadIdx.x; k < N; k += gridDim.x * kB)
+    if (!solid[k]) m = fminf(m, dt[k]);
+  for (int off = 32; off > 0; off >>= 1) m = fminf(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float a = lds[0];
+    for (int w = 1; w < kB / 64; w++) a = fminf(a, lds[w]);
+    partial[blockIdx.x] = a;
+  }
+}
+__global__ void dtcap_kernel(const float *__restrict__ partial, int nb, float factor, float *__restrict__ cap) {
+  if (threadIdx.x != 0) return;
+  float a = INFINITY;
+  for (int b = 0; b < nb; b++) a = fminf(a, partial[b]);
+  *cap = a * factor;
+}
+
+}  // namespace axik

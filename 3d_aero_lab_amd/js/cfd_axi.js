@@ -1,0 +1,62 @@
+'use strict';
+/* ============================================================================
+ * cfd_axi.js — AxiHIP: the axisymmetric engine solver object of the
+ * reference's protocol (GPUSolver / CPUSolver, src/cfd_gpu.js:207-323,
+ * src/cfd_cpu.js:50-214) on one MI355X through the Node-API addon.
+ *
+ *   const s = new AxiHIP(grid, zones, comps, fs, Q0, opt);   // throws when no device
+ *   s.step(n); const Q = s.read();   // 5 Float32Array planes; s.src, s.mfx diagnostics
+ *   s.setComps(c); s.setZones(z); s.setFs(fs); s.setState(Q); s.finish(); s.dispose();
+ * grid = {W, H, xf, rf, ...} (buildCFDGrid, src/cfd_grid.js:28-43; axiGrid below
+ * derives xc, rc, dx, dr from the faces), zones = buildZones' typed arrays.
+ * ========================================================================== */
+const { loadAddon } = require('./hip_solver.js');
+
+/* Grid record from Float64 faces (src/cfd_grid.js:36-42). */
+function axiGrid(xf, rf, level) {
+  const W = xf.length - 1, H = rf.length - 1;
+  const xc = new Float64Array(W), dx = new Float64Array(W), rc = new Float64Array(H), dr = new Float64Array(H);
+  for (let i = 0; i < W; i++) { xc[i] = 0.5 * (xf[i] + xf[i + 1]); dx[i] = xf[i + 1] - xf[i]; }
+  for (let j = 0; j < H; j++) { rc[j] = 0.5 * (rf[j] + rf[j + 1]); dr[j] = rf[j + 1] - rf[j]; }
+  return { W, H, N: W * H, xf: Float64Array.from(xf), rf: Float64Array.from(rf), xc, rc, dx, dr, level };
+}
+
+const flat5 = (Q, N) => { const b = new Float64Array(5 * N); for (let q = 0; q < 5; q++) b.set(Q[q], q * N); return b; };
+
+class AxiHIP {
+  constructor(grid, zones, comps, fs, Q0, opt) {
+    this.kind = 'hip';
+    this.opt = Object.assign({ cfl: 0.6, order: 2, visc: true, hf: 0.99 * 43e6 }, opt || {});   // src/cfd_gpu.js:208
+    this.grid = grid; this.W = grid.W; this.H = grid.H; this.iter = 0;
+    const A = loadAddon();
+    this.h = A.axiCreate(Float64Array.from(grid.xf), Float64Array.from(grid.rf), zones, Float64Array.from(comps),
+      Float64Array.from([fs.r, fs.u, fs.p]), flat5(Q0, grid.W * grid.H));
+    this.z = zones; this.comps = Float64Array.from(comps); this.fs = fs;
+    this._opt = null;
+    const N = grid.W * grid.H;
+    this._buf = new Float32Array(5 * N); this.src = new Float32Array(N); this.mfx = new Float32Array(N);
+  }
+  _syncOpt() {
+    const o = this.opt, key = [o.cfl, o.order, o.visc, o.hf, o.global, o.dtRatio].join(',');
+    if (key !== this._opt) {
+      loadAddon().axiSetOpt(this.h, o.cfl, o.order, o.visc ? 1 : 0, o.hf || 0, o.global ? 1 : 0, o.dtRatio || 0);
+      this._opt = key;
+    }
+  }
+  setComps(c) { this.comps = Float64Array.from(c); loadAddon().axiSetComps(this.h, this.comps); }
+  setZones(z) { this.z = z; loadAddon().axiSetZones(this.h, z, this.W * this.H); }
+  setFs(fs) { this.fs = fs; loadAddon().axiSetFs(this.h, Float64Array.from([fs.r, fs.u, fs.p])); }
+  setState(Q) { loadAddon().axiSetState(this.h, flat5(Q, this.W * this.H)); }
+  step(n) { this._syncOpt(); loadAddon().axiStep(this.h, n || 1); this.iter += n || 1; }
+  read(Q) {
+    const N = this.W * this.H;
+    loadAddon().axiRead(this.h, this._buf, this.src, this.mfx);
+    if (!Q) Q = [0, 1, 2, 3, 4].map(() => new Float32Array(N));
+    for (let q = 0; q < 5; q++) Q[q].set(this._buf.subarray(q * N, (q + 1) * N));
+    return Q;
+  }
+  finish() { loadAddon().axiSync(this.h); }
+  dispose() { if (this.h) { loadAddon().axiDispose(this.h); this.h = null; } }
+}
+
+module.exports = { AxiHIP, axiGrid };

@@ -1,0 +1,63 @@
+/* ============================================================================
+ * axi.h — C ABI of the MI355X axisymmetric engine solver (part of libaero3d.so).
+ *
+ * Replaces the reference's GPUSolver / CPUSolver (src/cfd_gpu.js:207-323,
+ * src/cfd_cpu.js:50-214): 5 conserved variables [rho, rho u, rho v, E, rho Y]
+ * on a W x H (x, r) cut-cell grid, variable gamma = 1.4 - 0.07 Y, HLLC +
+ * MUSCL-minmod, Riemann-invariant farfield, mixing-length r fluxes, blade-row
+ * body forces, combustor heat source, SSP-RK2 with local dt. FP32 state on the
+ * device. Conventions as aero3d.h (0 = ok, < 0 = error, axi_last_error()).
+ * ========================================================================== */
+#ifndef AXI_H
+#define AXI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct axi_solver axi_solver;
+
+/* Grid of buildCFDGrid (src/cfd_grid.js:28-43): W x H cells, Float64 faces. */
+typedef struct {
+  int W, H;
+  const double *xf, *rf;
+} axi_grid;
+
+/* Zones of buildZones (src/cfd_grid.js:81-191), N = W*H entries each (row j, column i at j*W+i). */
+typedef struct {
+  const uint8_t *solid, *comp;
+  const float *phi, *ae, *an, *wgt, *lmix, *tgt, *rate, *yrate, *mfd;
+} axi_zones;
+
+/* new GPUSolver(grid, zones, comps, fs, Q0, opt) (src/cfd_gpu.js:207): comps =
+ * 32 doubles (compParams, src/cfd_grid.js:260-276), fs = {r, u, p} freestream,
+ * Q0 = 5 planes of N doubles. */
+int axi_create(const axi_grid *g, const axi_zones *z, const double comps[32], const double fs[3], const double *Q0,
+               axi_solver **out);
+int axi_destroy(axi_solver *s);
+const char *axi_last_error(const axi_solver *s);
+/* opt (src/cfd_cpu.js:53, 63, 198-202): cfl, order (1|2), visc (0|1), hf heating
+ * value (<= 0 keeps 0.99 * 43e6), global (0|1) and dtRatio (0 = off) dt caps. */
+int axi_set_opt(axi_solver *s, double cfl, int order, int visc, double hf, int global, double dtRatio);
+/* setComps / setZones / setFs (src/cfd_gpu.js:252-265). */
+int axi_set_comps(axi_solver *s, const double comps[32]);
+int axi_set_zones(axi_solver *s, const axi_zones *z);
+int axi_set_fs(axi_solver *s, const double fs[3]);
+/* setState(Q) (src/cfd_gpu.js:266-274): 5 planes of N doubles. */
+int axi_set_state(axi_solver *s, const double *Q);
+/* step(n) (src/cfd_cpu.js:193-210), enqueued on the solver's stream. */
+int axi_step(axi_solver *s, int n);
+int axi_iter(const axi_solver *s, int *it);
+/* read() (src/cfd_gpu.js:303-316): Q as 5 planes of N floats, and the
+ * diagnostics src (blade-row energy or fuel source per fluid volume) and mfx
+ * (mean axial face mass flux) of the last stage. Any pointer may be null. Waits. */
+int axi_read(axi_solver *s, float *Q, float *src, float *mfx, float *dt);
+int axi_sync(axi_solver *s);
+/* n iterations bracketed by HIP events on the solver's stream. */
+int axi_time_steps(axi_solver *s, int n, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -18,6 +18,11 @@
 struct emul_dim3 { int x = 0, y = 0, z = 0; };
 extern thread_local emul_dim3 threadIdx;
 extern thread_local emul_dim3 blockIdx;
+inline emul_dim3 emul_grid_dim() { emul_dim3 d; d.x = 1; d.y = 1; d.z = 1; return d; }
+#define gridDim emul_grid_dim()
+// one OS thread per GPU thread: a wavefront shuffle has no CPU meaning here; the
+// kernels that use it are not run by the CPU harness
+template <class T> inline T __shfl_down(T v, int, int) { return v; }
 extern thread_local std::barrier<> *emul_block_barrier;
 inline void __syncthreads() { emul_block_barrier->arrive_and_wait(); }
 inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }

@@ -1,0 +1,145 @@
+'use strict';
+/* ============================================================================
+ * make_golden_axi.js — golden vectors of the reference's axisymmetric engine
+ * solver (CPUSolver, src/cfd_cpu.js:50-210) and its post-processing
+ * (cfdForces / cfdSection / cfdControl, src/cfd_post.js:44-123), by running the
+ * reference's JavaScript (read-only checkout, default /root/reference, override
+ * with AERO_REF) under Node.
+ *
+ *   node tests/golden/make_golden_axi.js [case ...]     cases: xcoarse medium longrun
+ *
+ * The reference's own harness (tools/cfdlib.js:6-20) uses `??`, which this
+ * Node (v12) cannot parse; the same setup sequence is written out here:
+ * solveEngine -> buildSpec -> FieldModel -> buildCFDGrid -> buildZones ->
+ * compParams -> initialState -> CPUSolver (cfl 0.6, order 2, visc, hf = etb LHV).
+ * Outputs (data only): grid faces, zones, comps, freestream, initial state,
+ * states / diagnostics after N iterations, thrust and control values.
+ * ========================================================================== */
+const fs = require('fs');
+const path = require('path');
+const vm = require('vm');
+const zlib = require('zlib');
+const crypto = require('crypto');
+
+const REF = process.env.AERO_REF || '/root/reference';
+const code = ['physics', 'geometry', 'field', 'cfd_grid', 'cfd_cpu', 'cfd_post']
+  .map(f => fs.readFileSync(path.join(REF, 'src', f + '.js'), 'utf8').replace(/^if \(typeof module[^\n]*\n/gm, '')).join('\n');
+vm.runInThisContext(code + '\nglobalThis.AXI={cfdControl,solveEngine,ENGINE_TYPES,DEFAULT_EFF,buildSpec,FieldModel,buildCFDGrid,buildZones,compParams,initialState,CPUSolver,cfdForces,cfdSection,spoolRatios,RGAS};');
+const A = globalThis.AXI;
+const OUT = __dirname;
+
+function setupTJ(level) {
+  const type = 'tj', T = A.ENGINE_TYPES[type];
+  const P = { type, design: Object.assign({}, T.design), eff: Object.assign({}, A.DEFAULT_EFF), alt: 0, M0: 0.25, thr: 1, ab: false, Tt7: 2000, nozzleType: T.nozzle };
+  const sol = A.solveEngine(P), op = sol.op, fl = sol.fl, nz = op.noz;
+  const spec = A.buildSpec(type, 1, { r8: Math.sqrt(nz.Athroat / Math.PI), r9: Math.sqrt(Math.max(nz.A, nz.Athroat) / Math.PI), A9: sol.ref.noz && sol.ref.noz.A, A19: sol.ref.nozB && sol.ref.nozB.A });
+  const model = new A.FieldModel(spec, sol, P);
+  const grid = A.buildCFDGrid(spec, level);
+  const zones = A.buildZones(grid, spec, model, op, P);
+  const comps = A.compParams(type, op, P);
+  const Q0 = A.initialState(grid, zones, spec, model, fl, false);
+  const fsv = { r: fl.P0 / (A.RGAS * fl.T0), u: fl.V0, p: fl.P0 };
+  const hf = P.eff.etb * P.eff.LHV;
+  const S = new A.CPUSolver(grid, zones, comps, fsv, Q0, { cfl: 0.6, order: 2, visc: true, hf });
+  return { P, sol, op, fl, spec, grid, zones, comps, base: Float64Array.from(comps), Q0, fsv, hf, S, type };
+}
+
+function sha(buf) { return crypto.createHash('sha256').update(buf).digest('hex'); }
+class Writer {
+  constructor(name) { this.dir = path.join(OUT, name); fs.mkdirSync(this.dir, { recursive: true }); this.m = { case: name, arrays: {}, scalars: {} }; }
+  arr(name, ta, hashOnly) {
+    const buf = Buffer.from(ta.buffer, ta.byteOffset, ta.byteLength);
+    const dtype = ta instanceof Float32Array ? 'f4' : ta instanceof Float64Array ? 'f8' : ta instanceof Uint8Array ? 'u1' : ta instanceof Int8Array ? 'i1' : '?';
+    let file = null;
+    if (!hashOnly) { file = name + '.bin.gz'; fs.writeFileSync(path.join(this.dir, file), zlib.gzipSync(buf, { level: 9 })); }
+    this.m.arrays[name] = { file, dtype, n: ta.length, sha256: sha(buf) };
+  }
+  set(k, v) { this.m.scalars[k] = v; }
+  close() { fs.writeFileSync(path.join(this.dir, 'manifest.json'), JSON.stringify(this.m, null, 1)); console.log(`wrote ${this.dir}`); }
+}
+const ZONE_KEYS = ['solid', 'phi', 'ae', 'an', 'comp', 'wgt', 'lmix', 'tgt', 'rate', 'yrate', 'mfd', 'stream'];
+
+function dumpSetup(w, C, q0hash) {
+  const g = C.grid;
+  w.arr('xf', g.xf); w.arr('rf', g.rf);
+  for (const k of ZONE_KEYS) w.arr('z_' + k, C.zones[k]);
+  w.arr('comps', C.comps);
+  for (let q = 0; q < 5; q++) w.arr('Q0_' + q, C.Q0[q], q0hash);
+  w.set('W', g.W); w.set('H', g.H); w.set('N', g.N);
+  w.set('fs', C.fsv); w.set('hf', C.hf); w.set('opt', { cfl: 0.6, order: 2, visc: true });
+  // what cfdControl reads from spec / op (src/cfd_post.js:110-123)
+  w.set('ctl', { s: C.spec.s, comb: { x1: C.spec.comb.x1, stream: C.spec.comb.stream }, ab: C.spec.ab ? { x1: C.spec.ab.x1, stream: C.spec.ab.stream } : null,
+    order: C.spec.order, Tt4: C.op.Tt4, Tt7: C.op.Tt7 || null, abOn: !!C.op.abOn, etam: C.P.eff.etam, P0: C.fl.P0, F1D: C.op.F });
+}
+function dumpState(w, tag, C, hashOnly) {
+  const S = C.S;
+  for (let q = 0; q < 5; q++) w.arr(`${tag}_Q${q}`, S.Q[q], hashOnly);
+  w.arr(`${tag}_src`, S.src, hashOnly); w.arr(`${tag}_mfx`, S.mfx, hashOnly); w.arr(`${tag}_dt`, S.dt, hashOnly);
+}
+function thrust(C) {
+  const f = A.cfdForces(C.grid, C.zones, C.S.Q, C.comps, C.fl.P0, C.hf, { S: C.S.src, F: C.S.mfx });
+  return { F: f.F, Fb: f.Fb, Fw: f.Fw, mfuel: f.mfuel, pw: Array.from(f.pw) };
+}
+
+function caseXcoarse() {
+  const w = new Writer('axi_tj_xcoarse');
+  const C = setupTJ('xcoarse');
+  dumpSetup(w, C, false);
+  const t0 = Date.now();
+  let it = 0;
+  const hist = [];
+  for (const target of [1, 10, 100, 250, 500]) {
+    C.S.step(target - it); it = target;
+    dumpState(w, 'it' + it, C, !(it === 10 || it === 100 || it === 500));
+    hist.push({ it, ...thrust(C) });
+  }
+  w.set('ms_per_iter', (Date.now() - t0) / it);
+  w.set('hist', hist);
+  // control step at it 500 (the long-run loop calls it every 250 iterations)
+  const f0 = A.cfdForces(C.grid, C.zones, C.S.Q, C.comps, C.fl.P0, 0, { S: C.S.src, F: C.S.mfx });
+  const ctl = A.cfdControl(C.type, C.spec, C.grid, C.zones, C.S.Q, C.S.mfx, C.comps, C.base, f0.pw, C.op, C.P.eff.etam, 250);
+  w.set('control', { Tt4: ctl.Tt4, Tt7: ctl.Tt7, ratios: ctl.ratios, comps_after: Array.from(C.comps) });
+  const sec = A.cfdSection(C.grid, C.zones, C.S.Q, C.spec.comb.x1 - 0.004 * C.spec.s, C.spec.order.indexOf(C.spec.comb.stream), C.S.mfx);
+  w.set('section', sec);
+  w.close();
+}
+
+function caseMedium() {
+  const w = new Writer('axi_tj_medium');
+  const C = setupTJ('medium');
+  dumpSetup(w, C, false);
+  const t0 = Date.now();
+  C.S.step(2);
+  w.set('ms_per_iter', (Date.now() - t0) / 2);
+  dumpState(w, 'it2', C, true);
+  w.set('hist', [{ it: 2, ...thrust(C) }]);
+  w.close();
+}
+
+/* README.md:69-73 run: FRESH=1 node tools/longrun.js tj medium 14000 2000 (control every 250 iterations). */
+function caseLongrun() {
+  const iters = +(process.env.AXI_LONG_ITERS || 14000), every = 2000;
+  const w = new Writer('axi_tj_medium_longrun');
+  const C = setupTJ('medium');
+  const hist = [];
+  const t0 = Date.now();
+  for (let n = 0; n < iters; n += every) {
+    for (let m = 0; m < every; m += 250) {
+      C.S.step(250);
+      const f0 = A.cfdForces(C.grid, C.zones, C.S.Q, C.comps, C.fl.P0, 0, { S: C.S.src, F: C.S.mfx });
+      A.cfdControl(C.type, C.spec, C.grid, C.zones, C.S.Q, C.S.mfx, C.comps, C.base, f0.pw, C.op, C.P.eff.etam, 250);
+    }
+    const h = { it: n + every, ...thrust(C), K4: C.comps[17], s: (Date.now() - t0) / 1000 };
+    hist.push(h);
+    console.log(JSON.stringify(h));
+    w.set('hist', hist);
+    w.m.arrays = {};
+    for (let q = 0; q < 5; q++) w.arr(`Q${q}`, C.S.Q[q], true);
+    w.arr('comps', C.comps);
+    w.close();
+  }
+}
+
+const want = process.argv.slice(2);
+const all = { xcoarse: caseXcoarse, medium: caseMedium, longrun: caseLongrun };
+for (const k of (want.length ? want : ['xcoarse', 'medium'])) { const t0 = Date.now(); all[k](); console.log(`${k}: ${((Date.now() - t0) / 1000).toFixed(1)} s`); }

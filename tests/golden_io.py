@@ -7,7 +7,7 @@ import os
 import numpy as np
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-_DT = {"f4": np.float32, "f8": np.float64, "u1": np.uint8, "i4": np.int32}
+_DT = {"f4": np.float32, "f8": np.float64, "u1": np.uint8, "i1": np.int8, "i4": np.int32}
 
 
 class Golden:

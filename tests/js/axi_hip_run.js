@@ -1,0 +1,43 @@
+'use strict';
+/* The reference's long-run flow (tools/longrun.js:7-12) on AxiHIP: step in
+ * chunks of 250, thrust from the solver's own sources after each chunk, and
+ * (with ctl=1) the exit-temperature control every 250 iterations.
+ *   node tests/js/axi_hip_run.js <golden dir> <iters> <report every> <ctl 0|1>  */
+const fs = require('fs'), path = require('path'), zlib = require('zlib');
+const PKG = path.join(__dirname, '..', '..', '3d_aero_lab_amd', 'js');
+const P = require(path.join(PKG, 'cfd_post.js'));
+const { AxiHIP, axiGrid } = require(path.join(PKG, 'cfd_axi.js'));
+const [dir, itersS, everyS, ctlS] = process.argv.slice(2);
+const iters = +itersS, every = +everyS, ctlOn = +ctlS;
+const m = JSON.parse(fs.readFileSync(path.join(dir, 'manifest.json'), 'utf8'));
+const T = { f4: Float32Array, f8: Float64Array, u1: Uint8Array, i1: Int8Array };
+const arr = n => { const a = m.arrays[n]; const b = zlib.gunzipSync(fs.readFileSync(path.join(dir, a.file))); return new T[a.dtype](b.buffer.slice(b.byteOffset, b.byteOffset + b.byteLength)); };
+const s = m.scalars;
+const grid = axiGrid(arr('xf'), arr('rf'));
+const zones = {};
+for (const k of ['solid', 'phi', 'ae', 'an', 'comp', 'wgt', 'lmix', 'tgt', 'rate', 'yrate', 'mfd', 'stream']) zones[k] = arr('z_' + k);
+const comps = Float64Array.from(arr('comps'));
+const Q0 = [0, 1, 2, 3, 4].map(q => arr(`Q0_${q}`));
+const S = new AxiHIP(grid, zones, comps, s.fs, Q0, Object.assign({}, s.opt, { hf: s.hf }));
+const hist = [];
+const t0 = Date.now();
+let it = 0;
+while (it < iters) {
+  const n = Math.min(250, iters - it);
+  S.step(n); it += n;
+  let Q = null;
+  if (ctlOn) {
+    Q = S.read();
+    const f0 = P.cfdForces(grid, zones, Q, comps, s.ctl.P0, 0, { S: S.src, F: S.mfx });
+    P.cfdControl('tj', s.ctl, grid, zones, Q, S.mfx, comps, f0.pw, s.ctl.etam, 250);
+    S.setComps(comps);
+  }
+  if (it % every === 0 || it === iters) {
+    if (!Q) Q = S.read();
+    const f = P.cfdForces(grid, zones, Q, comps, s.ctl.P0, s.hf, { S: S.src, F: S.mfx });
+    hist.push({ it, F: f.F, Fb: f.Fb, Fw: f.Fw, mfuel: f.mfuel, K4: comps[17], s: (Date.now() - t0) / 1000 });
+    console.error(JSON.stringify(hist[hist.length - 1]));
+  }
+}
+S.dispose();
+console.log(JSON.stringify({ hist }));

@@ -11,8 +11,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def header_symbols():
-    txt = open(os.path.join(ROOT, "include", "aero3d.h")).read()
-    return sorted(set(re.findall(r"\b(aero3d_[a-z_0-9]+)\s*\(", txt)))
+    txt = open(os.path.join(ROOT, "include", "aero3d.h")).read() + open(os.path.join(ROOT, "include", "axi.h")).read()
+    return sorted(set(re.findall(r"\b((?:aero3d|axi)_[a-z_0-9]+)\s*\(", txt)))
 
 
 def test_library_exports_header_symbols():

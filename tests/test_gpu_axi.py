@@ -1,0 +1,65 @@
+"""HIP axisymmetric solver against the axisymmetric oracle and the reference's
+golden vectors (TJ, sea level, M0.25, 100 % throttle).
+
+Bars: conserved variables within 1e-5 relative per variable scale (the
+reference's GPU/CPU convention, tools/gpucmp.mjs:19-21) after 10-100
+iterations; thrust within 1e-4 relative of the reference's after 100 and 500
+iterations (FP32 device arithmetic against JS doubles).
+"""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_axi as OA
+from axi_case import load, rel_diff
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def say(*a):
+    print("[gpu-axi]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("case,iters,opt", [("axi_tj_xcoarse", (10, 100), {}), ("axi_tj_medium", (20,), {}),
+                                             ("axi_tj_xcoarse", (5,), {"global": True}), ("axi_tj_xcoarse", (5,), {"dtRatio": 3.0}),
+                                             ("axi_tj_xcoarse", (10,), {"order": 1, "visc": False})])
+def test_axi_matches_oracle(case, iters, opt):
+    import aero3d
+    g, z, Q0, comps, xf, rf = load(case)
+    s = g.s
+    o_opt = dict(s["opt"], hf=s["hf"], **opt)
+    say(case, iters, opt)
+    h = aero3d.AxiHIP(xf, rf, z, comps, s["fs"], Q0, o_opt)
+    o = OA.OracleAxi(s["W"], s["H"], xf, rf, z, comps, s["fs"], list(Q0), o_opt)
+    done = 0
+    for it in iters:
+        h.step(it - done); o.step(it - done); done = it
+        Q, src, mfx, dt = h.read()
+        Qo, so, mo, dto = o.read()
+        d = rel_diff(Q, Qo, z["solid"])
+        fl = z["solid"] == 0
+        d["mfx"] = float(np.abs(mfx[fl] - mo[fl]).max() / np.abs(mo[fl]).max())
+        say(case, it, d)
+        assert max(d.values()) < 1e-5, d
+    assert h.iter == iters[-1]
+    h.dispose()
+
+
+def test_axi_thrust_matches_reference():
+    g, *_ = load("axi_tj_xcoarse")
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "axi_hip_run.js"), g.dir, "500", "100", "0"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    hist = {h["it"]: h for h in json.loads(r.stdout.strip().splitlines()[-1])["hist"]}
+    for ref in g.s["hist"]:
+        if ref["it"] in (100, 500):
+            h = hist[ref["it"]]
+            say("thrust", ref["it"], h["F"], ref["F"])
+            assert abs(h["F"] - ref["F"]) / abs(ref["F"]) < 1e-4
+            assert abs(h["mfuel"] - ref["mfuel"]) / abs(ref["mfuel"]) < 1e-4
