@@ -1,0 +1,60 @@
+'use strict';
+/* ============================================================================
+ * axi_run.js — the reference's long axisymmetric run (tools/longrun.js:2-14)
+ * on AxiHIP: n iterations in chunks of 250 with the exit-temperature control
+ * after every chunk, thrust and station averages at the end.
+ *
+ *   node axi_run.js <case dir> <iters> [report every]
+ * <case dir> holds the solver inputs (grid faces xf / rf, zones z_*, comps,
+ * initial state Q0_*, freestream, control fields) as written by
+ * tests/golden/make_golden_axi.js from the reference's setup.
+ * Prints one JSON line: {hist, thrust, stations, ms_per_iter_device_wall}.
+ * ========================================================================== */
+const fs = require('fs'), path = require('path'), zlib = require('zlib');
+const P = require('./cfd_post.js');
+const { AxiHIP, axiGrid } = require('./cfd_axi.js');
+
+function loadCase(dir) {
+  const m = JSON.parse(fs.readFileSync(path.join(dir, 'manifest.json'), 'utf8'));
+  const T = { f4: Float32Array, f8: Float64Array, u1: Uint8Array, i1: Int8Array };
+  const arr = n => { const a = m.arrays[n]; const b = zlib.gunzipSync(fs.readFileSync(path.join(dir, a.file))); return new T[a.dtype](b.buffer.slice(b.byteOffset, b.byteOffset + b.byteLength)); };
+  const zones = {};
+  for (const k of ['solid', 'phi', 'ae', 'an', 'comp', 'wgt', 'lmix', 'tgt', 'rate', 'yrate', 'mfd', 'stream']) zones[k] = arr('z_' + k);
+  return { s: m.scalars, grid: axiGrid(arr('xf'), arr('rf')), zones, comps: Float64Array.from(arr('comps')), Q0: [0, 1, 2, 3, 4].map(q => arr(`Q0_${q}`)) };
+}
+
+function run(dir, iters, every) {
+  const C = loadCase(dir), s = C.s, comps = C.comps;
+  const S = new AxiHIP(C.grid, C.zones, comps, s.fs, C.Q0, Object.assign({}, s.opt, { hf: s.hf }));
+  const hist = [];
+  let it = 0, Q = null;
+  const t0 = process.hrtime.bigint();
+  let tStep = 0n;
+  while (it < iters) {
+    const n = Math.min(250, iters - it);
+    const a = process.hrtime.bigint();
+    S.step(n); S.finish();
+    tStep += process.hrtime.bigint() - a;
+    it += n;
+    Q = S.read(Q);
+    const f0 = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, 0, { S: S.src, F: S.mfx });
+    P.cfdControl('tj', s.ctl, C.grid, C.zones, Q, S.mfx, comps, f0.pw, s.ctl.etam, 250);
+    S.setComps(comps);
+    if (it % every === 0 || it >= iters) {
+      const f = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, s.hf, { S: S.src, F: S.mfx });
+      hist.push({ it, F: f.F, Fb: f.Fb, Fw: f.Fw, mfuel: f.mfuel, K4: comps[17] });
+    }
+  }
+  const wall = Number(process.hrtime.bigint() - t0) / 1e9;
+  const f = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, s.hf, { S: S.src, F: S.mfx });
+  const st = P.stations(C.grid, C.zones, Q, S.mfx, s.ctl.order);
+  S.dispose();
+  return { cells: C.grid.N, iters, hist, thrust: { F: f.F, Fb: f.Fb, Fw: f.Fw, mfuel: f.mfuel }, F1D: s.ctl.F1D, stations: st,
+    wall_s: wall, step_s: Number(tStep) / 1e9 };
+}
+
+if (require.main === module) {
+  const [dir, iters, every] = process.argv.slice(2);
+  console.log(JSON.stringify(run(dir, +iters, +(every || 2000))));
+}
+module.exports = { loadCase, run };

@@ -109,4 +109,27 @@ function cfdControl(type, ctl, grid, zones, Q, mfx, comps, pw, etam, dIt) {
   return out;
 }
 
-module.exports = { RGAS, clamp, cfdPrim, cfdForces, colAt, cfdSection, spoolRatios, cfdControl };
+/* Station averages of the turbojet flowpath (tools/stations.js:8-16): at each
+ * station column, mass flow and mass-weighted Tt, Pt over the station's
+ * stream, with the cell mass flow taken from the mean axial face mass flux. */
+const TJ_STATIONS = [['2', 0.36, 'core'], ['3', 1.55, 'core'], ['4', 2.21, 'core'], ['5', 2.62, 'core'], ['7', 4.29, 'core'], ['9', 4.89, 'core']];
+function stations(grid, zones, Q, mfx, order, list) {
+  const { W, H, rc, dr } = grid, out = [];
+  for (const [id, x, sn] of (list || TJ_STATIONS)) {
+    const si = order.indexOf(sn), i = colAt(grid, x);
+    let m = 0, tt = 0, pt = 0, wsum = 0;
+    for (let j = 0; j < H; j++) {
+      const k = j * W + i;
+      if (zones.solid[k] || zones.stream[k] !== si) continue;
+      const f = mfx[k] * 2 * Math.PI * rc[j] * dr[j];
+      m += f;
+      const q = cfdPrim(Q, k), cp = q.g * 287.05 / (q.g - 1), T = q.p / (q.r * 287.05), Tt = T + 0.5 * (q.u * q.u + q.v * q.v) / cp, Pt = q.p * Math.pow(Tt / T, q.g / (q.g - 1));
+      const wk = Math.max(f, 0);
+      tt += wk * Tt; pt += wk * Pt; wsum += wk;
+    }
+    out.push({ id, x, mdot: m, Tt: tt / wsum, Pt: pt / wsum });
+  }
+  return out;
+}
+
+module.exports = { RGAS, clamp, cfdPrim, cfdForces, colAt, cfdSection, spoolRatios, cfdControl, stations, TJ_STATIONS };

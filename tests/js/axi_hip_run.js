@@ -23,7 +23,7 @@ const hist = [];
 const t0 = Date.now();
 let it = 0;
 while (it < iters) {
-  const n = Math.min(250, iters - it);
+  const n = Math.min(ctlOn ? 250 : every, iters - it);
   S.step(n); it += n;
   let Q = null;
   if (ctlOn) {
