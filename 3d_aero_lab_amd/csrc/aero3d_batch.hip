@@ -159,14 +159,16 @@ struct aero3d_batch {
   hipStream_t stream = nullptr;
   std::vector<Geometry> geos;
   std::vector<Slot> slots;
-  TableHdr *d_hdr = nullptr;
-  SlotDev *d_table = nullptr;
+  TableHdr *d_hdr = nullptr;         // one header per case group
+  SlotDev *d_table = nullptr;        // max_slots entries per case group
   int max_slots = 0;
   int grid = 0;                      // resident blocks of the persistent kernel
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  std::vector<hipGraph_t> graphs;    // per case group: three iterations
+  std::vector<hipGraphExec_t> execs;
   bool table_dirty = true;
   std::vector<int> order;            // slot ids in table order
+  int ngroups = 0;
+  int64_t group_cells = 2000000;     // cases are stepped in groups of about this many cells
   int zc = 3;
   std::string err;
 };
@@ -193,15 +195,20 @@ void prim5h(const double *a, double *o) {
   o[4] = std::max((1.4 - 1) * (a[4] - 0.5 * r * (u * u + v * v + w * w)), 1e-7);
 }
 
-// Rewrite the device case table from the active slots (stream-ordered).
+// Rewrite the device case tables from the active slots. Cases are split into
+// groups of about group_cells cells (a group's state and masks stay in the
+// Infinity Cache across its stages and iterations); each group has its own
+// header and table region, stepped by its own launches.
 int upload_table(aero3d_batch *b) {
-  std::vector<SlotDev> tab;
   b->order.clear();
-  int item = 0;
+  std::vector<std::vector<SlotDev>> groups(1);
+  int64_t cells = 0;
   for (int s = 0; s < (int)b->slots.size(); s++) {
     Slot &S = b->slots[s];
     if (!S.used) continue;
     const Geometry &g = b->geos[S.geo];
+    if (!groups.back().empty() && cells + g.N > b->group_cells) { groups.emplace_back(); cells = 0; }
+    cells += g.N;
     SlotDev d;
     d.cv = CaseView{g.NX, g.NY, g.NZ, g.N, g.dx, g.dy, g.dz, g.t, g.phi, g.ap};
     d.kc = S.kc;
@@ -209,26 +216,37 @@ int upload_table(aero3d_batch *b) {
     d.zc = S.zc;
     d.ntx = (g.NX + 63) / 64; d.nty = (g.NY + kTY - 1) / kTY;
     d.nitems = d.ntx * d.nty * ((g.NZ + S.zc - 1) / S.zc);
-    d.item0 = item;
-    item += d.nitems;
-    tab.push_back(d);
+    d.item0 = 0;
+    groups.back().push_back(d);
     b->order.push_back(s);
   }
-  TableHdr h{(int)tab.size(), item};
-  if (!tab.empty()) BCHK(b, hipMemcpyAsync(b->d_table, tab.data(), sizeof(SlotDev) * tab.size(), hipMemcpyHostToDevice, b->stream));
-  BCHK(b, hipMemcpyAsync(b->d_hdr, &h, sizeof h, hipMemcpyHostToDevice, b->stream));
-  // the host copies above must stay alive until the copy runs: wait here (table changes are rare)
+  if (groups.back().empty()) groups.pop_back();
+  b->ngroups = (int)groups.size();
+  std::vector<TableHdr> hdrs(groups.size());
+  for (size_t gi = 0; gi < groups.size(); gi++) {
+    int item = 0;
+    for (auto &d : groups[gi]) { d.item0 = item; item += d.nitems; }
+    hdrs[gi] = TableHdr{(int)groups[gi].size(), item};
+    BCHK(b, hipMemcpyAsync(b->d_table + gi * (size_t)b->max_slots, groups[gi].data(), sizeof(SlotDev) * groups[gi].size(),
+                           hipMemcpyHostToDevice, b->stream));
+  }
+  if (!hdrs.empty()) BCHK(b, hipMemcpyAsync(b->d_hdr, hdrs.data(), sizeof(TableHdr) * hdrs.size(), hipMemcpyHostToDevice, b->stream));
+  // the host copies above must stay alive until the copies run: wait here (table changes are rare)
   BCHK(b, hipStreamSynchronize(b->stream));
   b->table_dirty = false;
   return 0;
 }
 
-int build_graph(aero3d_batch *b) {
+int build_graph(aero3d_batch *b, int gi) {
+  hipGraph_t g;
+  hipGraphExec_t e;
   BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   for (int ph = 0; ph < 3; ph++)
-    for (int st = 0; st < 2; st++) batch_kernel<<<b->grid, dim3(64, kTY), 0, b->stream>>>(b->d_hdr, b->d_table, ph, st);
-  BCHK(b, hipStreamEndCapture(b->stream, &b->graph));
-  BCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    for (int st = 0; st < 2; st++)
+      batch_kernel<<<b->grid, dim3(64, kTY), 0, b->stream>>>(b->d_hdr + gi, b->d_table + gi * (size_t)b->max_slots, ph, st);
+  BCHK(b, hipStreamEndCapture(b->stream, &g));
+  BCHK(b, hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+  b->graphs[gi] = g; b->execs[gi] = e;
   return 0;
 }
 
@@ -236,11 +254,15 @@ int batch_steps(aero3d_batch *b, int n) {
   int rc;
   if (b->table_dirty && (rc = upload_table(b))) return rc;
   if (b->order.empty() || n <= 0) return 0;
-  if (!b->exec && (rc = build_graph(b))) return rc;
-  for (int k = 0; k + 3 <= n; k += 3) BCHK(b, hipGraphLaunch(b->exec, b->stream));
+  if ((int)b->execs.size() < b->ngroups) { b->execs.resize(b->ngroups, nullptr); b->graphs.resize(b->ngroups, nullptr); }
   const int r = n % 3;
-  for (int ph = 0; ph < r; ph++)
-    for (int st = 0; st < 2; st++) batch_kernel<<<b->grid, dim3(64, kTY), 0, b->stream>>>(b->d_hdr, b->d_table, ph, st);
+  for (int gi = 0; gi < b->ngroups; gi++) {
+    if (n >= 3 && !b->execs[gi] && (rc = build_graph(b, gi))) return rc;
+    for (int k = 0; k + 3 <= n; k += 3) BCHK(b, hipGraphLaunch(b->execs[gi], b->stream));
+    for (int ph = 0; ph < r; ph++)
+      for (int st = 0; st < 2; st++)
+        batch_kernel<<<b->grid, dim3(64, kTY), 0, b->stream>>>(b->d_hdr + gi, b->d_table + gi * (size_t)b->max_slots, ph, st);
+  }
   BCHK(b, hipGetLastError());
   for (int s : b->order) { b->slots[s].cur = (b->slots[s].cur + 2 * r) % 3; b->slots[s].it += n; }
   if (r) b->table_dirty = true;      // buffer indices moved
@@ -293,8 +315,8 @@ int aero3d_batch_create(int64_t arena_bytes, int max_slots, aero3d_batch **out) 
   b->max_slots = max_slots;
   b->slots.resize(max_slots);
   if (!b->arena.init((size_t)arena_bytes) || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc((void **)&b->d_hdr, sizeof(TableHdr)) != hipSuccess ||
-      hipMalloc((void **)&b->d_table, sizeof(SlotDev) * max_slots) != hipSuccess) {
+      hipMalloc((void **)&b->d_hdr, sizeof(TableHdr) * max_slots) != hipSuccess ||
+      hipMalloc((void **)&b->d_table, sizeof(SlotDev) * (size_t)max_slots * max_slots) != hipSuccess) {
     aero3d_batch_destroy(b);
     return AERO3D_E_HIP_BASE;
   }
@@ -310,8 +332,8 @@ int aero3d_batch_create(int64_t arena_bytes, int max_slots, aero3d_batch **out) 
 int aero3d_batch_destroy(aero3d_batch *b) {
   if (!b) return 0;
   if (b->stream) hipStreamSynchronize(b->stream);
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
+  for (auto e : b->execs) if (e) (void)hipGraphExecDestroy(e);
+  for (auto g : b->graphs) if (g) (void)hipGraphDestroy(g);
   if (b->arena.base) hipFree(b->arena.base);
   if (b->d_hdr) hipFree(b->d_hdr);
   if (b->d_table) hipFree(b->d_table);
@@ -429,6 +451,13 @@ int aero3d_batch_set_zchunk(aero3d_batch *b, int zc) {
   if (!b || zc < 1) return AERO3D_E_ARG;
   b->zc = zc;
   for (auto &S : b->slots) if (S.used) S.zc = zc;
+  b->table_dirty = true;
+  return 0;
+}
+
+int aero3d_batch_set_group_cells(aero3d_batch *b, int64_t cells) {
+  if (!b || cells < 1) return AERO3D_E_ARG;
+  b->group_cells = cells;
   b->table_dirty = true;
   return 0;
 }

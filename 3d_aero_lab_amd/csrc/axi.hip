@@ -45,6 +45,11 @@ struct axi_solver {
   float *zf = nullptr;            // 9 zone planes: phi ae an wgt lmix tgt rate yrate mfd
   float *Q = nullptr, *Q1 = nullptr, *Fx = nullptr, *Fr = nullptr, *nutF = nullptr, *dt = nullptr, *src = nullptr, *mfx = nullptr;
   float *partial = nullptr, *cap = nullptr;
+  AxiConsts *dc = nullptr;        // device copy of the constants (read by every kernel)
+  bool dc_dirty = true;
+  hipGraph_t graph = nullptr;     // captured chunk of kGraphIters iterations
+  hipGraphExec_t exec = nullptr;
+  bool graph_capped = false;
   int global = 0;
   double dtRatio = 0;
   std::string err;
@@ -83,27 +88,59 @@ int upload_zones(axi_solver *s, const axi_zones *z) {
   return 0;
 }
 
-int steps(axi_solver *s, int n) {
+constexpr int kGraphIters = 25;
+
+void enqueue_iters(axi_solver *s, int n) {
   const AxiView v = view(s);
   const int NF = (s->W + 1) * s->H + s->W * (s->H + 1);
   const dim3 gf((NF + kB - 1) / kB), gc((s->N + kB - 1) / kB);
   const bool capped = s->global || s->dtRatio != 0;
   const int nbm = std::min(1024, (s->N + kB - 1) / kB);
+  const AxiConsts *c = s->dc;
   for (int k = 0; k < n; k++) {
-    face_kernel<<<gf, kB, 0, s->stream>>>(v, s->c, s->Q, s->Fx, s->Fr, s->nutF);
+    face_kernel<<<gf, kB, 0, s->stream>>>(v, c, s->Q, s->Fx, s->Fr, s->nutF);
     if (capped) {
-      cell_kernel<<<gc, kB, 0, s->stream>>>(v, s->c, s->Q, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 2, s->cap);
+      cell_kernel<<<gc, kB, 0, s->stream>>>(v, c, s->Q, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 2, s->cap);
       dtmin_kernel<<<nbm, kB, 0, s->stream>>>(s->dt, s->solid, s->N, s->partial);
       dtcap_kernel<<<1, 64, 0, s->stream>>>(s->partial, nbm, s->global ? 1.f : (float)s->dtRatio, s->cap);
-      cell_kernel<<<gc, kB, 0, s->stream>>>(v, s->c, s->Q, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 3, s->cap);
+      cell_kernel<<<gc, kB, 0, s->stream>>>(v, c, s->Q, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 3, s->cap);
     } else {
-      cell_kernel<<<gc, kB, 0, s->stream>>>(v, s->c, s->Q, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 0, s->cap);
+      cell_kernel<<<gc, kB, 0, s->stream>>>(v, c, s->Q, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 0, s->cap);
     }
-    face_kernel<<<gf, kB, 0, s->stream>>>(v, s->c, s->Q1, s->Fx, s->Fr, s->nutF);
-    cell_kernel<<<gc, kB, 0, s->stream>>>(v, s->c, s->Q1, s->Q, s->Q, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 1, s->cap);
-    s->it++;
+    face_kernel<<<gf, kB, 0, s->stream>>>(v, c, s->Q1, s->Fx, s->Fr, s->nutF);
+    cell_kernel<<<gc, kB, 0, s->stream>>>(v, c, s->Q1, s->Q, s->Q, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 1, s->cap);
   }
+}
+
+// n iterations: the constants go to the device first (stream-ordered), then
+// whole chunks replay a captured graph of kGraphIters iterations (the kernel
+// arguments never change: Q and Q1 keep their roles, constants are read from
+// device memory) and the remainder is launched directly.
+int steps(axi_solver *s, int n) {
+  if (s->dc_dirty) {
+    ACHK(s, hipMemcpyAsync(s->dc, &s->c, sizeof(AxiConsts), hipMemcpyHostToDevice, s->stream));
+    ACHK(s, hipStreamSynchronize(s->stream));   // the host copy of the constants may change after return
+    s->dc_dirty = false;
+  }
+  const bool capped = s->global || s->dtRatio != 0;
+  if (s->exec && s->graph_capped != capped) {
+    (void)hipGraphExecDestroy(s->exec); (void)hipGraphDestroy(s->graph);
+    s->exec = nullptr; s->graph = nullptr;
+  }
+  int k = 0;
+  if (n >= kGraphIters) {
+    if (!s->exec) {
+      ACHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_iters(s, kGraphIters);
+      ACHK(s, hipStreamEndCapture(s->stream, &s->graph));
+      ACHK(s, hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+      s->graph_capped = capped;
+    }
+    for (; k + kGraphIters <= n; k += kGraphIters) ACHK(s, hipGraphLaunch(s->exec, s->stream));
+  }
+  if (k < n) enqueue_iters(s, n - k);
   ACHK(s, hipGetLastError());
+  s->it += n;
   return 0;
 }
 
@@ -133,7 +170,7 @@ int axi_create(const axi_grid *g, const axi_zones *z, const double comps[32], co
   auto al = [&](auto **p, size_t n) { return hipMalloc((void **)p, sizeof(**p) * (n ? n : 1)); };
   if (al(&s->rc, s->H) || al(&s->dx, s->W) || al(&s->dr, s->H) || al(&s->rf, s->H + 1) || al(&s->solid, N) || al(&s->comp, N) ||
       al(&s->zf, 9 * N) || al(&s->Q, 5 * N) || al(&s->Q1, 5 * N) || al(&s->Fx, 5 * NXF) || al(&s->Fr, 5 * NRF) || al(&s->nutF, NRF) ||
-      al(&s->dt, N) || al(&s->src, N) || al(&s->mfx, N) || al(&s->partial, 1024) || al(&s->cap, 1))
+      al(&s->dt, N) || al(&s->src, N) || al(&s->mfx, N) || al(&s->partial, 1024) || al(&s->cap, 1) || al(&s->dc, 1))
     return bail(afail(s, -1002, "axi_create: hipMalloc failed"));
   if (hipMemcpy(s->rc, rc.data(), 4 * rc.size(), hipMemcpyHostToDevice) || hipMemcpy(s->dx, dx.data(), 4 * dx.size(), hipMemcpyHostToDevice) ||
       hipMemcpy(s->dr, dr.data(), 4 * dr.size(), hipMemcpyHostToDevice) || hipMemcpy(s->rf, rf.data(), 4 * rf.size(), hipMemcpyHostToDevice) ||
@@ -149,7 +186,9 @@ int axi_create(const axi_grid *g, const axi_zones *z, const double comps[32], co
 int axi_destroy(axi_solver *s) {
   if (!s) return 0;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  void *p[] = {s->rc, s->dx, s->dr, s->rf, s->solid, s->comp, s->zf, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, s->partial, s->cap};
+  if (s->exec) (void)hipGraphExecDestroy(s->exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  void *p[] = {s->rc, s->dx, s->dr, s->rf, s->solid, s->comp, s->zf, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, s->partial, s->cap, s->dc};
   for (void *x : p) if (x) (void)hipFree(x);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -163,11 +202,13 @@ int axi_set_opt(axi_solver *s, double cfl, int order, int visc, double hf, int g
   if (!(cfl > 0 && cfl < 10) || (order != 1 && order != 2) || dtRatio < 0) return afail(s, -1, "set_opt: cfl in (0,10), order 1|2, dtRatio >= 0");
   s->c.cfl = (float)cfl; s->c.order = order; s->c.visc = visc ? 1 : 0; s->c.hf = hf > 0 ? (float)hf : 0.99f * 43e6f;
   s->global = global ? 1 : 0; s->dtRatio = dtRatio;
+  s->dc_dirty = true;
   return 0;
 }
 int axi_set_comps(axi_solver *s, const double comps[32]) {
   if (!s || !comps) return -1;
   for (int q = 0; q < 32; q++) s->c.comps[q] = (float)comps[q];
+  s->dc_dirty = true;
   return 0;
 }
 int axi_set_zones(axi_solver *s, const axi_zones *z) {
@@ -178,6 +219,7 @@ int axi_set_zones(axi_solver *s, const axi_zones *z) {
 int axi_set_fs(axi_solver *s, const double fs[3]) {
   if (!s || !fs) return -1;
   s->c.fs_r = (float)fs[0]; s->c.fs_u = (float)fs[1]; s->c.fs_p = (float)fs[2];
+  s->dc_dirty = true;
   return 0;
 }
 int axi_set_state(axi_solver *s, const double *Q) {

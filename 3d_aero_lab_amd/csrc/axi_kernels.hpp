@@ -97,8 +97,9 @@ __device__ void recon(const Prim &A, const Prim *Am, const Prim &B, const Prim *
 }
 
 // One thread per face (src/cfd_cpu.js:97-143).
-__global__ void __launch_bounds__(kB) face_kernel(AxiView v, AxiConsts c, const float *__restrict__ Q, float *__restrict__ Fx,
+__global__ void __launch_bounds__(kB) face_kernel(AxiView v, const AxiConsts *__restrict__ cp, const float *__restrict__ Q, float *__restrict__ Fx,
                                                   float *__restrict__ Fr, float *__restrict__ nutF) {
+  const AxiConsts &c = *cp;
   const int W = v.W, H = v.H, N = v.N;
   const int NXF = (W + 1) * H, NRF = W * (H + 1);
   const int t = blockIdx.x * kB + threadIdx.x;
@@ -177,10 +178,11 @@ __global__ void __launch_bounds__(kB) face_kernel(AxiView v, AxiConsts c, const 
 // (dt from the state, Qout = Qin + dt R); mode 1: stage 2 (Qout = (Qb + Qin +
 // dt R) / 2); mode 2: stage 1, dt only (for the global / ratio cap); mode 3:
 // stage 1 update with the stored dt capped at *cap.
-__global__ void __launch_bounds__(kB) cell_kernel(AxiView v, AxiConsts c, const float *__restrict__ Qin, const float *Qb,
+__global__ void __launch_bounds__(kB) cell_kernel(AxiView v, const AxiConsts *__restrict__ cp, const float *__restrict__ Qin, const float *Qb,
                                                   float *Qout, const float *__restrict__ Fx, const float *__restrict__ Fr,
                                                   const float *__restrict__ nutF, float *__restrict__ dt, float *__restrict__ src,
                                                   float *__restrict__ mfxo, int mode, const float *__restrict__ cap) {
+  const AxiConsts &c = *cp;
   const int W = v.W, N = v.N;
   const int k = blockIdx.x * kB + threadIdx.x;
   if (k >= N) return;

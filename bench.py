@@ -105,12 +105,13 @@ def run_converged_plain(built, conds, res_drop, max_iters, check_every, tile):
     return time.perf_counter() - t0, iters
 
 
-def run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes, max_slots=256):
+def run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes, max_slots=256, group_cells=2_000_000):
     """Submit requests (geometry index, M, alpha, arrival s) to the continuous-batching server as their
     arrival times come; tick until all are done. Returns results and the wall time."""
     import aero3d
     srv = aero3d.Server(arena_bytes=arena_bytes, max_slots=max_slots, check_every=check_every)
     srv.batch.set_zchunk(zc)
+    srv.batch.set_group_cells(group_cells)
     gids = [srv.batch.add_geometry(g, geo, fl) for (g, geo, fl, meta) in built]
     t_start = srv.clock()
     pending = sorted(range(len(reqs)), key=lambda k: reqs[k][3])
@@ -143,31 +144,34 @@ def batched_leg(res_drop=1e-4, max_iters=2000, check_every=50):
     built = geometry.build_cases([w[0] for w in wl])
     conds = [(w[1], w[2]) for w in wl]
     cells = sum(b[3]["N"] for b in built)
-    # autotune the batch z chunk on the first eight cases (throwaway batch)
-    tb = aero3d.Batch(arena_bytes=4 << 30, max_slots=16)
-    for (g, geo, fl, meta), (M, alpha) in list(zip(built, conds))[:8]:
+    # autotune the z chunk and the case-group size on a throwaway batch holding every case
+    tb = aero3d.Batch(arena_bytes=int(cells * 110 + (2 << 30)), max_slots=128)
+    for (g, geo, fl, meta), (M, alpha) in zip(built, conds):
         F, fix, _ = case_setup(meta, M, alpha)
         tb.add(tb.add_geometry(g, geo), F, fix, 0.5)
     best = None
-    for zc in (2, 3, 4, 6, 8):
-        tb.set_zchunk(zc)
-        tb.time_steps(1)
-        ms = tb.time_steps(6)
-        if best is None or ms < best[1]:
-            best = (zc, ms)
+    for zc in (3, 4, 6, 8):
+        for gc in (1_000_000, 2_000_000, 4_000_000, 1 << 40):
+            tb.set_zchunk(zc)
+            tb.set_group_cells(gc)
+            tb.time_steps(3)
+            ms = tb.time_steps(3)
+            if best is None or ms < best[1]:
+                best = (zc, ms, gc)
     tb.dispose()
-    zc = best[0]
-    print(f"[bench] batched: {cells} cells, zchunk {zc}; server run", file=sys.stderr, flush=True)
+    zc, group = best[0], best[2]
+    print(f"[bench] batched: {cells} cells, zchunk {zc}, group {group}; server run", file=sys.stderr, flush=True)
     reqs = [(k, M, alpha, 0.0) for k, (M, alpha) in enumerate(conds)]
-    done, wall_b, busy, cu = run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes=int(cells * 110 + (2 << 30)))
+    done, wall_b, busy, cu = run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes=int(cells * 110 + (2 << 30)),
+                                        group_cells=group)
     print("[bench] batched: plain path, one case at a time", file=sys.stderr, flush=True)
-    wall_p, iters_p = run_converged_plain(built, conds, res_drop, max_iters, check_every, (8, 3))
+    wall_p, iters_p = run_converged_plain(built, conds, res_drop, max_iters, check_every, (8, 0))
     it_b = {d["tag"]: d["iters"] for d in done}
     same = all(it_b[k] == iters_p[k] for k in range(len(conds)))
     return {"cases": len(conds), "cells": cells, "solves_per_s": len(conds) / wall_b, "wall_s": wall_b,
             "cell_updates_per_s": cu / wall_b, "plain_solves_per_s": len(conds) / wall_p, "plain_wall_s": wall_p,
             "speedup_vs_plain": wall_p / wall_b, "converged": sum(d["converged"] for d in done),
-            "iters_mean": sum(it_b.values()) / len(it_b), "same_iterations_as_plain": same, "zchunk": zc,
+            "iters_mean": sum(it_b.values()) / len(it_b), "same_iterations_as_plain": same, "zchunk": zc, "group_cells": group,
             "criterion": f"res/res0 <= {res_drop} (density residual every {check_every} iterations) or {max_iters} iterations"}
 
 
@@ -205,13 +209,47 @@ def serve_leg(n_requests=200, res_drop=1e-4, max_iters=2000, check_every=50):
             "criterion": f"res/res0 <= {res_drop} or {max_iters} iterations, residual every {check_every}"}
 
 
+AXI_CASE = os.path.join(ROOT, "tests", "golden", "axi_tj_medium")
+
+
+def axi_leg(iters=14000):
+    """Turbojet axisymmetric run (C5): the reference setup's medium grid (120,600 cells), 14,000 SSP-RK2
+    iterations with the Tt4 control every 250 iterations, thrust and station averages (js/axi_run.js)."""
+    import subprocess
+    print(f"[bench] axi: TJ medium, {iters} iterations", file=sys.stderr, flush=True)
+    r = subprocess.run(["node", os.path.join(ROOT, "3d_aero_lab_amd", "js", "axi_run.js"), AXI_CASE, str(iters), "2000"],
+                       capture_output=True, text=True, timeout=600, check=True)
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out["cell_updates_per_s"] = out["cells"] * iters / out["step_s"]
+    out["roofline_frac"] = out["cell_updates_per_s"] * 182 / (HBM_PEAK_GBS * 1e9)
+    out["thrust_kN"] = out["thrust"]["F"] / 1000
+    out["published_thrust_kN"] = 56.9
+    return out
+
+
+def axi_cpu_baseline(budget_s=10.0):
+    """The axisymmetric oracle (C restatement of the reference CPUSolver, 1 thread) on the same medium case."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_axi
+    from axi_case import load
+    g, z, Q0, comps, xf, rf = load("axi_tj_medium")
+    s = g.s
+    o = oracle_axi.OracleAxi(s["W"], s["H"], xf, rf, z, comps, s["fs"], list(Q0), dict(s["opt"], hf=s["hf"]))
+    t0 = time.perf_counter(); o.step(1); t1 = time.perf_counter()
+    n = max(1, min(500, int(budget_s / max(t1 - t0, 1e-3))))
+    t0 = time.perf_counter(); o.step(n); dt = time.perf_counter() - t0
+    return {"value": s["N"] * n / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{n} iterations of the TJ medium case (after 1 untimed), single thread"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--legs", default="plain,batched,serve", help="comma list of plain, batched, serve")
+    ap.add_argument("--legs", default="plain,batched,serve,axi", help="comma list of plain, batched, serve, axi")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--tile", default="auto", help="stage-kernel shape 'ty,zchunk' or 'auto' (autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -277,6 +315,10 @@ def main():
         out["batched"] = batched_leg()
     if "serve" in legs:
         out["served"] = serve_leg(a.serve_requests)
+    if "axi" in legs:
+        out["axisymmetric"] = axi_leg()
+        if not a.no_cpu:
+            out["axisymmetric"]["cpu_baseline"] = axi_cpu_baseline()
     if not a.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(grid, geo, F, fix)

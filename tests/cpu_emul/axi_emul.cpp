@@ -38,10 +38,10 @@ extern "C" int axi_emul_steps(int W, int H, const double *xf, const double *rf_,
   float cap = 0;
   const int NF = NXF + NRF;
   for (int it = 0; it < n; it++) {
-    launch(NF, [&] { face_kernel(v, c, Q, Fx.data(), Fr.data(), nutF.data()); });
-    launch(N, [&] { cell_kernel(v, c, Q, Q, Q1.data(), Fx.data(), Fr.data(), nutF.data(), dt.data(), src, mfx, 0, &cap); });
-    launch(NF, [&] { face_kernel(v, c, Q1.data(), Fx.data(), Fr.data(), nutF.data()); });
-    launch(N, [&] { cell_kernel(v, c, Q1.data(), Q, Q, Fx.data(), Fr.data(), nutF.data(), dt.data(), src, mfx, 1, &cap); });
+    launch(NF, [&] { face_kernel(v, &c, Q, Fx.data(), Fr.data(), nutF.data()); });
+    launch(N, [&] { cell_kernel(v, &c, Q, Q, Q1.data(), Fx.data(), Fr.data(), nutF.data(), dt.data(), src, mfx, 0, &cap); });
+    launch(NF, [&] { face_kernel(v, &c, Q1.data(), Fx.data(), Fr.data(), nutF.data()); });
+    launch(N, [&] { cell_kernel(v, &c, Q1.data(), Q, Q, Fx.data(), Fr.data(), nutF.data(), dt.data(), src, mfx, 1, &cap); });
   }
   return 0;
 }

@@ -63,3 +63,27 @@ def test_axi_thrust_matches_reference():
             say("thrust", ref["it"], h["F"], ref["F"])
             assert abs(h["F"] - ref["F"]) / abs(ref["F"]) < 1e-4
             assert abs(h["mfuel"] - ref["mfuel"]) / abs(ref["mfuel"]) < 1e-4
+
+
+def test_axi_graph_replay_equals_direct_launches():
+    """Chunks of 25 iterations replay a captured graph; the same iterations
+    launched directly (chunks below 25) give bit-identical states, also after
+    the constants change between replays (control gain)."""
+    import aero3d
+    g, z, Q0, comps, xf, rf = load("axi_tj_xcoarse")
+    s = g.s
+    opt = dict(s["opt"], hf=s["hf"])
+    a = aero3d.AxiHIP(xf, rf, z, comps, s["fs"], Q0, opt)
+    b = aero3d.AxiHIP(xf, rf, z, comps, s["fs"], Q0, opt)
+    a.step(50)
+    for _ in range(10):
+        b.step(5)
+    assert np.array_equal(a.read()[0], b.read()[0])
+    c2 = np.array(comps, np.float64)
+    c2[17] *= 1.02
+    a.set_comps(c2); b.set_comps(c2)
+    a.step(75)
+    for _ in range(25):
+        b.step(3)
+    Qa, Qb = a.read()[0], b.read()[0]
+    assert np.array_equal(Qa, Qb)

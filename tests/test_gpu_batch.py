@@ -40,9 +40,11 @@ def plain(g, F, fix, cfl, n):
     return Q
 
 
-def test_batch_equals_plain_and_oracle():
+@pytest.mark.parametrize("group_cells", [2_000_000, 40_000])
+def test_batch_equals_plain_and_oracle(group_cells):
     import aero3d
     b = aero3d.Batch(arena_bytes=1 << 30, max_slots=16)
+    b.set_group_cells(group_cells)        # 40k: every case in its own group
     cs = cases()
     gids = {name: b.add_geometry(g.grid(), g.geo()) for name, g, *_ in cs}
     slots = {}
