@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(kBlock) stage_kernel(CaseView cv, StageConsts 
 }
 
 template <int TY>
-__global__ void __launch_bounds__(64 * TY) tile_kernel(CaseView cv, StageConsts kc, const float *__restrict__ qin,
+__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4))) tile_kernel(CaseView cv, StageConsts kc, const float *__restrict__ qin,
                                                       const float *__restrict__ bse, float *__restrict__ out,
                                                       float *__restrict__ dtout, size_t ns, float wa, float wb, int ntx, int nty,
                                                       int zc) {
@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(64 * TY) tile_kernel(CaseView cv, StageConsts 
 // and those blocks split its NZ planes evenly. No second partial round of
 // blocks, one z-queue preamble per block.
 template <int TY>
-__global__ void __launch_bounds__(64 * TY) tile_kernel_bal(CaseView cv, StageConsts kc, const float *__restrict__ qin,
+__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4))) tile_kernel_bal(CaseView cv, StageConsts kc, const float *__restrict__ qin,
                                                           const float *__restrict__ bse, float *__restrict__ out,
                                                           float *__restrict__ dtout, size_t ns, float wa, float wb, int ntx, int nty) {
   __shared__ TileSmem<64, TY> sm;

@@ -56,8 +56,10 @@ struct TableHdr { int nslots, total; };
 
 // Persistent stage kernel over the case table. phase: iterations since the
 // table's buffer indices were written (mod 3); stage: 0 or 1 of SSP-RK2.
-__global__ void __launch_bounds__(64 * kTY) batch_kernel(const TableHdr *__restrict__ hdr, const SlotDev *__restrict__ slots,
-                                                          int phase, int stage) {
+// At least four waves per SIMD (<= 128 VGPRs): two 8-wave blocks per CU.
+template <int stage>
+__global__ void __launch_bounds__(64 * kTY) __attribute__((amdgpu_waves_per_eu(4))) batch_kernel(const TableHdr *__restrict__ hdr, const SlotDev *__restrict__ slots,
+                                                          int phase) {
   __shared__ TileSmem<64, kTY> sm;
   const int total = hdr->total, nsl = hdr->nslots;
   const int G = gridDim.x, r = xcd_remap(blockIdx.x, G);
@@ -66,6 +68,7 @@ __global__ void __launch_bounds__(64 * kTY) batch_kernel(const TableHdr *__restr
     if (item < total) {
       int lo = 0, hi = nsl - 1;
       while (lo < hi) { const int m = (lo + hi + 1) >> 1; if (slots[m].item0 <= item) lo = m; else hi = m - 1; }
+      lo = __builtin_amdgcn_readfirstlane(lo);      // block-uniform: keep the case record in scalar registers
       const SlotDev &S = slots[lo];
       const int li = item - S.item0;
       const int bx = li % S.ntx, by = (li / S.ntx) % S.nty, bz = li / (S.ntx * S.nty);
@@ -74,7 +77,7 @@ __global__ void __launch_bounds__(64 * kTY) batch_kernel(const TableHdr *__restr
       const size_t plane5 = 5 * (size_t)S.ns;
       float *Q = S.state + (size_t)q * plane5, *S1 = S.state + (size_t)((q + 1) % 3) * plane5,
             *Bb = S.state + (size_t)((q + 2) % 3) * plane5;
-      if (stage == 0) tile_stage<64, kTY>(S.cv, S.kc, Q, Q, S1, nullptr, S.ns, 0.f, 1.f, bx * 64, by * kTY, k0, k1, sm);
+      if constexpr (stage == 0) tile_stage<64, kTY>(S.cv, S.kc, Q, Q, S1, nullptr, S.ns, 0.f, 1.f, bx * 64, by * kTY, k0, k1, sm);
       else tile_stage<64, kTY>(S.cv, S.kc, S1, Q, Bb, nullptr, S.ns, 0.5f, 0.5f, bx * 64, by * kTY, k0, k1, sm);
     }
     __syncthreads();                // the next item reuses the LDS tile
@@ -168,7 +171,7 @@ struct aero3d_batch {
   bool table_dirty = true;
   std::vector<int> order;            // slot ids in table order
   int ngroups = 0;
-  int64_t group_cells = 2000000;     // cases are stepped in groups of about this many cells
+  int64_t group_cells = int64_t(1) << 40;  // cases are stepped in groups of about this many cells (default: one group)
   int zc = 3;
   std::string err;
 };
@@ -237,13 +240,19 @@ int upload_table(aero3d_batch *b) {
   return 0;
 }
 
+// Both stages of one SSP-RK2 iteration of case group gi at rotation phase ph.
+void launch_iteration(aero3d_batch *b, int gi, int ph) {
+  const TableHdr *h = b->d_hdr + gi;
+  const SlotDev *t = b->d_table + gi * (size_t)b->max_slots;
+  batch_kernel<0><<<b->grid, dim3(64, kTY), 0, b->stream>>>(h, t, ph);
+  batch_kernel<1><<<b->grid, dim3(64, kTY), 0, b->stream>>>(h, t, ph);
+}
+
 int build_graph(aero3d_batch *b, int gi) {
   hipGraph_t g;
   hipGraphExec_t e;
   BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  for (int ph = 0; ph < 3; ph++)
-    for (int st = 0; st < 2; st++)
-      batch_kernel<<<b->grid, dim3(64, kTY), 0, b->stream>>>(b->d_hdr + gi, b->d_table + gi * (size_t)b->max_slots, ph, st);
+  for (int ph = 0; ph < 3; ph++) launch_iteration(b, gi, ph);
   BCHK(b, hipStreamEndCapture(b->stream, &g));
   BCHK(b, hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
   b->graphs[gi] = g; b->execs[gi] = e;
@@ -259,9 +268,7 @@ int batch_steps(aero3d_batch *b, int n) {
   for (int gi = 0; gi < b->ngroups; gi++) {
     if (n >= 3 && !b->execs[gi] && (rc = build_graph(b, gi))) return rc;
     for (int k = 0; k + 3 <= n; k += 3) BCHK(b, hipGraphLaunch(b->execs[gi], b->stream));
-    for (int ph = 0; ph < r; ph++)
-      for (int st = 0; st < 2; st++)
-        batch_kernel<<<b->grid, dim3(64, kTY), 0, b->stream>>>(b->d_hdr + gi, b->d_table + gi * (size_t)b->max_slots, ph, st);
+    for (int ph = 0; ph < r; ph++) launch_iteration(b, gi, ph);
   }
   BCHK(b, hipGetLastError());
   for (int s : b->order) { b->slots[s].cur = (b->slots[s].cur + 2 * r) % 3; b->slots[s].it += n; }
@@ -323,7 +330,7 @@ int aero3d_batch_create(int64_t arena_bytes, int max_slots, aero3d_batch **out) 
   int dev = 0, ncu = 0, occ = 0;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_kernel, 64 * kTY, 0);
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_kernel<1>, 64 * kTY, 0);
   b->grid = std::max(1, ncu * std::max(1, occ));
   *out = b;
   return 0;

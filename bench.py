@@ -105,7 +105,7 @@ def run_converged_plain(built, conds, res_drop, max_iters, check_every, tile):
     return time.perf_counter() - t0, iters
 
 
-def run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes, max_slots=256, group_cells=2_000_000):
+def run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes, max_slots=256, group_cells=1 << 40):
     """Submit requests (geometry index, M, alpha, arrival s) to the continuous-batching server as their
     arrival times come; tick until all are done. Returns results and the wall time."""
     import aero3d
