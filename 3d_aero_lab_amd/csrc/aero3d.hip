@@ -109,23 +109,21 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4)
   tile_stage<64, TY>(cv, kc, qin, bse, out, dtout, ns, wa, wb, bx * 64, by * TY, k0, k1, sm);
 }
 
-// Balanced launch: exactly G blocks (the device's resident capacity times a
-// small multiple). Block r (XCD-grouped) marches an equal contiguous share of
-// the planes of one tile: tile t owns blocks [ceil(t G / nt), ceil((t+1) G / nt)),
-// and those blocks split its NZ planes evenly. No second partial round of
-// blocks, one z-queue preamble per block.
+// Balanced launch: every tile gets the same number c of z chunks (c = G / nt
+// with G a multiple of the device's resident blocks), chunk l of a tile holds
+// planes [l NZ / c, (l+1) NZ / c). Logical block L = l * nt + t is chunk-major
+// and the XCD remap hands each XCD a contiguous range of L, i.e. whole chunk
+// levels: x-y neighbour tiles (which read each other's halo rows) and the
+// chunk below (whose top planes start this chunk's z queue) share an L2.
 template <int TY>
 __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4))) tile_kernel_bal(CaseView cv, StageConsts kc, const float *__restrict__ qin,
                                                           const float *__restrict__ bse, float *__restrict__ out,
                                                           float *__restrict__ dtout, size_t ns, float wa, float wb, int ntx, int nty) {
   __shared__ TileSmem<64, TY> sm;
-  const int G = gridDim.x, nt = ntx * nty;
-  const int r = xcd_remap(blockIdx.x, G);
-  const int t = (int)(((long long)r * nt) / G);
-  const int first = (int)(((long long)t * G + nt - 1) / nt), next = (int)(((long long)(t + 1) * G + nt - 1) / nt);
-  const int c = next - first, l = r - first;
+  const int G = gridDim.x, nt = ntx * nty, c = G / nt;
+  const int L = xcd_remap(blockIdx.x, G);
+  const int l = L / nt, t = L % nt;
   const int k0 = (int)(((long long)l * cv.NZ) / c), k1 = (int)(((long long)(l + 1) * cv.NZ) / c);
-  if (k1 <= k0) return;            // block-uniform: more blocks than planes for this tile
   const int bx = t % ntx, by = t / ntx;
   tile_stage<64, TY>(cv, kc, qin, bse, out, dtout, ns, wa, wb, bx * 64, by * TY, k0, k1, sm);
 }
@@ -185,9 +183,9 @@ void launch_stage(aero3d_solver *s, const CaseView &cv, const StageConsts &kc, c
   if (s->tile_zc <= 0) {            // balanced launch, G = multiple of the resident block count
     const int mult = s->tile_zc == 0 ? 1 : -s->tile_zc;
     const long long nt = (long long)ntx * nty;
-    long long G = (long long)mult * (s->tile_ty == 4 ? s->resident4 : s->resident8);
-    G = std::min(G, nt * s->NZ);
-    G = std::max(G, nt);
+    const long long res = (long long)mult * (s->tile_ty == 4 ? s->resident4 : s->resident8);
+    const long long c = std::max(1LL, std::min((long long)s->NZ, res / nt));   // z chunks per tile
+    const long long G = nt * c;
     if (s->tile_ty == 4)
       tile_kernel_bal<4><<<(int)G, dim3(64, 4), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
     else

@@ -124,7 +124,7 @@ __device__ __forceinline__ int load_prim(const CaseView &cv, const StageConsts &
   }
   const bool mirror = j < 0;
   const int jj = mirror ? -1 - j : j;
-  const int q = (k * cv.NY + jj) * cv.NX + i;
+  const unsigned q = (unsigned)((k * cv.NY + jj) * cv.NX + i);
   const int tq = cv.t[q];
   if (tq >= 3) {
 #pragma unroll
@@ -147,12 +147,12 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
   const int NX = cv.NX, NY = cv.NY, NZ = cv.NZ, NXY = NX * NY;
   const int i = i0 + tx, j = j0 + ty;
   const bool own = i < NX && j < NY;
-  const int qc = own ? j * NX + i : 0;                 // column base index (plane 0)
+  const unsigned qc = own ? (unsigned)(j * NX + i) : 0u;   // column base index (plane 0)
   const bool o2 = kc.order == 2;
   const float ihx = own ? frcp(cv.dx[i]) : 0.f, ihy = own ? frcp(cv.dy[j]) : 0.f;
   const float *__restrict__ ap0 = cv.ap, *__restrict__ ap1 = cv.ap + cv.N, *__restrict__ ap2 = cv.ap + 2 * (size_t)cv.N,
                             *__restrict__ ap3 = cv.ap + 3 * (size_t)cv.N;
-  auto apz = [&](int k) -> float { return (k < 0 || k >= NZ || !own) ? 1.f : ap2[k * NXY + qc]; };
+  auto apz = [&](int k) -> float { return (k < 0 || k >= NZ || !own) ? 1.f : ap2[(unsigned)(k * NXY) + qc]; };
 
   // ---- z queue preamble: planes k0-2 .. k0+1
   float Pz[4][5];          // primitives of planes k-1, k, k+1, k+2 (own column)
@@ -170,7 +170,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
 #pragma unroll
   for (int v = 0; v < 5; v++) sz0[v] = sz1[v];                                                              // sz0 = s(k)
   // queue: Pz[0] = k-1, Pz[1] = k, Pz[2] = k+1; Pz[3] loaded each plane
-  float sUpPrev = (own && k0 > 0) ? ap3[(k0 - 1) * NXY + qc] : 0.f;
+  float sUpPrev = (own && k0 > 0) ? ap3[(unsigned)((k0 - 1) * NXY) + qc] : 0.f;
 
   for (int k = k0; k < k1; k++) {
     // ---------------- load phase
@@ -192,14 +192,15 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       sm.T[hy + 2][hx + 2] = (int8_t)T;
     }
     // apertures: own x / y faces, plus the two faces left of / below the tile
-    sm.A0[ty][tx + 2] = own ? ap0[k * NXY + qc] : 1.f;
-    sm.A1[ty + 2][tx] = own ? ap1[k * NXY + qc] : 1.f;
+    const unsigned qk = (unsigned)(k * NXY) + qc;
+    sm.A0[ty][tx + 2] = own ? ap0[qk] : 1.f;
+    sm.A1[ty + 2][tx] = own ? ap1[qk] : 1.f;
     if (tid < 2 * TY) {
       const int r = tid >> 1, ii = (tid & 1) - 2, gi = i0 + ii, gj = j0 + r;
-      sm.A0[r][ii + 2] = (gi < 0 || gi >= NX || gj >= NY) ? 1.f : ap0[(k * NY + gj) * NX + gi];
+      sm.A0[r][ii + 2] = (gi < 0 || gi >= NX || gj >= NY) ? 1.f : ap0[(unsigned)((k * NY + gj) * NX + gi)];
     } else if (tid >= 64 && tid < 64 + 2 * TX) {
       const int u = tid - 64, c = u >> 1, jj = (u & 1) - 2, gi = i0 + c, gj = j0 + jj;
-      sm.A1[jj + 2][c] = (gj < 0 || gj >= NY || gi >= NX) ? 1.f : ap1[(k * NY + gj) * NX + gi];
+      sm.A1[jj + 2][c] = (gj < 0 || gj >= NY || gi >= NX) ? 1.f : ap1[(unsigned)((k * NY + gj) * NX + gi)];
     }
     __syncthreads();
 
@@ -220,27 +221,29 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Sy[v][ty + 1][tx] = s[v];
     }
-    if (tid < 2 * TY + 2 * TX) {
+    // the tile-edge slopes go to waves 2.. (waves 0 and 1 take the tile-edge fluxes)
+    const int et = (tid + TX * TY - 2 * TX) % (TX * TY);
+    for (int e = et; e < 2 * TY + 2 * TX; e += TX * TY) {
       float P0[5], Pm[5], Pp[5], s[5];
-      if (tid < 2 * TY) {                               // x slope of cell -1 or TX in row r
-        const int r = tid >> 1, cx = (tid & 1) ? TX : -1, xi = cx + 2, yj = r + 2;
+      if (e < 2 * TY) {                                 // x slope of cell -1 or TX in row r
+        const int r = e >> 1, cx = (e & 1) ? TX : -1, xi = cx + 2, yj = r + 2;
 #pragma unroll
         for (int v = 0; v < 5; v++) { P0[v] = sm.P[v][yj][xi]; Pm[v] = sm.P[v][yj][xi - 1]; Pp[v] = sm.P[v][yj][xi + 1]; }
         // faces of cell cx: ap0 of cells cx-1 and cx; cell TX's + face lies outside the tile
         const int gi = i0 + cx, gj = j0 + r;
         const float am = sm.A0[r][cx + 1];
-        const float apf = (cx == -1) ? sm.A0[r][cx + 2] : ((gi < 0 || gi >= NX || gj >= NY) ? 1.f : ap0[(k * NY + gj) * NX + gi]);
+        const float apf = (cx == -1) ? sm.A0[r][cx + 2] : ((gi < 0 || gi >= NX || gj >= NY) ? 1.f : ap0[(unsigned)((k * NY + gj) * NX + gi)]);
         const bool ok = o2 && sm.T[yj][xi] == 0 && sm.T[yj][xi - 1] == 0 && sm.T[yj][xi + 1] == 0 && am == 1.f && apf == 1.f;
         slope5(Pm, P0, Pp, ok, s);
 #pragma unroll
         for (int v = 0; v < 5; v++) sm.Sx[v][r][cx + 1] = s[v];
       } else {                                          // y slope of row -1 or TY in column c
-        const int u = tid - 2 * TY, c = u >> 1, ry = (u & 1) ? TY : -1, xi = c + 2, yj = ry + 2;
+        const int u = e - 2 * TY, c = u >> 1, ry = (u & 1) ? TY : -1, xi = c + 2, yj = ry + 2;
 #pragma unroll
         for (int v = 0; v < 5; v++) { P0[v] = sm.P[v][yj][xi]; Pm[v] = sm.P[v][yj - 1][xi]; Pp[v] = sm.P[v][yj + 1][xi]; }
         const int gi = i0 + c, gj = j0 + ry;
         const float am = sm.A1[ry + 1][c];
-        const float apf = (ry == -1) ? sm.A1[ry + 2][c] : ((gj < 0 || gj >= NY || gi >= NX) ? 1.f : ap1[(k * NY + gj) * NX + gi]);
+        const float apf = (ry == -1) ? sm.A1[ry + 2][c] : ((gj < 0 || gj >= NY || gi >= NX) ? 1.f : ap1[(unsigned)((k * NY + gj) * NX + gi)]);
         const bool ok = o2 && sm.T[yj][xi] == 0 && sm.T[yj - 1][xi] == 0 && sm.T[yj + 1][xi] == 0 && am == 1.f && apf == 1.f;
         slope5(Pm, P0, Pp, ok, s);
 #pragma unroll
@@ -292,7 +295,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
 
     // ---------------- cell update
     if (own) {
-      const int q = k * NXY + qc;
+      const unsigned q = qk;
       const int tq = Tz[1];
       if (tq != 0) {
         if (tq >= 3) {
@@ -338,7 +341,14 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         const float ph = cv.phi[q], iph = frcp(ph);
         float Qc[5], Bq[5], Bw[5];
 #pragma unroll
-        for (int c = 0; c < 5; c++) { Qc[c] = qin[c * ns + q]; Bq[c] = bse[c * ns + q]; }
+        for (int c = 0; c < 5; c++) Qc[c] = qin[c * ns + q];
+        if (qin == bse) {            // stage 1: the base state is the input state (one read)
+#pragma unroll
+          for (int c = 0; c < 5; c++) Bq[c] = Qc[c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 5; c++) Bq[c] = bse[c * ns + q];
+        }
         prim_fast(Bq[0], Bq[1], Bq[2], Bq[3], Bq[4], Bw);
         const float a = fsqrt(kGamma * Bw[4] * frcp(Bw[0]));
         const float dt = kc.cfl * fmaxf(ph, 0.25f) *

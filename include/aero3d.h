@@ -140,9 +140,9 @@ int aero3d_batch_add(aero3d_batch *b, int gid, const double F[5], const double f
 int aero3d_batch_remove(aero3d_batch *b, int slot);
 /* Stage-kernel z chunk of every case (tiles are 64 x 8 columns). */
 int aero3d_batch_set_zchunk(aero3d_batch *b, int zchunk);
-/* Cases are stepped in groups of about `cells` cells (default: all cases in one group): a
- * group's state and masks stay in the Infinity Cache across its stages and
-  * iterations before the next group runs; default: one group of all cases. */
+/* Cases are stepped in groups of about `cells` cells (default: all cases in
+ * one group): a group's state and masks stay in the Infinity Cache across its
+ * stages and iterations before the next group runs. */
 int aero3d_batch_set_group_cells(aero3d_batch *b, int64_t cells);
 /* n SSP-RK2 iterations of every resident case (src/air_cfd.js:252-258), enqueued. */
 int aero3d_batch_step(aero3d_batch *b, int n);
