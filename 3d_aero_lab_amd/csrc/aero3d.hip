@@ -135,6 +135,18 @@ __global__ void fill_kernel(float *__restrict__ dst, size_t ns, int N, float f0,
   dst[q] = f0; dst[ns + q] = f1; dst[2 * ns + q] = f2; dst[3 * ns + q] = f3; dst[4 * ns + q] = f4;
 }
 
+// Fixed cells (flags 3..5) of buffer `dst` := their conserved fixed states. The
+// stage kernels take a fixed cell's primitives from the stored state, so every
+// state a stage reads holds them (the stages write them too).
+struct FixStates { float q[3][5]; };
+__global__ void fixcells_kernel(float *__restrict__ dst, size_t ns, int N, const uint8_t *__restrict__ t, FixStates f) {
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= N) return;
+  const int g = t[q] - 3;
+  if (g < 0) return;
+  for (int c = 0; c < 5; c++) dst[c * ns + q] = f.q[g][c];
+}
+
 __global__ void copy_kernel(float *__restrict__ dst, const float *__restrict__ src, int N) {
   const int q = blockIdx.x * kBlock + threadIdx.x;
   if (q < N) dst[q] = src[q];
@@ -172,6 +184,16 @@ CaseView make_view(const aero3d_solver *s) {
 }
 
 inline float *buf(aero3d_solver *s, int b) { return s->state + (size_t)(b % 3) * 5 * s->ns; }
+
+// Write the fixed states into the fixed cells of the current state.
+int apply_fix(aero3d_solver *s) {
+  FixStates f;
+  for (int g = 0; g < 3; g++)
+    for (int c = 0; c < 5; c++) f.q[g][c] = (float)s->fix[g][c];
+  fixcells_kernel<<<(s->N + kBlock - 1) / kBlock, kBlock, 0, s->stream>>>(buf(s, s->cur), s->ns, s->N, s->t, f);
+  HIPCHK(s, hipGetLastError());
+  return 0;
+}
 
 void launch_stage(aero3d_solver *s, const CaseView &cv, const StageConsts &kc, const float *qin, const float *bse, float *out,
                   float *dtout, float wa, float wb) {
@@ -291,7 +313,7 @@ int aero3d_set_free(aero3d_solver *s, const double F[5]) {
 int aero3d_set_fix(aero3d_solver *s, const double fix[15]) {
   if (!s || !fix) return fail(s, AERO3D_E_ARG, "set_fix: null argument");
   std::memcpy(s->fix, fix, sizeof s->fix);
-  return 0;
+  return apply_fix(s);
 }
 int aero3d_set_opt(aero3d_solver *s, double cfl, int order) {
   if (!s) return AERO3D_E_ARG;
@@ -305,7 +327,7 @@ int aero3d_reset(aero3d_solver *s) {
   fill_kernel<<<(s->N + kBlock - 1) / kBlock, kBlock, 0, s->stream>>>(buf(s, 0), s->ns, s->N, (float)s->F[0], (float)s->F[1],
                                                                       (float)s->F[2], (float)s->F[3], (float)s->F[4]);
   HIPCHK(s, hipGetLastError());
-  return 0;
+  return apply_fix(s);
 }
 int aero3d_step(aero3d_solver *s, int n) {
   if (!s || n < 0) return fail(s, AERO3D_E_ARG, "step: n must be >= 0");
@@ -338,6 +360,7 @@ int aero3d_write(aero3d_solver *s, const float *r, const float *ru, const float 
   const float *src[5] = {r, ru, rv, rw, E};
   for (int c = 0; c < 5; c++)
     HIPCHK(s, hipMemcpyAsync(Q + c * s->ns, src[c], sizeof(float) * (size_t)s->N, hipMemcpyHostToDevice, s->stream));
+  if (int rc = apply_fix(s)) return rc;
   HIPCHK(s, hipStreamSynchronize(s->stream));
   return 0;
 }

@@ -89,6 +89,16 @@ __global__ void bfill_kernel(float *__restrict__ dst, size_t ns, int N, float f0
   if (q >= N) return;
   dst[q] = f0; dst[ns + q] = f1; dst[2 * ns + q] = f2; dst[3 * ns + q] = f3; dst[4 * ns + q] = f4;
 }
+// Fixed cells (flags 3..5) := their conserved fixed states (the stage kernels
+// take a fixed cell's primitives from the stored state).
+struct BFixStates { float q[3][5]; };
+__global__ void bfixcells_kernel(float *__restrict__ dst, size_t ns, int N, const uint8_t *__restrict__ t, BFixStates f) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= N) return;
+  const int g = t[q] - 3;
+  if (g < 0) return;
+  for (int c = 0; c < 5; c++) dst[c * ns + q] = f.q[g][c];
+}
 __global__ void bcopy_kernel(float *__restrict__ dst, const float *__restrict__ src, int N) {
   const int q = blockIdx.x * 256 + threadIdx.x;
   if (q < N) dst[q] = src[q];
@@ -437,6 +447,11 @@ int aero3d_batch_add(aero3d_batch *b, int gid, const double F[5], const double f
   G.refs++;
   // reset(): state := F
   bfill_kernel<<<(G.N + 255) / 256, 256, 0, b->stream>>>(S.state, ns, G.N, (float)F[0], (float)F[1], (float)F[2], (float)F[3], (float)F[4]);
+  BCHK(b, hipGetLastError());
+  BFixStates fs;
+  for (int gg = 0; gg < 3; gg++)
+    for (int v = 0; v < 5; v++) fs.q[gg][v] = S.kc.fixq[gg][v];
+  bfixcells_kernel<<<(G.N + 255) / 256, 256, 0, b->stream>>>(S.state, ns, G.N, G.t, fs);
   BCHK(b, hipGetLastError());
   b->table_dirty = true;
   *slot = s;

@@ -18,7 +18,8 @@
 //   k+1, the z flux of face k-1/2 carried from the previous plane.
 // Per plane: load (own plane k+2 into the queue, halo of plane k into LDS),
 // slopes, fluxes, cell update; three barriers.
-// Divisions use the hardware reciprocal and square root (1 ulp).
+// Divisions use the hardware reciprocal, square root and reciprocal square
+// root (1 ulp).
 // ============================================================================
 #pragma once
 #include "euler3d.hpp"
@@ -27,6 +28,7 @@ namespace aero3d {
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // Primitives (rho, u, v, w, p) of conserved a[] (cfdPrim5, src/air_cfd.js:169).
 __device__ __forceinline__ void prim_fast(float a0, float a1, float a2, float a3, float a4, float *o) {
@@ -35,12 +37,15 @@ __device__ __forceinline__ void prim_fast(float a0, float a1, float a2, float a3
   o[4] = fmaxf(kGm1 * (a4 - 0.5f * r * (u * u + v * v + w * w)), 1e-7f);
 }
 
-// HLLC flux along axis d (src/air_cfd.js:179-195) with reciprocals.
+// HLLC flux along axis d (src/air_cfd.js:179-195) with four hardware
+// transcendentals per face: two reciprocal square roots for the sound speeds,
+// the reciprocals of the contact-speed denominator and of S - SM (the star
+// energy is regrouped so that no 1/r or 1/(S - u) is needed).
 template <int d>
 __device__ __forceinline__ void hllc_fast(const float *L, const float *R, float *out) {
   const float rL = L[0], rR = R[0], pL = L[4], pR = R[4], uL = L[1 + d], uR = R[1 + d];
-  const float irL = frcp(rL), irR = frcp(rR);
-  const float aL = fsqrt(kGamma * pL * irL), aR = fsqrt(kGamma * pR * irR);
+  const float gpL = kGamma * pL, gpR = kGamma * pR;   // a = sqrt(g p / r) = g p / sqrt(g p r)
+  const float aL = gpL * frsq(gpL * rL), aR = gpR * frsq(gpR * rR);
   const float EL = pL * 2.5f + 0.5f * rL * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]);
   const float ER = pR * 2.5f + 0.5f * rR * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]);
   const float SL = fminf(uL - aL, uR - aR), SR = fmaxf(uL + aL, uR + aR);
@@ -57,15 +62,15 @@ __device__ __forceinline__ void hllc_fast(const float *L, const float *R, float 
   const bool left = SM >= 0.f;
   const float k1 = left ? L[1] : R[1], k2 = left ? L[2] : R[2], k3 = left ? L[3] : R[3];
   const float S = left ? SL : SR, uK = left ? uL : uR, rK = left ? rL : rR, pK = left ? pL : pR, EK = left ? EL : ER;
-  const float irK = left ? irL : irR;
-  const float SmU = S - uK;
-  const float f = rK * SmU * frcp(S - SM);
+  const float SmU = S - uK, iSS = frcp(S - SM);
+  const float f = rK * SmU * iSS;
   float Fm[4] = {rK * uK, rK * k1 * uK, rK * k2 * uK, rK * k3 * uK};
   Fm[1 + d] += pK;
   const float Um[4] = {rK, rK * k1, rK * k2, rK * k3};
   float Us[4] = {f, f * k1, f * k2, f * k3};
   Us[1 + d] = f * SM;
-  const float Es = f * (EK * irK + (SM - uK) * (SM + pK * frcp(rK * SmU)));
+  // f (E/r + (SM - u)(SM + p / (r (S - u)))) with f = r (S - u) / (S - SM)
+  const float Es = EK * SmU * iSS + (SM - uK) * (f * SM + pK * iSS);
 #pragma unroll
   for (int c = 0; c < 4; c++) out[c] = Fm[c] + S * (Us[c] - Um[c]);
   out[4] = uK * (EK + pK) + S * (Es - EK);
@@ -112,28 +117,30 @@ struct TileSmem {
   float Fy[5][TY + 1][TX];   // y fluxes of faces -1/2..TY-1/2
 };
 
-// Classify and load the primitives of cell (i, j, k) of one case:
-// freestream ghost outside the box, mirror below y = 0, fixed state for
-// flags >= 3, else the primitives of qin (src/air_cfd.js:205-210).
+// Classify and load the primitives of cell (i, j, k) of one case: freestream
+// ghost outside the box, mirror below y = 0, else the primitives of the stored
+// state (src/air_cfd.js:205-210). Fixed cells (flags 3..5) hold their fixed
+// conserved state in every state buffer (the solver writes it at reset, write
+// and set_fix, and every stage writes it), so their primitives come from the
+// stored state like any other cell's: no per-lane lookup and no branch on the
+// flag, and the flag and the five values are loaded in one memory round trip
+// (cell 0 stands in for cells outside the box). P is written once per
+// component from register values.
 __device__ __forceinline__ int load_prim(const CaseView &cv, const StageConsts &kc, const float *__restrict__ qin, size_t ns,
                                          int i, int j, int k, float *P) {
-  if (i < 0 || i >= cv.NX || j >= cv.NY || k < 0 || k >= cv.NZ) {
-#pragma unroll
-    for (int v = 0; v < 5; v++) P[v] = kc.Fw[v];
-    return -1;
-  }
+  const bool inbox = !(i < 0 || i >= cv.NX || j >= cv.NY || k < 0 || k >= cv.NZ);
   const bool mirror = j < 0;
   const int jj = mirror ? -1 - j : j;
-  const unsigned q = (unsigned)((k * cv.NY + jj) * cv.NX + i);
+  const unsigned q = inbox ? (unsigned)((k * cv.NY + jj) * cv.NX + i) : 0u;
   const int tq = cv.t[q];
-  if (tq >= 3) {
+  float w[5];
+  prim_fast(qin[q], qin[ns + q], qin[2 * ns + q], qin[3 * ns + q], qin[4 * ns + q], w);
 #pragma unroll
-    for (int v = 0; v < 5; v++) P[v] = kc.fixw[tq - 3][v];
-  } else {
-    prim_fast(qin[q], qin[ns + q], qin[2 * ns + q], qin[3 * ns + q], qin[4 * ns + q], P);
+  for (int v = 0; v < 5; v++) {
+    const float x = !inbox ? kc.Fw[v] : w[v];
+    P[v] = (v == 2 && inbox && mirror) ? -x : x;
   }
-  if (mirror) { P[2] = -P[2]; return -2; }
-  return tq;
+  return !inbox ? -1 : (mirror ? -2 : tq);
 }
 
 // One stage over the z chunk [k0, k1) of tile (i0, j0) of case cv.

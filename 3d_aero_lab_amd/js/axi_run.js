@@ -38,7 +38,7 @@ function run(dir, iters, every) {
     it += n;
     Q = S.read(Q);
     const f0 = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, 0, { S: S.src, F: S.mfx });
-    P.cfdControl('tj', s.ctl, C.grid, C.zones, Q, S.mfx, comps, f0.pw, s.ctl.etam, 250);
+    P.cfdControl(s.type || 'tj', s.ctl, C.grid, C.zones, Q, S.mfx, comps, f0.pw, s.ctl.etam, 250);
     S.setComps(comps);
     if (it % every === 0 || it >= iters) {
       const f = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, s.hf, { S: S.src, F: S.mfx });
@@ -47,7 +47,7 @@ function run(dir, iters, every) {
   }
   const wall = Number(process.hrtime.bigint() - t0) / 1e9;
   const f = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, s.hf, { S: S.src, F: S.mfx });
-  const st = P.stations(C.grid, C.zones, Q, S.mfx, s.ctl.order);
+  const st = P.stations(C.grid, C.zones, Q, S.mfx, s.ctl.order, s.type || 'tj');
   S.dispose();
   return { cells: C.grid.N, iters, hist, thrust: { F: f.F, Fb: f.Fb, Fw: f.Fw, mfuel: f.mfuel }, F1D: s.ctl.F1D, stations: st,
     wall_s: wall, step_s: Number(tStep) / 1e9 };

@@ -109,13 +109,20 @@ function cfdControl(type, ctl, grid, zones, Q, mfx, comps, pw, etam, dIt) {
   return out;
 }
 
-/* Station averages of the turbojet flowpath (tools/stations.js:8-16): at each
+/* Station averages of an engine flowpath (tools/stations.js:8-16): at each
  * station column, mass flow and mass-weighted Tt, Pt over the station's
- * stream, with the cell mass flow taken from the mean axial face mass flux. */
-const TJ_STATIONS = [['2', 0.36, 'core'], ['3', 1.55, 'core'], ['4', 2.21, 'core'], ['5', 2.62, 'core'], ['7', 4.29, 'core'], ['9', 4.89, 'core']];
+ * stream, with the cell mass flow taken from the mean axial face mass flux.
+ * list: an engine type ('tj', 'hbtf', 'lbtf') or [[id, x, stream], ...]. */
+const STATIONS = {
+  hbtf: [['2', 0.58, 'inlet'], ['13', 0.95, 'bypass'], ['25', 1.47, 'core'], ['3', 2.05, 'core'], ['4', 2.43, 'core'], ['45', 2.62, 'core'], ['5', 3.08, 'core'], ['19', 2.19, 'bypass']],
+  tj: [['2', 0.36, 'core'], ['3', 1.55, 'core'], ['4', 2.21, 'core'], ['5', 2.62, 'core'], ['7', 4.29, 'core'], ['9', 4.89, 'core']],
+  lbtf: [['2', 0.3, 'inlet'], ['13', 0.9, 'bypass'], ['3', 1.55, 'core'], ['4', 2.0, 'core'], ['5', 2.55, 'core'], ['6A', 3.0, 'ab'], ['9', 4.94, 'ab']],
+};
+const TJ_STATIONS = STATIONS.tj;
 function stations(grid, zones, Q, mfx, order, list) {
   const { W, H, rc, dr } = grid, out = [];
-  for (const [id, x, sn] of (list || TJ_STATIONS)) {
+  const L = typeof list === 'string' ? STATIONS[list] : (list || TJ_STATIONS);
+  for (const [id, x, sn] of L) {
     const si = order.indexOf(sn), i = colAt(grid, x);
     let m = 0, tt = 0, pt = 0, wsum = 0;
     for (let j = 0; j < H; j++) {
@@ -132,4 +139,4 @@ function stations(grid, zones, Q, mfx, order, list) {
   return out;
 }
 
-module.exports = { RGAS, clamp, cfdPrim, cfdForces, colAt, cfdSection, spoolRatios, cfdControl, stations, TJ_STATIONS };
+module.exports = { RGAS, clamp, cfdPrim, cfdForces, colAt, cfdSection, spoolRatios, cfdControl, stations, STATIONS, TJ_STATIONS };

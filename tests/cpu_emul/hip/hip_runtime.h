@@ -27,5 +27,7 @@ extern thread_local std::barrier<> *emul_block_barrier;
 inline void __syncthreads() { emul_block_barrier->arrive_and_wait(); }
 inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
 inline float __builtin_amdgcn_sqrtf(float x) { return std::sqrt(x); }
+inline float __builtin_amdgcn_rsqf(float x) { return 1.0f / std::sqrt(x); }
 #include <math.h>
 inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }

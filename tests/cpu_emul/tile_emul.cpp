@@ -49,6 +49,10 @@ extern "C" int emul_steps(int NX, int NY, int NZ, const float *dx, const float *
   const size_t N = cv.N;
   // exact-size heap buffers so AddressSanitizer sees any overrun
   std::vector<float> S1(5 * N), B(5 * N), dt(N);
+  // the solver keeps fixed cells at their conserved fixed states (reset, write, set_fix)
+  for (size_t q = 0; q < N; q++)
+    if (t[q] >= 3)
+      for (int c = 0; c < 5; c++) Q[c * N + q] = kc.fixq[t[q] - 3][c];
   for (int it = 0; it < n; it++) {
     if (ty == 4) {
       run_stage<4>(cv, kc, Q, Q, S1.data(), nullptr, N, 0.f, 1.f, zc);

@@ -6,12 +6,15 @@
  * reference's JavaScript (read-only checkout, default /root/reference, override
  * with AERO_REF) under Node.
  *
- *   node tests/golden/make_golden_axi.js [case ...]     cases: xcoarse medium longrun
+ *   node tests/golden/make_golden_axi.js [case ...]
+ * cases: xcoarse medium longrun (TJ), hbtf_xcoarse hbtf_medium lbtf_xcoarse,
+ * longrun_hbtf longrun_lbtf (the README.md:71-75 validation rows)
  *
  * The reference's own harness (tools/cfdlib.js:6-20) uses `??`, which this
  * Node (v12) cannot parse; the same setup sequence is written out here:
  * solveEngine -> buildSpec -> FieldModel -> buildCFDGrid -> buildZones ->
- * compParams -> initialState -> CPUSolver (cfl 0.6, order 2, visc, hf = etb LHV).
+ * compParams -> initialState -> CPUSolver (cfl 0.6, order 2, visc, hf = etb LHV),
+ * with Tt7 2100 K for the LBTF and 2000 K otherwise (tools/cfdlib.js:8).
  * Outputs (data only): grid faces, zones, comps, freestream, initial state,
  * states / diagnostics after N iterations, thrust and control values.
  * ========================================================================== */
@@ -28,9 +31,9 @@ vm.runInThisContext(code + '\nglobalThis.AXI={cfdControl,solveEngine,ENGINE_TYPE
 const A = globalThis.AXI;
 const OUT = __dirname;
 
-function setupTJ(level) {
-  const type = 'tj', T = A.ENGINE_TYPES[type];
-  const P = { type, design: Object.assign({}, T.design), eff: Object.assign({}, A.DEFAULT_EFF), alt: 0, M0: 0.25, thr: 1, ab: false, Tt7: 2000, nozzleType: T.nozzle };
+function setup(type, level) {
+  const T = A.ENGINE_TYPES[type];
+  const P = { type, design: Object.assign({}, T.design), eff: Object.assign({}, A.DEFAULT_EFF), alt: 0, M0: 0.25, thr: 1, ab: false, Tt7: type === 'lbtf' ? 2100 : 2000, nozzleType: T.nozzle };
   const sol = A.solveEngine(P), op = sol.op, fl = sol.fl, nz = op.noz;
   const spec = A.buildSpec(type, 1, { r8: Math.sqrt(nz.Athroat / Math.PI), r9: Math.sqrt(Math.max(nz.A, nz.Athroat) / Math.PI), A9: sol.ref.noz && sol.ref.noz.A, A19: sol.ref.nozB && sol.ref.nozB.A });
   const model = new A.FieldModel(spec, sol, P);
@@ -65,7 +68,7 @@ function dumpSetup(w, C, q0hash) {
   for (const k of ZONE_KEYS) w.arr('z_' + k, C.zones[k]);
   w.arr('comps', C.comps);
   for (let q = 0; q < 5; q++) w.arr('Q0_' + q, C.Q0[q], q0hash);
-  w.set('W', g.W); w.set('H', g.H); w.set('N', g.N);
+  w.set('W', g.W); w.set('H', g.H); w.set('N', g.N); w.set('type', C.type);
   w.set('fs', C.fsv); w.set('hf', C.hf); w.set('opt', { cfl: 0.6, order: 2, visc: true });
   // what cfdControl reads from spec / op (src/cfd_post.js:110-123)
   w.set('ctl', { s: C.spec.s, comb: { x1: C.spec.comb.x1, stream: C.spec.comb.stream }, ab: C.spec.ab ? { x1: C.spec.ab.x1, stream: C.spec.ab.stream } : null,
@@ -83,7 +86,7 @@ function thrust(C) {
 
 function caseXcoarse() {
   const w = new Writer('axi_tj_xcoarse');
-  const C = setupTJ('xcoarse');
+  const C = setup('tj', 'xcoarse');
   dumpSetup(w, C, false);
   const t0 = Date.now();
   let it = 0;
@@ -95,18 +98,23 @@ function caseXcoarse() {
   }
   w.set('ms_per_iter', (Date.now() - t0) / it);
   w.set('hist', hist);
-  // control step at it 500 (the long-run loop calls it every 250 iterations)
+  controlAndSection(w, C);
+  w.close();
+}
+
+/* one control step on the last state (the long-run loop calls it every 250
+ * iterations) and the combustor-exit section average */
+function controlAndSection(w, C) {
   const f0 = A.cfdForces(C.grid, C.zones, C.S.Q, C.comps, C.fl.P0, 0, { S: C.S.src, F: C.S.mfx });
   const ctl = A.cfdControl(C.type, C.spec, C.grid, C.zones, C.S.Q, C.S.mfx, C.comps, C.base, f0.pw, C.op, C.P.eff.etam, 250);
   w.set('control', { Tt4: ctl.Tt4, Tt7: ctl.Tt7, ratios: ctl.ratios, comps_after: Array.from(C.comps) });
   const sec = A.cfdSection(C.grid, C.zones, C.S.Q, C.spec.comb.x1 - 0.004 * C.spec.s, C.spec.order.indexOf(C.spec.comb.stream), C.S.mfx);
   w.set('section', sec);
-  w.close();
 }
 
 function caseMedium() {
   const w = new Writer('axi_tj_medium');
-  const C = setupTJ('medium');
+  const C = setup('tj', 'medium');
   dumpSetup(w, C, false);
   const t0 = Date.now();
   C.S.step(2);
@@ -116,11 +124,11 @@ function caseMedium() {
   w.close();
 }
 
-/* README.md:69-73 run: FRESH=1 node tools/longrun.js tj medium 14000 2000 (control every 250 iterations). */
-function caseLongrun() {
-  const iters = +(process.env.AXI_LONG_ITERS || 14000), every = 2000;
-  const w = new Writer('axi_tj_medium_longrun');
-  const C = setupTJ('medium');
+/* README.md:69-75 runs: FRESH=1 node tools/longrun.js <type> <level> <iters> 2000 (control every 250 iterations). */
+function caseLongrun(type, level, iters0, name) {
+  const iters = +(process.env.AXI_LONG_ITERS || iters0), every = 2000;
+  const w = new Writer(name);
+  const C = setup(type, level);
   const hist = [];
   const t0 = Date.now();
   for (let n = 0; n < iters; n += every) {
@@ -140,6 +148,41 @@ function caseLongrun() {
   }
 }
 
+/* setup, then full states after 10 and 100 iterations (thrust history at 1, 10, 100) */
+function caseShort(type, level) {
+  const w = new Writer(`axi_${type}_${level}`);
+  const C = setup(type, level);
+  dumpSetup(w, C, false);
+  const t0 = Date.now();
+  let it = 0;
+  const hist = [];
+  for (const target of [1, 10, 100]) {
+    C.S.step(target - it); it = target;
+    dumpState(w, 'it' + it, C, it === 1);
+    hist.push({ it, ...thrust(C) });
+  }
+  w.set('ms_per_iter', (Date.now() - t0) / it);
+  w.set('hist', hist);
+  controlAndSection(w, C);
+  w.close();
+}
+function caseSetupOnly(type, level) {
+  const w = new Writer(`axi_${type}_${level}`);
+  const C = setup(type, level);
+  dumpSetup(w, C, false);
+  C.S.step(2);
+  dumpState(w, 'it2', C, true);
+  w.set('hist', [{ it: 2, ...thrust(C) }]);
+  w.close();
+}
+
 const want = process.argv.slice(2);
-const all = { xcoarse: caseXcoarse, medium: caseMedium, longrun: caseLongrun };
+const all = {
+  xcoarse: caseXcoarse, medium: caseMedium,
+  longrun: () => caseLongrun('tj', 'medium', 14000, 'axi_tj_medium_longrun'),
+  hbtf_xcoarse: () => caseShort('hbtf', 'xcoarse'), lbtf_xcoarse: () => caseShort('lbtf', 'xcoarse'),
+  hbtf_medium: () => caseSetupOnly('hbtf', 'medium'),
+  longrun_hbtf: () => caseLongrun('hbtf', 'medium', 14000, 'axi_hbtf_medium_longrun'),
+  longrun_lbtf: () => caseLongrun('lbtf', 'xcoarse', 10000, 'axi_lbtf_xcoarse_longrun'),
+};
 for (const k of (want.length ? want : ['xcoarse', 'medium'])) { const t0 = Date.now(); all[k](); console.log(`${k}: ${((Date.now() - t0) / 1000).toFixed(1)} s`); }

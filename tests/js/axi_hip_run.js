@@ -29,7 +29,7 @@ while (it < iters) {
   if (ctlOn) {
     Q = S.read();
     const f0 = P.cfdForces(grid, zones, Q, comps, s.ctl.P0, 0, { S: S.src, F: S.mfx });
-    P.cfdControl('tj', s.ctl, grid, zones, Q, S.mfx, comps, f0.pw, s.ctl.etam, 250);
+    P.cfdControl(s.type || 'tj', s.ctl, grid, zones, Q, S.mfx, comps, f0.pw, s.ctl.etam, 250);
     S.setComps(comps);
   }
   if (it % every === 0 || it === iters) {

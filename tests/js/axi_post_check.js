@@ -21,5 +21,5 @@ const out = { F: f.F, Fb: f.Fb, Fw: f.Fw, mfuel: f.mfuel, pw: Array.from(f.pw) }
 const sec = P.cfdSection(grid, zones, Q, s.ctl.comb.x1 - 0.004 * s.ctl.s, s.ctl.order.indexOf(s.ctl.comb.stream), src.F);
 const f0 = P.cfdForces(grid, zones, Q, comps, s.ctl.P0, 0, src);
 const c2 = Float64Array.from(comps);
-const ctl = P.cfdControl('tj', s.ctl, grid, zones, Q, src.F, c2, f0.pw, s.ctl.etam, 250);
+const ctl = P.cfdControl(s.type || 'tj', s.ctl, grid, zones, Q, src.F, c2, f0.pw, s.ctl.etam, 250);
 console.log(JSON.stringify({ thrust: out, section: sec, control: { Tt4: ctl.Tt4, Tt7: ctl.Tt7, ratios: ctl.ratios, comps_after: Array.from(c2) } }));

@@ -23,12 +23,16 @@ from axi_case import load, rel_diff
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_axi_oracle_matches_reference():
-    g, z, Q0, comps, xf, rf = load("axi_tj_xcoarse")
+@pytest.mark.parametrize("case,iters", [("axi_tj_xcoarse", (10, 100, 500)), ("axi_hbtf_xcoarse", (10, 100)),
+                                        ("axi_lbtf_xcoarse", (10, 100))])
+def test_axi_oracle_matches_reference(case, iters):
+    """TJ (one stream), HBTF (inlet / core / bypass streams, fan and booster
+    rows, split cells) and LBTF (fan, mixed exhaust)."""
+    g, z, Q0, comps, xf, rf = load(case)
     s = g.s
     o = OA.OracleAxi(s["W"], s["H"], xf, rf, z, comps, s["fs"], list(Q0), dict(s["opt"], hf=s["hf"]))
     done = 0
-    for it in (10, 100, 500):
+    for it in iters:
         o.step(it - done)
         done = it
         Q, src, mfx, dt = o.read()
@@ -40,13 +44,14 @@ def test_axi_oracle_matches_reference():
 
 
 @pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
-def test_js_post_bit_exact_on_reference_states():
-    g, *_ = load("axi_tj_xcoarse")
-    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "axi_post_check.js"), g.dir, "it500"],
+@pytest.mark.parametrize("case,it", [("axi_tj_xcoarse", 500), ("axi_hbtf_xcoarse", 100), ("axi_lbtf_xcoarse", 100)])
+def test_js_post_bit_exact_on_reference_states(case, it):
+    g, *_ = load(case)
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "axi_post_check.js"), g.dir, f"it{it}"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     out = json.loads(r.stdout)
-    h = [x for x in g.s["hist"] if x["it"] == 500][0]
+    h = [x for x in g.s["hist"] if x["it"] == it][0]
     for k in ("F", "Fb", "Fw", "mfuel", "pw"):
         assert out["thrust"][k] == h[k], k
     assert out["section"] == g.s["section"]
