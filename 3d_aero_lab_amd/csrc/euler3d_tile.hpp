@@ -298,23 +298,32 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
     face_flux<2>(Pz[1], sz0, Tz[1], Pz[2], sz1n, Tz[2], A1z, Fzp);
     // own apertures into registers (LDS apertures are rewritten by the next plane's load)
     const float ax_m = sm.A0[ty][tx + 1], ax_p = sm.A0[ty][tx + 2], ay_m = sm.A1[ty + 1][tx], ay_p = sm.A1[ty + 2][tx];
+    // the cell update's loads go out before the barrier (their latency overlaps it)
+    float Qc[5], Bq[5], phq = 1.f, sUpq = 0.f, dzq = 1.f;
+    if (own) {
+#pragma unroll
+      for (int c = 0; c < 5; c++) Qc[c] = qin[c * ns + qk];
+      if (qin == bse) {            // stage 1: the base state is the input state (one read)
+#pragma unroll
+        for (int c = 0; c < 5; c++) Bq[c] = Qc[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 5; c++) Bq[c] = bse[c * ns + qk];
+      }
+      phq = cv.phi[qk]; sUpq = ap3[qk]; dzq = cv.dz[k];
+    }
     __syncthreads();
 
     // ---------------- cell update
     if (own) {
       const unsigned q = qk;
       const int tq = Tz[1];
-      if (tq != 0) {
-        if (tq >= 3) {
+      if (tq != 0) {               // solid and fixed cells keep their state (a fixed cell holds its fixed state)
 #pragma unroll
-          for (int c = 0; c < 5; c++) out[c * ns + q] = kc.fixq[tq - 3][c];
-        } else {
-#pragma unroll
-          for (int c = 0; c < 5; c++) out[c * ns + q] = bse[c * ns + q];
-        }
+        for (int c = 0; c < 5; c++) out[c * ns + q] = Bq[c];
         if (dtout) dtout[q] = 0.f;
       } else {
-        const float ihz = frcp(cv.dz[k]);
+        const float ihz = frcp(dzq);
         const float *C = Pz[1];
         const float pC = C[4];
         float R[5];
@@ -337,7 +346,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         for (int c = 0; c < 5; c++) R[c] += (A0z * Fzm[c]) * ihz;
         R[3] += pC * (A1z - A0z) * ihz;
         // baffle transpiration (src/air_cfd.js:226-233)
-        const float sUp = ap3[q], sDn = sUpPrev;
+        const float sUp = sUpq, sDn = sUpPrev;
         if (sUp != 0.f || sDn != 0.f) {
           const float r = C[0], u = C[1], v = C[2], w = C[3];
           const float Hh = (pC * 2.5f + 0.5f * r * (u * u + v * v + w * w) + pC) * frcp(r);
@@ -345,17 +354,8 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
           const float mh = m * ihz;
           R[0] += mh; R[1] += mh * u; R[2] += mh * v; R[3] += mh * w; R[4] += mh * Hh;
         }
-        const float ph = cv.phi[q], iph = frcp(ph);
-        float Qc[5], Bq[5], Bw[5];
-#pragma unroll
-        for (int c = 0; c < 5; c++) Qc[c] = qin[c * ns + q];
-        if (qin == bse) {            // stage 1: the base state is the input state (one read)
-#pragma unroll
-          for (int c = 0; c < 5; c++) Bq[c] = Qc[c];
-        } else {
-#pragma unroll
-          for (int c = 0; c < 5; c++) Bq[c] = bse[c * ns + q];
-        }
+        const float ph = phq, iph = frcp(ph);
+        float Bw[5];
         prim_fast(Bq[0], Bq[1], Bq[2], Bq[3], Bq[4], Bw);
         const float a = fsqrt(kGamma * Bw[4] * frcp(Bw[0]));
         const float dt = kc.cfl * fmaxf(ph, 0.25f) *
@@ -371,7 +371,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         for (int c = 0; c < 5; c++) out[c * ns + q] = n[c];
         if (dtout) dtout[q] = dt;
       }
-      sUpPrev = ap3[q];
+      sUpPrev = sUpq;
     }
     // ---------------- shift the z queue
 #pragma unroll
