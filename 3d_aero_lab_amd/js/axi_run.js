@@ -4,27 +4,37 @@
  * on AxiHIP: n iterations in chunks of 250 with the exit-temperature control
  * after every chunk, thrust and station averages at the end.
  *
- *   node axi_run.js <case dir> <iters> [report every]
+ *   node axi_run.js <case dir> <iters> [report every] [spec]
  * <case dir> holds the solver inputs (grid faces xf / rf, zones z_*, comps,
- * initial state Q0_*, freestream, control fields) as written by
- * tests/golden/make_golden_axi.js from the reference's setup.
+ * initial state Q0_*, freestream, control fields, engine description) as
+ * written by tests/golden/make_golden_axi.js from the reference's setup; with
+ * `spec` the grid, zones and constants are built from the engine description
+ * (cfd_grid.js) instead of read.
  * Prints one JSON line: {hist, thrust, stations, ms_per_iter_device_wall}.
  * ========================================================================== */
 const fs = require('fs'), path = require('path'), zlib = require('zlib');
 const P = require('./cfd_post.js');
 const { AxiHIP, axiGrid } = require('./cfd_axi.js');
 
-function loadCase(dir) {
+/* Solver inputs of a case directory. With fromSpec, the grid, zones and
+ * component constants are built by cfd_grid.js from the case's engine
+ * description (`axispec`) and only the initial state is read. */
+function loadCase(dir, fromSpec) {
   const m = JSON.parse(fs.readFileSync(path.join(dir, 'manifest.json'), 'utf8'));
   const T = { f4: Float32Array, f8: Float64Array, u1: Uint8Array, i1: Int8Array };
   const arr = n => { const a = m.arrays[n]; const b = zlib.gunzipSync(fs.readFileSync(path.join(dir, a.file))); return new T[a.dtype](b.buffer.slice(b.byteOffset, b.byteOffset + b.byteLength)); };
+  const Q0 = [0, 1, 2, 3, 4].map(q => arr(`Q0_${q}`));
+  if (fromSpec) {
+    const { grid, zones, comps } = require('./cfd_grid.js').axiSetup(m.scalars.axispec);
+    return { s: m.scalars, grid, zones, comps, Q0 };
+  }
   const zones = {};
   for (const k of ['solid', 'phi', 'ae', 'an', 'comp', 'wgt', 'lmix', 'tgt', 'rate', 'yrate', 'mfd', 'stream']) zones[k] = arr('z_' + k);
-  return { s: m.scalars, grid: axiGrid(arr('xf'), arr('rf')), zones, comps: Float64Array.from(arr('comps')), Q0: [0, 1, 2, 3, 4].map(q => arr(`Q0_${q}`)) };
+  return { s: m.scalars, grid: axiGrid(arr('xf'), arr('rf')), zones, comps: Float64Array.from(arr('comps')), Q0 };
 }
 
-function run(dir, iters, every) {
-  const C = loadCase(dir), s = C.s, comps = C.comps;
+function run(dir, iters, every, fromSpec) {
+  const C = loadCase(dir, fromSpec), s = C.s, comps = C.comps;
   const S = new AxiHIP(C.grid, C.zones, comps, s.fs, C.Q0, Object.assign({}, s.opt, { hf: s.hf }));
   const hist = [];
   let it = 0, Q = null;
@@ -54,7 +64,7 @@ function run(dir, iters, every) {
 }
 
 if (require.main === module) {
-  const [dir, iters, every] = process.argv.slice(2);
-  console.log(JSON.stringify(run(dir, +iters, +(every || 2000))));
+  const [dir, iters, every, setup] = process.argv.slice(2);
+  console.log(JSON.stringify(run(dir, +iters, +(every || 2000), setup === 'spec')));
 }
 module.exports = { loadCase, run };

@@ -8,7 +8,7 @@
  *
  *   node tests/golden/make_golden_axi.js [case ...]
  * cases: xcoarse medium longrun (TJ), hbtf_xcoarse hbtf_medium lbtf_xcoarse,
- * longrun_hbtf longrun_lbtf (the README.md:71-75 validation rows)
+ * longrun_hbtf longrun_lbtf (the README.md:71-75 validation rows), setups
  *
  * The reference's own harness (tools/cfdlib.js:6-20) uses `??`, which this
  * Node (v12) cannot parse; the same setup sequence is written out here:
@@ -62,8 +62,32 @@ class Writer {
 }
 const ZONE_KEYS = ['solid', 'phi', 'ae', 'an', 'comp', 'wgt', 'lmix', 'tgt', 'rate', 'yrate', 'mfd', 'stream'];
 
+/* The engine geometry and 1D operating point the axisymmetric setup reads
+ * (buildCFDGrid / buildZones / compParams, src/cfd_grid.js:28-276), as data:
+ * stream walls and liners as their polyline points, body polygons, blade rows,
+ * combustor / afterburner extents, exits, cycle temperatures and mass flows. */
+function axiSpec(C) {
+  const sp = C.spec, op = C.op, lp = L => L.pts.map(p => [p[0], p[1]]);
+  const streams = {};
+  for (const k of sp.order) { const st = sp.streams[k]; streams[k] = { x0: st.x0, x1: st.x1, lo: lp(st.lo), hi: lp(st.hi) }; }
+  const opk = {};
+  for (const k of ['Tt2', 'Tt3', 'Tt4', 'Tt5', 'Tt6', 'Tt7', 'Tt13', 'Tt25', 'Tt45', 'abOn', 'mdot', 'f', 'mcore', 'mbyp', 'm7'])
+    if (op[k] !== undefined && op[k] !== null && !(typeof op[k] === 'number' && isNaN(op[k]))) opk[k] = op[k];
+  return {
+    type: sp.type, s: sp.s, level: C.grid.level, domain: { xb: sp.domain.xb, rb: sp.domain.rb }, view: { len: sp.view.len, Rmax: sp.view.Rmax },
+    order: sp.order, streams,
+    bodies: sp.bodies.map(b => ({ key: b.key, mat: b.mat, poly: b.poly })),
+    rows: sp.rows.map(r => ({ x0: r.x0, x1: r.x1, stream: r.stream, rotor: !!r.rotor, comp: r.comp })),
+    comb: { xd: sp.comb.xd, x1: sp.comb.x1, stream: sp.comb.stream, linerIn: lp(sp.comb.linerIn), linerOut: lp(sp.comb.linerOut) },
+    ab: sp.ab ? { xFH: sp.ab.xFH, x1: sp.ab.x1, stream: sp.ab.stream, rings: sp.ab.rings } : null,
+    exitsGeom: sp.exitsGeom.map(e => ({ x: e.x, R: e.R })),
+    op: opk, eff: { ef: C.P.eff.ef, ec: C.P.eff.ec, et: C.P.eff.et },
+  };
+}
+
 function dumpSetup(w, C, q0hash) {
   const g = C.grid;
+  w.set('axispec', axiSpec(C));
   w.arr('xf', g.xf); w.arr('rf', g.rf);
   for (const k of ZONE_KEYS) w.arr('z_' + k, C.zones[k]);
   w.arr('comps', C.comps);
@@ -176,12 +200,27 @@ function caseSetupOnly(type, level) {
   w.close();
 }
 
+/* setups only (engine description + sha256 of every setup array) of the
+ * three engines at every grid level: the setup restatement's fixtures */
+function caseSetups() {
+  const out = {};
+  for (const type of ['tj', 'hbtf', 'lbtf']) for (const level of ['xcoarse', 'coarse', 'medium', 'fine']) {
+    const C = setup(type, level), z = C.zones, h = {};
+    h.xf = sha(Buffer.from(C.grid.xf.buffer)); h.rf = sha(Buffer.from(C.grid.rf.buffer)); h.comps = sha(Buffer.from(C.comps.buffer));
+    for (const k of ZONE_KEYS) h['z_' + k] = sha(Buffer.from(z[k].buffer, z[k].byteOffset, z[k].byteLength));
+    out[`${type}_${level}`] = { W: C.grid.W, H: C.grid.H, axispec: axiSpec(C), sha256: h };
+  }
+  fs.mkdirSync(path.join(OUT, 'axi_setups'), { recursive: true });
+  fs.writeFileSync(path.join(OUT, 'axi_setups', 'manifest.json'), JSON.stringify({ case: 'axi_setups', setups: out }));
+  console.log('wrote axi_setups');
+}
+
 const want = process.argv.slice(2);
 const all = {
   xcoarse: caseXcoarse, medium: caseMedium,
   longrun: () => caseLongrun('tj', 'medium', 14000, 'axi_tj_medium_longrun'),
   hbtf_xcoarse: () => caseShort('hbtf', 'xcoarse'), lbtf_xcoarse: () => caseShort('lbtf', 'xcoarse'),
-  hbtf_medium: () => caseSetupOnly('hbtf', 'medium'),
+  hbtf_medium: () => caseSetupOnly('hbtf', 'medium'), setups: caseSetups,
   longrun_hbtf: () => caseLongrun('hbtf', 'medium', 14000, 'axi_hbtf_medium_longrun'),
   longrun_lbtf: () => caseLongrun('lbtf', 'xcoarse', 10000, 'axi_lbtf_xcoarse_longrun'),
 };

@@ -1,0 +1,27 @@
+"""Axisymmetric setup restatement (3d_aero_lab_amd/js/cfd_grid.js: buildCFDGrid,
+buildZones, updateHeatZones, streamMassFlows, compParams; src/cfd_grid.js:11-276)
+against the reference's own setups: grid faces, every zone array and the
+component constants byte-identical (sha256) for the turbojet, the high- and
+low-bypass turbofans at all four grid levels (tests/golden/axi_setups) and for
+the five setups the solver fixtures carry."""
+import json
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CASES = ["axi_setups", "axi_tj_xcoarse", "axi_tj_medium", "axi_hbtf_xcoarse", "axi_hbtf_medium", "axi_lbtf_xcoarse"]
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+@pytest.mark.parametrize("case", CASES)
+def test_axi_setup_byte_identical(case):
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "axi_setup_check.js"), os.path.join(ROOT, "tests", "golden", case)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    res = json.loads(r.stdout)
+    assert res
+    bad = {name: [k for k, ok in v.items() if not ok] for name, v in res.items() if not all(v.values())}
+    assert not bad, bad
