@@ -30,6 +30,23 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// Global access by a 32-bit element index from a wave-uniform base: the byte
+// offset is formed in 32 bits, so the access takes the scalar-base + vector-
+// offset form with no 64-bit address arithmetic per access (planes of up to
+// 2^30 cells).
+template <class T>
+__device__ __forceinline__ T gld(const T *__restrict__ base, unsigned i) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + i * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void gst(T *__restrict__ base, unsigned i, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + i * (unsigned)sizeof(T)) = v;
+}
+
+// minmod(a, b) = 0 when a b <= 0, else the one of smaller magnitude
+// (src/air_cfd.js:196): the median of a, b and 0, one instruction.
+__device__ __forceinline__ float mmod3(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, 0.f); }
+
 // Primitives (rho, u, v, w, p) of conserved a[] (cfdPrim5, src/air_cfd.js:169).
 __device__ __forceinline__ void prim_fast(float a0, float a1, float a2, float a3, float a4, float *o) {
   const float r = fmaxf(a0, 1e-6f), ir = frcp(r), u = a1 * ir, v = a2 * ir, w = a3 * ir;
@@ -101,7 +118,7 @@ __device__ __forceinline__ void face_flux(const float *PL, const float *sL, int 
 // zero unless the three cells are fluid and both faces of c fully open.
 __device__ __forceinline__ void slope5(const float *Pm, const float *P0, const float *Pp, bool ok, float *s) {
 #pragma unroll
-  for (int v = 0; v < 5; v++) s[v] = ok ? mmod(P0[v] - Pm[v], Pp[v] - P0[v]) : 0.f;
+  for (int v = 0; v < 5; v++) s[v] = ok ? mmod3(P0[v] - Pm[v], Pp[v] - P0[v]) : 0.f;
 }
 
 template <int TX, int TY>
@@ -132,9 +149,9 @@ __device__ __forceinline__ int load_prim(const CaseView &cv, const StageConsts &
   const bool mirror = j < 0;
   const int jj = mirror ? -1 - j : j;
   const unsigned q = inbox ? (unsigned)((k * cv.NY + jj) * cv.NX + i) : 0u;
-  const int tq = cv.t[q];
+  const int tq = gld(cv.t, q);
   float w[5];
-  prim_fast(qin[q], qin[ns + q], qin[2 * ns + q], qin[3 * ns + q], qin[4 * ns + q], w);
+  prim_fast(gld(qin, q), gld(qin + ns, q), gld(qin + 2 * ns, q), gld(qin + 3 * ns, q), gld(qin + 4 * ns, q), w);
 #pragma unroll
   for (int v = 0; v < 5; v++) {
     const float x = !inbox ? kc.Fw[v] : w[v];
@@ -159,7 +176,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
   const float ihx = own ? frcp(cv.dx[i]) : 0.f, ihy = own ? frcp(cv.dy[j]) : 0.f;
   const float *__restrict__ ap0 = cv.ap, *__restrict__ ap1 = cv.ap + cv.N, *__restrict__ ap2 = cv.ap + 2 * (size_t)cv.N,
                             *__restrict__ ap3 = cv.ap + 3 * (size_t)cv.N;
-  auto apz = [&](int k) -> float { return (k < 0 || k >= NZ || !own) ? 1.f : ap2[(unsigned)(k * NXY) + qc]; };
+  auto apz = [&](int k) -> float { return (k < 0 || k >= NZ || !own) ? 1.f : gld(ap2, (unsigned)(k * NXY) + qc); };
 
   // ---- z queue preamble: planes k0-2 .. k0+1
   float Pz[4][5];          // primitives of planes k-1, k, k+1, k+2 (own column)
@@ -177,7 +194,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
 #pragma unroll
   for (int v = 0; v < 5; v++) sz0[v] = sz1[v];                                                              // sz0 = s(k)
   // queue: Pz[0] = k-1, Pz[1] = k, Pz[2] = k+1; Pz[3] loaded each plane
-  float sUpPrev = (own && k0 > 0) ? ap3[(unsigned)((k0 - 1) * NXY) + qc] : 0.f;
+  float sUpPrev = (own && k0 > 0) ? gld(ap3, (unsigned)((k0 - 1) * NXY) + qc) : 0.f;
 
   for (int k = k0; k < k1; k++) {
     // ---------------- load phase
@@ -200,14 +217,14 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
     }
     // apertures: own x / y faces, plus the two faces left of / below the tile
     const unsigned qk = (unsigned)(k * NXY) + qc;
-    sm.A0[ty][tx + 2] = own ? ap0[qk] : 1.f;
-    sm.A1[ty + 2][tx] = own ? ap1[qk] : 1.f;
+    sm.A0[ty][tx + 2] = own ? gld(ap0, qk) : 1.f;
+    sm.A1[ty + 2][tx] = own ? gld(ap1, qk) : 1.f;
     if (tid < 2 * TY) {
       const int r = tid >> 1, ii = (tid & 1) - 2, gi = i0 + ii, gj = j0 + r;
-      sm.A0[r][ii + 2] = (gi < 0 || gi >= NX || gj >= NY) ? 1.f : ap0[(unsigned)((k * NY + gj) * NX + gi)];
+      sm.A0[r][ii + 2] = (gi < 0 || gi >= NX || gj >= NY) ? 1.f : gld(ap0, (unsigned)((k * NY + gj) * NX + gi));
     } else if (tid >= 64 && tid < 64 + 2 * TX) {
       const int u = tid - 64, c = u >> 1, jj = (u & 1) - 2, gi = i0 + c, gj = j0 + jj;
-      sm.A1[jj + 2][c] = (gj < 0 || gj >= NY || gi >= NX) ? 1.f : ap1[(unsigned)((k * NY + gj) * NX + gi)];
+      sm.A1[jj + 2][c] = (gj < 0 || gj >= NY || gi >= NX) ? 1.f : gld(ap1, (unsigned)((k * NY + gj) * NX + gi));
     }
     __syncthreads();
 
@@ -239,7 +256,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         // faces of cell cx: ap0 of cells cx-1 and cx; cell TX's + face lies outside the tile
         const int gi = i0 + cx, gj = j0 + r;
         const float am = sm.A0[r][cx + 1];
-        const float apf = (cx == -1) ? sm.A0[r][cx + 2] : ((gi < 0 || gi >= NX || gj >= NY) ? 1.f : ap0[(unsigned)((k * NY + gj) * NX + gi)]);
+        const float apf = (cx == -1) ? sm.A0[r][cx + 2] : ((gi < 0 || gi >= NX || gj >= NY) ? 1.f : gld(ap0, (unsigned)((k * NY + gj) * NX + gi)));
         const bool ok = o2 && sm.T[yj][xi] == 0 && sm.T[yj][xi - 1] == 0 && sm.T[yj][xi + 1] == 0 && am == 1.f && apf == 1.f;
         slope5(Pm, P0, Pp, ok, s);
 #pragma unroll
@@ -250,7 +267,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         for (int v = 0; v < 5; v++) { P0[v] = sm.P[v][yj][xi]; Pm[v] = sm.P[v][yj - 1][xi]; Pp[v] = sm.P[v][yj + 1][xi]; }
         const int gi = i0 + c, gj = j0 + ry;
         const float am = sm.A1[ry + 1][c];
-        const float apf = (ry == -1) ? sm.A1[ry + 2][c] : ((gj < 0 || gj >= NY || gi >= NX) ? 1.f : ap1[(unsigned)((k * NY + gj) * NX + gi)]);
+        const float apf = (ry == -1) ? sm.A1[ry + 2][c] : ((gj < 0 || gj >= NY || gi >= NX) ? 1.f : gld(ap1, (unsigned)((k * NY + gj) * NX + gi)));
         const bool ok = o2 && sm.T[yj][xi] == 0 && sm.T[yj - 1][xi] == 0 && sm.T[yj + 1][xi] == 0 && am == 1.f && apf == 1.f;
         slope5(Pm, P0, Pp, ok, s);
 #pragma unroll
@@ -302,15 +319,15 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
     float Qc[5], Bq[5], phq = 1.f, sUpq = 0.f, dzq = 1.f;
     if (own) {
 #pragma unroll
-      for (int c = 0; c < 5; c++) Qc[c] = qin[c * ns + qk];
+      for (int c = 0; c < 5; c++) Qc[c] = gld(qin + c * ns, qk);
       if (qin == bse) {            // stage 1: the base state is the input state (one read)
 #pragma unroll
         for (int c = 0; c < 5; c++) Bq[c] = Qc[c];
       } else {
 #pragma unroll
-        for (int c = 0; c < 5; c++) Bq[c] = bse[c * ns + qk];
+        for (int c = 0; c < 5; c++) Bq[c] = gld(bse + c * ns, qk);
       }
-      phq = cv.phi[qk]; sUpq = ap3[qk]; dzq = cv.dz[k];
+      phq = gld(cv.phi, qk); sUpq = gld(ap3, qk); dzq = cv.dz[k];
     }
     __syncthreads();
 
@@ -320,8 +337,8 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       const int tq = Tz[1];
       if (tq != 0) {               // solid and fixed cells keep their state (a fixed cell holds its fixed state)
 #pragma unroll
-        for (int c = 0; c < 5; c++) out[c * ns + q] = Bq[c];
-        if (dtout) dtout[q] = 0.f;
+        for (int c = 0; c < 5; c++) gst(out + c * ns, q, Bq[c]);
+        if (dtout) gst(dtout, q, 0.f);
       } else {
         const float ihz = frcp(dzq);
         const float *C = Pz[1];
@@ -368,8 +385,8 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         const float ke = 0.5f * (n[1] * n[1] + n[2] * n[2] + n[3] * n[3]) * frcp(n[0]);
         if (kGm1 * (n[4] - ke) < 1e-5f) n[4] = 2.5e-5f + ke;
 #pragma unroll
-        for (int c = 0; c < 5; c++) out[c * ns + q] = n[c];
-        if (dtout) dtout[q] = dt;
+        for (int c = 0; c < 5; c++) gst(out + c * ns, q, n[c]);
+        if (dtout) gst(dtout, q, dt);
       }
       sUpPrev = sUpq;
     }
