@@ -57,7 +57,10 @@ __device__ __forceinline__ void prim_fast(float a0, float a1, float a2, float a3
 // HLLC flux along axis d (src/air_cfd.js:179-195) with four hardware
 // transcendentals per face: two reciprocal square roots for the sound speeds,
 // the reciprocals of the contact-speed denominator and of S - SM (the star
-// energy is regrouped so that no 1/r or 1/(S - u) is needed).
+// energy is regrouped so that no 1/r or 1/(S - u) is needed). The momentum
+// fluxes are the mass flux times the upwind velocity components plus the
+// pressure (and, in the star region, S (f SM - r u)) on the normal one: the
+// reference's F + S (U* - U) with the common factor taken out.
 template <int d>
 __device__ __forceinline__ void hllc_fast(const float *L, const float *R, float *out) {
   const float rL = L[0], rR = R[0], pL = L[4], pR = R[4], uL = L[1 + d], uR = R[1 + d];
@@ -66,30 +69,29 @@ __device__ __forceinline__ void hllc_fast(const float *L, const float *R, float 
   const float EL = pL * 2.5f + 0.5f * rL * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]);
   const float ER = pR * 2.5f + 0.5f * rR * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]);
   const float SL = fminf(uL - aL, uR - aR), SR = fmaxf(uL + aL, uR + aR);
-  if (SL >= 0.f || SR <= 0.f) {
+  if (SL >= 0.f || SR <= 0.f) {              // supersonic: the upwind side's flux
     const bool left = SL >= 0.f;
-    const float k0 = left ? L[0] : R[0], k1 = left ? L[1] : R[1], k2 = left ? L[2] : R[2], k3 = left ? L[3] : R[3];
-    const float k4 = left ? pL : pR, uK = left ? uL : uR, E = left ? EL : ER;
-    out[0] = k0 * uK; out[1] = k0 * k1 * uK; out[2] = k0 * k2 * uK; out[3] = k0 * k3 * uK;
-    out[1 + d] += k4;
-    out[4] = uK * (E + k4);
+    const float r = left ? rL : rR, u = left ? uL : uR, p = left ? pL : pR, E = left ? EL : ER;
+    const float m = r * u;
+    out[0] = m;
+#pragma unroll
+    for (int c = 1; c < 4; c++) out[c] = m * (left ? L[c] : R[c]);
+    out[1 + d] += p;
+    out[4] = u * (E + p);
     return;
   }
   const float SM = (pR - pL + rL * uL * (SL - uL) - rR * uR * (SR - uR)) * frcp(rL * (SL - uL) - rR * (SR - uR));
   const bool left = SM >= 0.f;
-  const float k1 = left ? L[1] : R[1], k2 = left ? L[2] : R[2], k3 = left ? L[3] : R[3];
   const float S = left ? SL : SR, uK = left ? uL : uR, rK = left ? rL : rR, pK = left ? pL : pR, EK = left ? EL : ER;
   const float SmU = S - uK, iSS = frcp(S - SM);
-  const float f = rK * SmU * iSS;
-  float Fm[4] = {rK * uK, rK * k1 * uK, rK * k2 * uK, rK * k3 * uK};
-  Fm[1 + d] += pK;
-  const float Um[4] = {rK, rK * k1, rK * k2, rK * k3};
-  float Us[4] = {f, f * k1, f * k2, f * k3};
-  Us[1 + d] = f * SM;
+  const float f = rK * SmU * iSS, ru = rK * uK;
+  const float m = ru + S * (f - rK);         // mass flux rK uK + S (f - rK)
+  out[0] = m;
+#pragma unroll
+  for (int c = 1; c < 4; c++) out[c] = m * (left ? L[c] : R[c]);
+  out[1 + d] = ru * uK + pK + S * (f * SM - ru);
   // f (E/r + (SM - u)(SM + p / (r (S - u)))) with f = r (S - u) / (S - SM)
   const float Es = EK * SmU * iSS + (SM - uK) * (f * SM + pK * iSS);
-#pragma unroll
-  for (int c = 0; c < 4; c++) out[c] = Fm[c] + S * (Us[c] - Um[c]);
   out[4] = uK * (EK + pK) + S * (Es - EK);
 }
 
@@ -107,9 +109,12 @@ __device__ __forceinline__ void face_flux(const float *PL, const float *sL, int 
   float L[5], R[5];
 #pragma unroll
   for (int v = 0; v < 5; v++) { L[v] = PL[v] + 0.5f * sL[v]; R[v] = PR[v] - 0.5f * sR[v]; }
-  if (L[0] <= 0.f || L[4] <= 0.f || R[0] <= 0.f || R[4] <= 0.f) {
+  // first-order fallback where a reconstructed density or pressure is not
+  // positive; rare, so the selections sit behind a wave-uniform test
+  const bool bad = L[0] <= 0.f || L[4] <= 0.f || R[0] <= 0.f || R[4] <= 0.f;
+  if (__builtin_amdgcn_ballot_w64(bad)) {
 #pragma unroll
-    for (int v = 0; v < 5; v++) { L[v] = PL[v]; R[v] = PR[v]; }
+    for (int v = 0; v < 5; v++) { L[v] = bad ? PL[v] : L[v]; R[v] = bad ? PR[v] : R[v]; }
   }
   hllc_fast<d>(L, R, F);
 }

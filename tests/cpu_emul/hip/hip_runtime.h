@@ -23,6 +23,8 @@ inline emul_dim3 emul_grid_dim() { emul_dim3 d; d.x = 1; d.y = 1; d.z = 1; retur
 // one OS thread per GPU thread: a wavefront shuffle has no CPU meaning here; the
 // kernels that use it are not run by the CPU harness
 template <class T> inline T __shfl_down(T v, int, int) { return v; }
+// a wave vote used only as a guard around per-lane selections: the lane's own vote gives the same result
+inline uint64_t __builtin_amdgcn_ballot_w64(bool b) { return b ? 1u : 0u; }
 extern thread_local std::barrier<> *emul_block_barrier;
 inline void __syncthreads() { emul_block_barrier->arrive_and_wait(); }
 inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
