@@ -109,6 +109,50 @@ function cfdControl(type, ctl, grid, zones, Q, mfx, comps, pw, etam, dIt) {
   return out;
 }
 
+/* Display fields of an axisymmetric state (cfdDerived, src/cfd_post.js:10-40):
+ * region (0 exterior, 2 + stream), material, T, Tt, P, Pt, |V|, M, rho, Vx,
+ * Vth (0), Vr, Y, entropy change ds against the freestream, and the azimuthal
+ * vorticity du/dr - dv/dx by central differences over fluid neighbours.
+ * disp: cut cells under half full show as solid (the mask is cached on zones
+ * as dsolid). G: a previous result of the same size to refill, or null. */
+const DERIVED_KEYS = ['T', 'Tt', 'P', 'Pt', 'V', 'M', 'rho', 'Vx', 'Vth', 'Vr', 'Y', 'vort', 'ds'];
+function cfdDerived(grid, zones, Q, fl, G, disp) {
+  const { W, H, N, rc } = grid;
+  if (!G || G.N !== N) {
+    G = { W, H, N, reg: new Uint8Array(N), mat: new Uint8Array(N) };
+    for (const k of DERIVED_KEYS) G[k] = new Float32Array(N);
+  }
+  const S = disp ? (zones.dsolid || (zones.dsolid = Uint8Array.from(zones.solid, (v, k) => v || zones.phi[k] < 0.5 ? 1 : 0))) : zones.solid;
+  const T0 = fl.T0, P0 = fl.P0;
+  for (let k = 0; k < N; k++) {
+    if (S[k]) {
+      G.reg[k] = 1; G.mat[k] = zones.mat[k];
+      for (const q of ['T', 'Tt', 'P', 'Pt', 'V', 'M', 'rho', 'Vx', 'Vr', 'Y', 'vort', 'ds']) G[q][k] = NaN;
+      continue;
+    }
+    const st = zones.stream[k];
+    G.reg[k] = st < 0 ? 0 : 2 + st;
+    const r = Math.max(Q[0][k], 1e-4), u = Q[1][k] / r, v = Q[2][k] / r, Y = clamp(Q[4][k] / r, 0, 1), g = 1.4 - 0.07 * Y;
+    const p = Math.max(1, (g - 1) * (Q[3][k] - 0.5 * r * (u * u + v * v)));
+    const T = p / (r * RGAS), cp = g * RGAS / (g - 1), V2 = u * u + v * v, a = Math.sqrt(g * p / r);
+    const Tt = T + V2 / (2 * cp);
+    G.T[k] = T; G.P[k] = p; G.rho[k] = r; G.Vx[k] = u; G.Vr[k] = v; G.Vth[k] = 0;
+    G.V[k] = Math.sqrt(V2); G.M[k] = Math.sqrt(V2) / a;
+    G.Tt[k] = Tt; G.Pt[k] = p * Math.pow(Tt / T, g / (g - 1)); G.Y[k] = Y;
+    G.ds[k] = cp * Math.log(T / T0) - RGAS * Math.log(p / P0);
+  }
+  for (let j = 0; j < H; j++) for (let i = 0; i < W; i++) {
+    const k = j * W + i;
+    if (S[k]) continue;
+    const kl = i > 0 && !S[k - 1] ? k - 1 : k, kr = i < W - 1 && !S[k + 1] ? k + 1 : k;
+    const kd = j > 0 && !S[k - W] ? k - W : k, ku = j < H - 1 && !S[k + W] ? k + W : k;
+    const dxx = grid.xc[kr % W] - grid.xc[kl % W], drr = rc[(ku / W) | 0] - rc[(kd / W) | 0];
+    const dvdx = dxx > 0 ? (G.Vr[kr] - G.Vr[kl]) / dxx : 0, dudr = drr > 0 ? (G.Vx[ku] - G.Vx[kd]) / drr : 0;
+    G.vort[k] = dudr - dvdx;
+  }
+  return G;
+}
+
 /* Station averages of an engine flowpath (tools/stations.js:8-16): at each
  * station column, mass flow and mass-weighted Tt, Pt over the station's
  * stream, with the cell mass flow taken from the mean axial face mass flux.
@@ -139,4 +183,4 @@ function stations(grid, zones, Q, mfx, order, list) {
   return out;
 }
 
-module.exports = { RGAS, clamp, cfdPrim, cfdForces, colAt, cfdSection, spoolRatios, cfdControl, stations, STATIONS, TJ_STATIONS };
+module.exports = { RGAS, clamp, cfdPrim, cfdDerived, DERIVED_KEYS, cfdForces, colAt, cfdSection, spoolRatios, cfdControl, stations, STATIONS, TJ_STATIONS };

@@ -6,7 +6,7 @@
  *
  *   node tests/golden/make_golden.js [case ...]
  *
- * Cases: specs sphere transport1m transport60k fighter30k interceptor
+ * Cases: specs sphere transport1m transport60k fighter30k interceptor sampling
  * Each case writes tests/golden/<case>/manifest.json plus gzip'd raw arrays.
  * The reference runs only here; its outputs are data (grid faces, masks,
  * states after N iterations, force coefficients), never its source.
@@ -35,7 +35,7 @@ class Writer {
     const buf = Buffer.from(ta.buffer, ta.byteOffset, ta.byteLength);
     const file = name + '.bin.gz';
     fs.writeFileSync(path.join(this.dir, file), zlib.gzipSync(buf, { level: 9 }));
-    const dtype = ta instanceof Float32Array ? 'f4' : ta instanceof Float64Array ? 'f8' : ta instanceof Uint8Array ? 'u1' : ta instanceof Int32Array ? 'i4' : '?';
+    const dtype = ta instanceof Float32Array ? 'f4' : ta instanceof Float64Array ? 'f8' : ta instanceof Uint8Array ? 'u1' : ta instanceof Int32Array ? 'i4' : ta instanceof Int8Array ? 'i1' : '?';
     this.m.arrays[name] = { file, dtype, n: ta.length, sha256: sha(buf) };
   }
   hash(name, ta) { const buf = Buffer.from(ta.buffer, ta.byteOffset, ta.byteLength); this.m.arrays[name] = { file: null, dtype: ta instanceof Float32Array ? 'f4' : 'u1', n: ta.length, sha256: sha(buf) }; }
@@ -191,7 +191,56 @@ function caseInterceptor() {
   w.close();
 }
 
+/* Display sampling (src/air_cfd.js:454-490, 637-646) on the fighter's state
+ * after 200 iterations: trilinear stencils at seeded probe points (with the
+ * nearest-fluid fallback), sampled primitives, dimensional states, particle
+ * velocities. Probe arrays in the callers' types (Int32 indices, Float32
+ * weights, src/air_cfd.js:603, 621). */
+function caseSampling() {
+  const w = new Writer('sampling');
+  const { ac, G } = refAircraft('fighter');
+  const save = CFD_GRIDS.coarse.cells; CFD_GRIDS.coarse.cells = 30000;
+  const g = cfdGrid(ac, 'coarse'); CFD_GRIDS.coarse.cells = save;
+  const geo = cfdGeom(ac, G, g, 3);
+  const M = 0.6, a = 3 * DEG, F = [1, M * Math.cos(a), 0, M * Math.sin(a), 1 / (CG * (CG - 1)) + 0.5 * M * M];
+  const s = new CfdCPU(g, geo); s.opt.cfl = 0.7; s.opt.order = 2; s.setFree(F); s.setFix([F, F, F]); s.reset(); s.step(200);
+  const H = s.read();
+  dumpState(w, 'it200', s);
+  const rnd = Mine.bodies.mulberry32(20261205);
+  const n = 3000, pts = new Float64Array(3 * n);
+  const X = g.X.c, Y = g.Y.c, Z = g.Z.c, L = ac.fus.L;
+  for (let p = 0; p < n; p++) {
+    let x, y, z;
+    if (p % 3 === 0) { x = L * rnd(); y = (rnd() - 0.5) * 0.2 * L; z = (rnd() - 0.5) * 0.1 * L; }   // near / inside the body
+    else { x = X[0] - 1 + (X[X.length - 1] - X[0] + 2) * rnd(); y = (2 * rnd() - 1) * (Y[Y.length - 1] + 1); z = Z[0] - 1 + (Z[Z.length - 1] - Z[0] + 2) * rnd(); }
+    pts[3 * p] = x; pts[3 * p + 1] = y; pts[3 * p + 2] = z;
+  }
+  const idx = new Int32Array(8 * n), wt = new Float32Array(8 * n), ok = new Uint8Array(n), got = new Uint8Array(n);
+  const Wv = new Float64Array(5 * n), st = new Float64Array(12 * n), vel = new Float64Array(3 * n), vok = new Uint8Array(n);
+  const fsr = { T: 288.15, a: Math.sqrt(1.4 * 287.05 * 288.15), P: 101325, rho: 1.225 };
+  fsr.V = M * fsr.a; fsr.q = 0.5 * fsr.rho * fsr.V * fsr.V;
+  const KEYS = ['P', 'T', 'rho', 'V', 'M', 'Tt', 'Pt', 'Cp', 'Taw', 'Vx', 'Vy', 'Vz'];
+  Object.assign(CFD3, { g, H, mask: { t: geo.t }, M });
+  const W = [0, 0, 0, 0, 0], o = {}, vo = [0, 0, 0];
+  for (let p = 0; p < n; p++) {
+    const x = pts[3 * p], y = pts[3 * p + 1], z = pts[3 * p + 2];
+    ok[p] = cfdStencil(g, geo.t, x, y, z, idx, wt, 8 * p) ? 1 : 0;
+    if (cfdSample(H, idx, wt, 8 * p, y < 0 ? -1 : 1, W)) {
+      got[p] = 1; for (let c = 0; c < 5; c++) Wv[5 * p + c] = W[c];
+      cfdState(fsr, W, o); KEYS.forEach((k, c) => { st[12 * p + c] = o[k]; });
+    }
+    if (cfdVelAt(x, y, z, vo)) { vok[p] = 1; vel[3 * p] = vo[0]; vel[3 * p + 1] = vo[1]; vel[3 * p + 2] = vo[2]; }
+  }
+  w.arr('t', geo.t); w.arr('fx', g.X.f); w.arr('fy', g.Y.f); w.arr('fz', g.Z.f);
+  w.arr('pts', pts); w.arr('idx', idx); w.arr('w', wt); w.arr('ok', ok); w.arr('got', got); w.arr('W', Wv); w.arr('state', st);
+  w.arr('vel', vel); w.arr('vok', vok);
+  w.set('fs', fsr); w.set('M', M); w.set('keys', KEYS); w.set('NX', g.NX); w.set('NY', g.NY); w.set('NZ', g.NZ);
+  w.set('counts', { ok: ok.reduce((a, v) => a + v, 0), got: got.reduce((a, v) => a + v, 0), vok: vok.reduce((a, v) => a + v, 0) });
+  w.close();
+}
+
 const want = process.argv.slice(2);
 const all = { specs: caseSpecs, sphere: caseSphere, transport1m: () => caseTransport('transport1m', 1.0e6, 0, [], false),
-  transport60k: () => caseTransport('transport60k', 60e3, 100, [2, 100], true), fighter30k: caseFighter, interceptor: caseInterceptor };
+  transport60k: () => caseTransport('transport60k', 60e3, 100, [2, 100], true), fighter30k: caseFighter, interceptor: caseInterceptor,
+  sampling: caseSampling };
 for (const k of (want.length ? want : Object.keys(all))) { const t0 = Date.now(); all[k](); console.log(`${k}: ${((Date.now() - t0) / 1000).toFixed(1)} s`); }

@@ -27,7 +27,7 @@ const crypto = require('crypto');
 const REF = process.env.AERO_REF || '/root/reference';
 const code = ['physics', 'geometry', 'field', 'cfd_grid', 'cfd_cpu', 'cfd_post']
   .map(f => fs.readFileSync(path.join(REF, 'src', f + '.js'), 'utf8').replace(/^if \(typeof module[^\n]*\n/gm, '')).join('\n');
-vm.runInThisContext(code + '\nglobalThis.AXI={cfdControl,solveEngine,ENGINE_TYPES,DEFAULT_EFF,buildSpec,FieldModel,buildCFDGrid,buildZones,compParams,initialState,CPUSolver,cfdForces,cfdSection,spoolRatios,RGAS};');
+vm.runInThisContext(code + '\nglobalThis.AXI={cfdDerived,cfdControl,solveEngine,ENGINE_TYPES,DEFAULT_EFF,buildSpec,FieldModel,buildCFDGrid,buildZones,compParams,initialState,CPUSolver,cfdForces,cfdSection,spoolRatios,RGAS};');
 const A = globalThis.AXI;
 const OUT = __dirname;
 
@@ -60,7 +60,7 @@ class Writer {
   set(k, v) { this.m.scalars[k] = v; }
   close() { fs.writeFileSync(path.join(this.dir, 'manifest.json'), JSON.stringify(this.m, null, 1)); console.log(`wrote ${this.dir}`); }
 }
-const ZONE_KEYS = ['solid', 'phi', 'ae', 'an', 'comp', 'wgt', 'lmix', 'tgt', 'rate', 'yrate', 'mfd', 'stream'];
+const ZONE_KEYS = ['solid', 'phi', 'ae', 'an', 'comp', 'wgt', 'lmix', 'tgt', 'rate', 'yrate', 'mfd', 'stream', 'mat'];
 
 /* The engine geometry and 1D operating point the axisymmetric setup reads
  * (buildCFDGrid / buildZones / compParams, src/cfd_grid.js:28-276), as data:
@@ -94,6 +94,7 @@ function dumpSetup(w, C, q0hash) {
   for (let q = 0; q < 5; q++) w.arr('Q0_' + q, C.Q0[q], q0hash);
   w.set('W', g.W); w.set('H', g.H); w.set('N', g.N); w.set('type', C.type);
   w.set('fs', C.fsv); w.set('hf', C.hf); w.set('opt', { cfl: 0.6, order: 2, visc: true });
+  w.set('fl', { T0: C.fl.T0, P0: C.fl.P0 });
   // what cfdControl reads from spec / op (src/cfd_post.js:110-123)
   w.set('ctl', { s: C.spec.s, comb: { x1: C.spec.comb.x1, stream: C.spec.comb.stream }, ab: C.spec.ab ? { x1: C.spec.ab.x1, stream: C.spec.ab.stream } : null,
     order: C.spec.order, Tt4: C.op.Tt4, Tt7: C.op.Tt7 || null, abOn: !!C.op.abOn, etam: C.P.eff.etam, P0: C.fl.P0, F1D: C.op.F });
@@ -122,8 +123,20 @@ function caseXcoarse() {
   }
   w.set('ms_per_iter', (Date.now() - t0) / it);
   w.set('hist', hist);
+  derived(w, C);
   controlAndSection(w, C);
   w.close();
+}
+
+/* display fields (cfdDerived, src/cfd_post.js:10-40) of the solver's current
+ * state, with and without the display solid rule, as sha256 of each array */
+const DER_KEYS = ['reg', 'mat', 'T', 'Tt', 'P', 'Pt', 'V', 'M', 'rho', 'Vx', 'Vth', 'Vr', 'Y', 'vort', 'ds'];
+function derived(w, C) {
+  for (const disp of [false, true]) {
+    const z = Object.assign({}, C.zones);   // cfdDerived caches its display mask on the zones record
+    const G = A.cfdDerived(C.grid, z, C.S.Q, C.fl, null, disp);
+    for (const k of DER_KEYS) w.arr(`der${disp ? 1 : 0}_${k}`, G[k], true);
+  }
 }
 
 /* one control step on the last state (the long-run loop calls it every 250
@@ -187,6 +200,7 @@ function caseShort(type, level) {
   }
   w.set('ms_per_iter', (Date.now() - t0) / it);
   w.set('hist', hist);
+  derived(w, C);
   controlAndSection(w, C);
   w.close();
 }

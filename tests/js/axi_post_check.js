@@ -22,4 +22,13 @@ const sec = P.cfdSection(grid, zones, Q, s.ctl.comb.x1 - 0.004 * s.ctl.s, s.ctl.
 const f0 = P.cfdForces(grid, zones, Q, comps, s.ctl.P0, 0, src);
 const c2 = Float64Array.from(comps);
 const ctl = P.cfdControl(s.type || 'tj', s.ctl, grid, zones, Q, src.F, c2, f0.pw, s.ctl.etam, 250);
-console.log(JSON.stringify({ thrust: out, section: sec, control: { Tt4: ctl.Tt4, Tt7: ctl.Tt7, ratios: ctl.ratios, comps_after: Array.from(c2) } }));
+// display fields with and without the display solid rule, as sha256 per array
+const crypto = require('crypto');
+const sha = ta => crypto.createHash('sha256').update(Buffer.from(ta.buffer, ta.byteOffset, ta.byteLength)).digest('hex');
+zones.mat = arr('z_mat');
+const derived = {};
+for (const disp of [0, 1]) {
+  const G = P.cfdDerived(grid, Object.assign({}, zones), Q, s.fl, null, !!disp);
+  for (const k of ['reg', 'mat', ...P.DERIVED_KEYS]) derived[`der${disp}_${k}`] = sha(G[k]) === m.arrays[`der${disp}_${k}`].sha256;
+}
+console.log(JSON.stringify({ thrust: out, section: sec, control: { Tt4: ctl.Tt4, Tt7: ctl.Tt7, ratios: ctl.ratios, comps_after: Array.from(c2) }, derived }));

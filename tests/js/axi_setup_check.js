@@ -6,7 +6,7 @@
  * The directory holds one setup (axi_<type>_<level>) or many (axi_setups). */
 const fs = require('fs'), path = require('path'), crypto = require('crypto');
 const G = require(path.join(__dirname, '..', '..', '3d_aero_lab_amd', 'js', 'cfd_grid.js'));
-const ZONE_KEYS = ['solid', 'phi', 'ae', 'an', 'comp', 'wgt', 'lmix', 'tgt', 'rate', 'yrate', 'mfd', 'stream'];
+const ZONE_KEYS = ['solid', 'phi', 'ae', 'an', 'comp', 'wgt', 'lmix', 'tgt', 'rate', 'yrate', 'mfd', 'stream', 'mat'];
 const sha = ta => crypto.createHash('sha256').update(Buffer.from(ta.buffer, ta.byteOffset, ta.byteLength)).digest('hex');
 
 function check(spec, W, H, ref) {

@@ -7,7 +7,8 @@ pow / expm1 differ in the last bit on some inputs (measured: pow on 9 % of
 random pairs), which enters through the Riemann farfield; the bar is 1e-12
 relative per variable after 500 iterations. The JS post functions (thrust,
 section average, exit-temperature control) run on the reference's states and
-must reproduce its numbers exactly.
+must reproduce its numbers exactly, and so must the display fields
+(cfdDerived).
 """
 import json
 import os
@@ -57,3 +58,5 @@ def test_js_post_bit_exact_on_reference_states(case, it):
     assert out["section"] == g.s["section"]
     assert out["control"]["comps_after"] == g.s["control"]["comps_after"]
     assert out["control"]["Tt4"] == g.s["control"]["Tt4"] and out["control"]["ratios"] == g.s["control"]["ratios"]
+    # display fields (cfdDerived, src/cfd_post.js:10-40), both solid rules: every array byte-identical
+    assert out["derived"] and all(out["derived"].values()), [k for k, v in out["derived"].items() if not v]

@@ -37,6 +37,19 @@ def test_js_masks_byte_identical(case, spec):
     assert meta["nSolid"] == g.s["nSolid"] and meta["nCut"] == g.s["nCut"]
 
 
+def test_display_sampling_bit_exact():
+    """cfdStencil / cfdSample / cfdState / cfdVelAt (src/air_cfd.js:454-490,
+    637-646) on the reference's fighter state after 200 iterations at 3000
+    seeded probe points (full, partial and fallback stencils, points below the
+    symmetry plane and outside the box): every output equals the reference's."""
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "sampling_check.js"), os.path.join(ROOT, "tests", "golden", "sampling")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = json.loads(r.stdout)
+    assert out.pop("n") == 3000
+    assert all(out.values()), out
+
+
 def test_addon_throws_without_device():
     try:
         import torch
