@@ -8,7 +8,8 @@
  *
  *   node tests/golden/make_golden_axi.js [case ...]
  * cases: xcoarse medium longrun (TJ), hbtf_xcoarse hbtf_medium lbtf_xcoarse,
- * longrun_hbtf longrun_lbtf (the README.md:71-75 validation rows), setups
+ * longrun_hbtf longrun_lbtf (the README.md:71-75 validation rows), setups,
+ * interp, tj_coarse
  *
  * The reference's own harness (tools/cfdlib.js:6-20) uses `??`, which this
  * Node (v12) cannot parse; the same setup sequence is written out here:
@@ -229,12 +230,43 @@ function caseSetups() {
   console.log('wrote axi_setups');
 }
 
+/* interpState (grid sequencing, src/app_cfd.js:331-358 with fracIdx :321-326),
+ * taken as text from the reference's app module and evaluated here (the rest
+ * of that module needs a browser page): the xcoarse reference state carried
+ * onto the medium grid for the TJ and the HBTF, as sha256 of the five planes. */
+function refInterp() {
+  const src = fs.readFileSync(path.join(REF, 'src', 'app_cfd.js'), 'utf8');
+  const grab = name => {
+    const i = src.indexOf(`function ${name}(`);
+    let d = 0, k = src.indexOf('{', i);
+    for (; k < src.length; k++) { if (src[k] === '{') d++; else if (src[k] === '}' && --d === 0) break; }
+    return src.slice(i, k + 1);
+  };
+  vm.runInThisContext(grab('fracIdx') + '\n' + grab('interpState') + '\nglobalThis.AXI_INTERP = { fracIdx, interpState };');
+  return globalThis.AXI_INTERP;
+}
+function caseInterp() {
+  const I = refInterp(), out = {};
+  for (const [type, iters] of [['tj', 500], ['hbtf', 100]]) {
+    const A0 = setup(type, 'xcoarse'), B = setup(type, 'medium');
+    A0.S.step(iters);
+    globalThis.CUR = { sol: { fl: B.fl } };      // the all-solid fallback reads the freestream from the app state
+    const Q = I.interpState(A0.grid, A0.zones, A0.S.Q, B.grid, B.zones);
+    out[type] = { from: 'xcoarse', to: 'medium', iters, sha256: Q.map(a => sha(Buffer.from(a.buffer))),
+      sample: Q.map(a => Array.from(a.subarray(0, 64))) };
+  }
+  fs.mkdirSync(path.join(OUT, 'axi_interp'), { recursive: true });
+  fs.writeFileSync(path.join(OUT, 'axi_interp', 'manifest.json'), JSON.stringify({ case: 'axi_interp', interp: out }));
+  console.log('wrote axi_interp');
+}
+
 const want = process.argv.slice(2);
 const all = {
   xcoarse: caseXcoarse, medium: caseMedium,
   longrun: () => caseLongrun('tj', 'medium', 14000, 'axi_tj_medium_longrun'),
   hbtf_xcoarse: () => caseShort('hbtf', 'xcoarse'), lbtf_xcoarse: () => caseShort('lbtf', 'xcoarse'),
-  hbtf_medium: () => caseSetupOnly('hbtf', 'medium'), setups: caseSetups,
+  hbtf_medium: () => caseSetupOnly('hbtf', 'medium'), setups: caseSetups, interp: caseInterp,
+  tj_coarse: () => caseSetupOnly('tj', 'coarse'),
   longrun_hbtf: () => caseLongrun('hbtf', 'medium', 14000, 'axi_hbtf_medium_longrun'),
   longrun_lbtf: () => caseLongrun('lbtf', 'xcoarse', 10000, 'axi_lbtf_xcoarse_longrun'),
 };
