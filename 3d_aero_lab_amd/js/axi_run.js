@@ -4,17 +4,18 @@
  * on AxiHIP: n iterations in chunks of 250 with the exit-temperature control
  * after every chunk, thrust and station averages at the end.
  *
- *   node axi_run.js <case dir> <iters> [report every] [spec]
+ *   node axi_run.js <case dir> <iters> [report every] [spec] [load=<file>] [save=<file>]
  * <case dir> holds the solver inputs (grid faces xf / rf, zones z_*, comps,
  * initial state Q0_*, freestream, control fields, engine description) as
  * written by tests/golden/make_golden_axi.js from the reference's setup; with
  * `spec` the grid, zones and constants are built from the engine description
- * (cfd_grid.js) instead of read.
+ * (cfd_grid.js) instead of read; load= resumes from a checkpoint of the
+ * reference's long-run format, save= writes one at the end.
  * Prints one JSON line: {hist, thrust, stations, ms_per_iter_device_wall}.
  * ========================================================================== */
 const fs = require('fs'), path = require('path'), zlib = require('zlib');
 const P = require('./cfd_post.js');
-const { AxiHIP, axiGrid } = require('./cfd_axi.js');
+const { AxiHIP, axiGrid, saveCheckpoint, loadCheckpoint } = require('./cfd_axi.js');
 
 /* Solver inputs of a case directory. With fromSpec, the grid, zones and
  * component constants are built by cfd_grid.js from the case's engine
@@ -33,9 +34,15 @@ function loadCase(dir, fromSpec) {
   return { s: m.scalars, grid: axiGrid(arr('xf'), arr('rf')), zones, comps: Float64Array.from(arr('comps')), Q0 };
 }
 
-function run(dir, iters, every, fromSpec) {
-  const C = loadCase(dir, fromSpec), s = C.s, comps = C.comps;
-  const S = new AxiHIP(C.grid, C.zones, comps, s.fs, C.Q0, Object.assign({}, s.opt, { hf: s.hf }));
+function run(dir, iters, every, fromSpec, ckpt) {
+  const C = loadCase(dir, fromSpec), s = C.s;
+  let comps = C.comps, Q0 = C.Q0;
+  if (ckpt && ckpt.load) {                       // resume (tools/cfdlib.js:16-17)
+    const L = loadCheckpoint(ckpt.load, C.grid.N, comps.length);
+    if (L.Q) Q0 = L.Q;
+    if (L.comps) comps = L.comps;
+  }
+  const S = new AxiHIP(C.grid, C.zones, comps, s.fs, Q0, Object.assign({}, s.opt, { hf: s.hf }));
   const hist = [];
   let it = 0, Q = null;
   const t0 = process.hrtime.bigint();
@@ -59,12 +66,15 @@ function run(dir, iters, every, fromSpec) {
   const f = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, s.hf, { S: S.src, F: S.mfx });
   const st = P.stations(C.grid, C.zones, Q, S.mfx, s.ctl.order, s.type || 'tj');
   S.dispose();
+  if (ckpt && ckpt.save) saveCheckpoint(ckpt.save, Q, comps);
   return { cells: C.grid.N, iters, hist, thrust: { F: f.F, Fb: f.Fb, Fw: f.Fw, mfuel: f.mfuel }, F1D: s.ctl.F1D, stations: st,
     wall_s: wall, step_s: Number(tStep) / 1e9 };
 }
 
 if (require.main === module) {
-  const [dir, iters, every, setup] = process.argv.slice(2);
-  console.log(JSON.stringify(run(dir, +iters, +(every || 2000), setup === 'spec')));
+  const pos = process.argv.slice(2).filter(a => !a.includes('=')), kv = {};
+  for (const a of process.argv.slice(2).filter(a => a.includes('='))) { const [k, v] = a.split('='); kv[k] = v; }
+  const [dir, iters, every, setup] = pos;
+  console.log(JSON.stringify(run(dir, +iters, +(every || 2000), setup === 'spec', { load: kv.load, save: kv.save })));
 }
 module.exports = { loadCase, run };

@@ -1,0 +1,96 @@
+'use strict';
+/* ============================================================================
+ * axi_seq.js — the reference application's convergence loop for the
+ * axisymmetric solver on AxiHIP (setupCFD / cfdRead, src/app_cfd.js:270-297,
+ * 361-404), headless: read back every `every` iterations, density residual,
+ * exit-temperature control once past 1500 iterations (and 300 since the
+ * start of the current grid), convergence when the residual is below 2e-5 or
+ * the thrust has stayed within 0.3 % for 1800 iterations with the residual
+ * below 6e-5. With grid sequencing, the run starts on the coarse grid and
+ * moves to the target grid (interpState, control gains carried) once the
+ * coarse grid has converged after 2500 iterations, or after 9000.
+ *
+ *   node axi_seq.js <coarse case dir> <target case dir> [every] [cap]   (sequenced)
+ *   node axi_seq.js - <target case dir> [every] [cap]                   (target grid only)
+ * Case directories as written by tests/golden/make_golden_axi.js (setup arrays
+ * and initial state). Prints one JSON line: per-grid iterations, residual and
+ * thrust histories, final thrust, wall time.
+ * ========================================================================== */
+const P = require('./cfd_post.js');
+const { interpState } = require('./cfd_grid.js');
+const { AxiHIP } = require('./cfd_axi.js');
+const { loadCase } = require('./axi_run.js');
+
+/* density residual of the reference's readback: RMS over every second
+ * non-solid cell of the relative density change, per iteration (src/app_cfd.js:368-371) */
+function residual(zones, Q0, prev, dIt) {
+  let s2 = 0, n = 0;
+  for (let k = 0; k < Q0.length; k += 2) {
+    if (zones.solid[k]) continue;
+    const d = (Q0[k] - prev[k]) / prev[k];
+    if (d === d) { s2 += d * d; n++; }
+  }
+  return Math.sqrt(s2 / Math.max(1, n)) / dIt;
+}
+
+/* Runs one grid until the reference's criteria; returns the final state. */
+function runGrid(C, comps, Q0, seq, every, cap, log) {
+  const s = C.s;
+  const S = new AxiHIP(C.grid, C.zones, comps, s.fs, Q0, Object.assign({}, s.opt, { hf: s.hf }));
+  const minIt = seq ? 2500 : 3000;
+  let it = 0, prev = null, res = NaN, Q = null, f = null;
+  const lastF = [];
+  const t0 = process.hrtime.bigint();
+  for (;;) {
+    S.step(every); it += every;
+    Q = S.read(Q);
+    if (prev) res = residual(C.zones, Q[0], prev, every);
+    prev = Float32Array.from(Q[0]);
+    f = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, s.hf, { S: S.src, F: S.mfx });
+    if (log.itTotal + it > 1500 && it > 300) {
+      const f0 = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, 0, { S: S.src, F: S.mfx });
+      P.cfdControl(s.type || 'tj', s.ctl, C.grid, C.zones, Q, S.mfx, comps, f0.pw, s.ctl.etam, every);
+      S.setComps(comps);
+    }
+    log.hist.push({ grid: C.grid.W + 'x' + C.grid.H, it: log.itTotal + it, res, F: f.F });
+    lastF.push([it, f.F]);
+    while (lastF.length > 2 && lastF[1][0] < it - 2000) lastF.shift();
+    const span = lastF.length > 1 ? it - lastF[0][0] : 0;
+    let fmin = Infinity, fmax = -Infinity;
+    for (const [, v] of lastF) { fmin = Math.min(fmin, v); fmax = Math.max(fmax, v); }
+    const steady = span >= 1800 && (fmax - fmin) < 0.003 * Math.max(1, Math.abs(f.F)) && res < 6e-5;
+    if (it > minIt && (res < 2e-5 || steady || (!seq && it > cap))) break;
+    if (seq && it > 9000) break;
+    if (!seq && it >= cap) break;
+  }
+  const wall = Number(process.hrtime.bigint() - t0) / 1e9;
+  S.dispose();
+  log.itTotal += it;
+  log.grids.push({ W: C.grid.W, H: C.grid.H, iters: it, res, F: f.F, wall_s: wall });
+  return Q;
+}
+
+function run(coarseDir, targetDir, every, cap) {
+  const log = { itTotal: 0, hist: [], grids: [] };
+  const T = loadCase(targetDir);
+  const comps = Float64Array.from(T.comps);
+  let Q0 = T.Q0;
+  const t0 = process.hrtime.bigint();
+  if (coarseDir && coarseDir !== '-') {
+    const Cc = loadCase(coarseDir), cc = Float64Array.from(Cc.comps);
+    const Qc = runGrid(Cc, cc, Cc.Q0, true, every, cap, log);
+    const Q64 = [0, 1, 2, 3, 4].map(q => Float64Array.from(Qc[q]));
+    Q0 = interpState(Cc.grid, Cc.zones, Q64, T.grid, T.zones, T.s.fl);
+    comps[17] = cc[17]; comps[29] = cc[29];                 // control gains carried (src/app_cfd.js:281)
+  }
+  runGrid(T, comps, Q0, false, every, cap, log);
+  const wall = Number(process.hrtime.bigint() - t0) / 1e9;
+  const last = log.grids[log.grids.length - 1];
+  return { grids: log.grids, iters: log.itTotal, F: last.F, res: last.res, wall_s: wall, hist: log.hist };
+}
+
+if (require.main === module) {
+  const [coarse, target, every, cap] = process.argv.slice(2);
+  console.log(JSON.stringify(run(coarse, target, +(every || 250), +(cap || 60000))));
+}
+module.exports = { run, residual };

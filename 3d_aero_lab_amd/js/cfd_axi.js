@@ -7,6 +7,7 @@
  *   const s = new AxiHIP(grid, zones, comps, fs, Q0, opt);   // throws when no device
  *   s.step(n); const Q = s.read();   // 5 Float32Array planes; s.src, s.mfx diagnostics
  *   s.setComps(c); s.setZones(z); s.setFs(fs); s.setState(Q); s.finish(); s.dispose();
+ *   saveCheckpoint(file, Q, comps) / loadCheckpoint(file, N, 32): the long runs' files.
  * grid = {W, H, xf, rf, ...} (buildCFDGrid, src/cfd_grid.js:28-43; axiGrid below
  * derives xc, rc, dx, dr from the faces), zones = buildZones' typed arrays.
  * ========================================================================== */
@@ -59,4 +60,30 @@ class AxiHIP {
   dispose() { if (this.h) { loadAddon().axiDispose(this.h); this.h = null; } }
 }
 
-module.exports = { AxiHIP, axiGrid };
+/* Checkpoint of a long run in the reference's format (save and setup's load,
+ * tools/cfdlib.js:16-21): the five state planes as one raw Float64 file and
+ * the component constants as JSON beside it (<file>.comps.json). */
+function saveCheckpoint(file, Q, comps) {
+  const fs = require('fs'), N = Q[0].length, b = new Float64Array(5 * N);
+  for (let q = 0; q < 5; q++) b.set(Q[q], q * N);
+  fs.writeFileSync(file, Buffer.from(b.buffer));
+  fs.writeFileSync(file + '.comps.json', JSON.stringify(Array.from(comps)));
+}
+/* {Q, comps} of a checkpoint; a part that is missing or sized for another
+ * grid / constants layout comes back null (the reference's loader then keeps
+ * the setup's). */
+function loadCheckpoint(file, N, ncomps) {
+  const fs = require('fs');
+  let Q = null, comps = null;
+  if (fs.existsSync(file)) {
+    const buf = fs.readFileSync(file), b = new Float64Array(buf.buffer.slice(buf.byteOffset, buf.byteOffset + buf.byteLength));
+    if (b.length === 5 * N) Q = [0, 1, 2, 3, 4].map(q => b.slice(q * N, (q + 1) * N));
+  }
+  if (fs.existsSync(file + '.comps.json')) {
+    const cj = JSON.parse(fs.readFileSync(file + '.comps.json', 'utf8'));
+    if (cj.length === ncomps) comps = Float64Array.from(cj);
+  }
+  return { Q, comps };
+}
+
+module.exports = { AxiHIP, axiGrid, saveCheckpoint, loadCheckpoint };

@@ -224,6 +224,15 @@ def axi_leg(iters=14000):
     out["roofline_frac"] = out["cell_updates_per_s"] * 182 / (HBM_PEAK_GBS * 1e9)
     out["thrust_kN"] = out["thrust"]["F"] / 1000
     out["published_thrust_kN"] = 56.9
+    # the application's convergence loop with and without grid sequencing (js/axi_seq.js)
+    seq = {}
+    for mode, coarse in (("sequenced", os.path.join(ROOT, "tests", "golden", "axi_tj_coarse")), ("direct", "-")):
+        r = subprocess.run(["node", os.path.join(ROOT, "3d_aero_lab_amd", "js", "axi_seq.js"), coarse, AXI_CASE, "250", "60000"],
+                           capture_output=True, text=True, timeout=600, check=True)
+        d = json.loads(r.stdout.strip().splitlines()[-1])
+        seq[mode] = {"grids": [{k: g[k] for k in ("W", "H", "iters")} for g in d["grids"]], "thrust_kN": d["F"] / 1000,
+                     "residual": d["res"], "wall_s": d["wall_s"]}
+    out["convergence_loop"] = seq
     return out
 
 

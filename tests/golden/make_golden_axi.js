@@ -9,7 +9,7 @@
  *   node tests/golden/make_golden_axi.js [case ...]
  * cases: xcoarse medium longrun (TJ), hbtf_xcoarse hbtf_medium lbtf_xcoarse,
  * longrun_hbtf longrun_lbtf (the README.md:71-75 validation rows), setups,
- * interp, tj_coarse
+ * interp, tj_coarse, checkpoint
  *
  * The reference's own harness (tools/cfdlib.js:6-20) uses `??`, which this
  * Node (v12) cannot parse; the same setup sequence is written out here:
@@ -260,13 +260,34 @@ function caseInterp() {
   console.log('wrote axi_interp');
 }
 
+/* Checkpoint format of the reference's long runs (save, tools/cfdlib.js:21):
+ * the state after 10 TJ xcoarse iterations written by that function (taken
+ * as text, like interpState) to a scratch file; its bytes go in as sha256. */
+function caseCheckpoint() {
+  const src = fs.readFileSync(path.join(REF, 'tools', 'cfdlib.js'), 'utf8');
+  const i = src.indexOf('function save(');
+  const j = src.indexOf('\n', i);
+  globalThis.fs = fs;                      // the function writes with the file module
+  vm.runInThisContext(src.slice(i, j) + '\nglobalThis.AXI_SAVE = save;');
+  const C = setup('tj', 'xcoarse');
+  C.S.step(10);
+  const os = require('os'), f = path.join(os.tmpdir(), 'axi_ckpt_ref.bin');
+  globalThis.AXI_SAVE(C, f);
+  const bin = fs.readFileSync(f), js = fs.readFileSync(f + '.comps.json', 'utf8');
+  fs.mkdirSync(path.join(OUT, 'axi_ckpt'), { recursive: true });
+  fs.writeFileSync(path.join(OUT, 'axi_ckpt', 'manifest.json'), JSON.stringify({ case: 'axi_ckpt', state: 'axi_tj_xcoarse it10',
+    bin_bytes: bin.length, bin_sha256: sha(bin), comps_json: js }));
+  fs.unlinkSync(f); fs.unlinkSync(f + '.comps.json');
+  console.log('wrote axi_ckpt');
+}
+
 const want = process.argv.slice(2);
 const all = {
   xcoarse: caseXcoarse, medium: caseMedium,
   longrun: () => caseLongrun('tj', 'medium', 14000, 'axi_tj_medium_longrun'),
   hbtf_xcoarse: () => caseShort('hbtf', 'xcoarse'), lbtf_xcoarse: () => caseShort('lbtf', 'xcoarse'),
   hbtf_medium: () => caseSetupOnly('hbtf', 'medium'), setups: caseSetups, interp: caseInterp,
-  tj_coarse: () => caseSetupOnly('tj', 'coarse'),
+  tj_coarse: () => caseSetupOnly('tj', 'coarse'), checkpoint: caseCheckpoint,
   longrun_hbtf: () => caseLongrun('hbtf', 'medium', 14000, 'axi_hbtf_medium_longrun'),
   longrun_lbtf: () => caseLongrun('lbtf', 'xcoarse', 10000, 'axi_lbtf_xcoarse_longrun'),
 };

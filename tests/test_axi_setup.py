@@ -38,3 +38,17 @@ def test_interp_state_byte_identical():
     assert r.returncode == 0, r.stderr[-2000:]
     res = json.loads(r.stdout)
     assert set(res) == {"tj", "hbtf"} and all(all(v) for v in res.values()), res
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_checkpoint_format_matches_reference(tmp_path):
+    """Long-run checkpoint files (save / load, tools/cfdlib.js:16-21): the TJ
+    xcoarse state after 10 iterations written by js/cfd_axi.js has the same
+    bytes as the reference's own save() of it, its constants the same JSON
+    text; loading returns the planes exactly and ignores a file sized for
+    another grid, as the reference's loader does."""
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "axi_ckpt_check.js"), os.path.join(ROOT, "tests", "golden"), str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    res = json.loads(r.stdout)
+    assert all(res.values()), res
