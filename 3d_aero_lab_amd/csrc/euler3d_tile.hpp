@@ -16,8 +16,9 @@
 //   x / y apertures, x / y slopes, x / y face fluxes.
 //   Registers per column: primitives of planes k-1..k+2, z slopes of k and
 //   k+1, the z flux of face k-1/2 carried from the previous plane.
-// Per plane: load (own plane k+2 into the queue, halo of plane k into LDS),
-// slopes, fluxes, cell update; three barriers.
+// Per plane: load (own plane k+2 into the queue, halo of plane k into LDS
+// from registers loaded one plane earlier), slopes, fluxes, cell update;
+// three barriers.
 // Divisions use the hardware reciprocal, square root and reciprocal square
 // root (1 ulp).
 // ============================================================================
@@ -201,6 +202,39 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
   // queue: Pz[0] = k-1, Pz[1] = k, Pz[2] = k+1; Pz[3] loaded each plane
   float sUpPrev = (own && k0 > 0) ? gld(ap3, (unsigned)((k0 - 1) * NXY) + qc) : 0.f;
 
+  // halo ring (no corners: 4 rows of TX, 4 columns of TY), NHP cells per
+  // thread; each cell's raw state is loaded one plane ahead (ha / ht) so the
+  // loads of plane k + 1 are in flight while plane k is computed
+  constexpr int NT = TX * TY, NH = 4 * TX + 4 * TY, NHP = (NH + NT - 1) / NT, PW = TileSmem<TX, TY>::PW;
+  int hoff[NHP];             // LDS offset (hy + 2) * PW + hx + 2 in a plane of sm.P, -1: no cell
+  unsigned hq[NHP];          // plane-0 index of the cell (mirrored row below y = 0)
+  bool hin[NHP], hmir[NHP];  // inside the box in x / y; mirror ghost
+  float ha[NHP][5];
+  int ht[NHP];
+#pragma unroll
+  for (int p = 0; p < NHP; p++) {
+    const int h = tid + p * NT;
+    int hx = 0, hy = 0;
+    if (h < 4 * TX) { const int r = h / TX; hx = h % TX; hy = r < 2 ? r - 2 : TY + r - 2; }
+    else { const int c = (h - 4 * TX) / TY; hy = (h - 4 * TX) % TY; hx = c < 2 ? c - 2 : TX + c - 2; }
+    const int gi = i0 + hx, gj = j0 + hy;
+    hoff[p] = h < NH ? (hy + 2) * PW + hx + 2 : -1;
+    hin[p] = h < NH && gi >= 0 && gi < NX && gj < NY;
+    hmir[p] = gj < 0;
+    hq[p] = hin[p] ? (unsigned)((hmir[p] ? -1 - gj : gj) * NX + gi) : 0u;
+  }
+  // raw state of plane kk (cell 0 stands in outside the box)
+  auto halo_load = [&](int kk) {
+#pragma unroll
+    for (int p = 0; p < NHP; p++) {
+      const unsigned q = hin[p] ? (unsigned)(kk * NXY) + hq[p] : 0u;
+      ht[p] = gld(cv.t, q);
+#pragma unroll
+      for (int v = 0; v < 5; v++) ha[p][v] = gld(qin + v * ns, q);
+    }
+  };
+  halo_load(k0);
+
   for (int k = k0; k < k1; k++) {
     // ---------------- load phase
     Tz[3] = load_prim(cv, kc, qin, ns, own ? i : -1, j, k + 2, Pz[3]);
@@ -209,17 +243,21 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
 #pragma unroll
     for (int v = 0; v < 5; v++) sm.P[v][ty + 2][tx + 2] = Pz[1][v];
     sm.T[ty + 2][tx + 2] = (int8_t)Tz[1];
-    // halo ring (no corners): 4 rows of TX, 4 columns of TY
-    for (int h = tid; h < 4 * TX + 4 * TY; h += TX * TY) {
-      int hx, hy;
-      if (h < 4 * TX) { const int r = h / TX; hx = h % TX; hy = r < 2 ? r - 2 : TY + r - 2; }
-      else { const int c = (h - 4 * TX) / TY; hy = (h - 4 * TX) % TY; hx = c < 2 ? c - 2 : TX + c - 2; }
-      float P[5];
-      const int T = load_prim(cv, kc, qin, ns, i0 + hx, j0 + hy, k, P);
+    // halo ring of plane k from the prefetched raw state (primitives as
+    // load_prim), then the prefetch of plane k + 1
 #pragma unroll
-      for (int v = 0; v < 5; v++) sm.P[v][hy + 2][hx + 2] = P[v];
-      sm.T[hy + 2][hx + 2] = (int8_t)T;
+    for (int p = 0; p < NHP; p++) {
+      if (hoff[p] < 0) continue;
+      float w[5];
+      prim_fast(ha[p][0], ha[p][1], ha[p][2], ha[p][3], ha[p][4], w);
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        const float x = !hin[p] ? kc.Fw[v] : w[v];
+        (&sm.P[v][0][0])[hoff[p]] = (v == 2 && hin[p] && hmir[p]) ? -x : x;
+      }
+      (&sm.T[0][0])[hoff[p]] = (int8_t)(!hin[p] ? -1 : (hmir[p] ? -2 : ht[p]));
     }
+    if (k + 1 < k1) halo_load(k + 1);
     // apertures: own x / y faces, plus the two faces left of / below the tile
     const unsigned qk = (unsigned)(k * NXY) + qc;
     sm.A0[ty][tx + 2] = own ? gld(ap0, qk) : 1.f;

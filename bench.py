@@ -81,16 +81,18 @@ def batched_workload(seed=20261203, n=64):
 
 
 def run_converged_plain(built, conds, res_drop, max_iters, check_every, tile):
-    """Each case alone on the plain path, one after another, to res/res0 <= res_drop or max_iters."""
+    """Each case alone on the plain path, one after another, to res/res0 <= res_drop or max_iters.
+    Returns the wall time, the iterations and the final (CL, CD, Cm) of each case."""
     import aero3d
     t0 = time.perf_counter()
-    iters = []
+    iters, coefs = [], []
     for (grid, geo, fl, meta), (M, alpha) in zip(built, conds):
-        F, fix, _ = case_setup(meta, M, alpha)
+        F, fix, fref = case_setup(meta, M, alpha)
         s = aero3d.CfdHIP(grid, geo)
         s.opt["cfl"] = 0.5
         s.set_tile(*tile)
         s.set_free(F); s.set_fix(fix); s.reset()
+        s.set_force_list(fl)
         s.residual()
         res0 = None
         while True:
@@ -101,8 +103,20 @@ def run_converged_plain(built, conds, res_drop, max_iters, check_every, tile):
             if (res0 and r / res0 <= res_drop) or s.it >= max_iters or not math.isfinite(r):
                 break
         iters.append(s.it)
+        c = s.forces(*fref)
+        coefs.append((c["CL"], c["CD"], c["Cm"]))
         s.dispose()
-    return time.perf_counter() - t0, iters
+    return time.perf_counter() - t0, iters, coefs
+
+
+def coef_rel_diff(done, coefs):
+    """Largest relative CL / CD difference between server results (by tag) and plain coefficients."""
+    worst = 0.0
+    for d in done:
+        cl, cd, _ = coefs[d["tag"]]
+        for a, b in ((d["CL"], cl), (d["CD"], cd)):
+            worst = max(worst, abs(a - b) / max(abs(b), 1e-12))
+    return worst
 
 
 def run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes, max_slots=256, group_cells=1 << 40):
@@ -165,13 +179,15 @@ def batched_leg(res_drop=1e-4, max_iters=2000, check_every=50):
     done, wall_b, busy, cu = run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes=int(cells * 110 + (2 << 30)),
                                         group_cells=group)
     print("[bench] batched: plain path, one case at a time", file=sys.stderr, flush=True)
-    wall_p, iters_p = run_converged_plain(built, conds, res_drop, max_iters, check_every, (8, 0))
+    wall_p, iters_p, coefs_p = run_converged_plain(built, conds, res_drop, max_iters, check_every, (8, 0))
     it_b = {d["tag"]: d["iters"] for d in done}
     same = all(it_b[k] == iters_p[k] for k in range(len(conds)))
+    dcoef = coef_rel_diff(done, coefs_p)
     return {"cases": len(conds), "cells": cells, "solves_per_s": len(conds) / wall_b, "wall_s": wall_b,
             "cell_updates_per_s": cu / wall_b, "plain_solves_per_s": len(conds) / wall_p, "plain_wall_s": wall_p,
             "speedup_vs_plain": wall_p / wall_b, "converged": sum(d["converged"] for d in done),
-            "iters_mean": sum(it_b.values()) / len(it_b), "same_iterations_as_plain": same, "zchunk": zc, "group_cells": group,
+            "iters_mean": sum(it_b.values()) / len(it_b), "same_iterations_as_plain": same, "cl_cd_max_rel_diff_vs_plain": dcoef,
+            "zchunk": zc, "group_cells": group,
             "criterion": f"res/res0 <= {res_drop} (density residual every {check_every} iterations) or {max_iters} iterations"}
 
 
@@ -203,9 +219,23 @@ def serve_leg(n_requests=200, res_drop=1e-4, max_iters=2000, check_every=50):
     done, wall, busy, cu = run_server(built, reqs, res_drop, max_iters, check_every, 3, arena_bytes=64 << 30)
     lat = sorted(d["t_done"] - d["arrival"] for d in done)
     pct = lambda p: lat[min(len(lat) - 1, int(round(p / 100 * (len(lat) - 1))))]
+    # the same requests one at a time on the plain path (arrival times ignored: its best case)
+    print("[bench] serve: plain path, one request at a time", file=sys.stderr, flush=True)
+    wall_p, iters_p, coefs_p = run_converged_plain([built[r[0]] for r in reqs], [(r[1], r[2]) for r in reqs], res_drop, max_iters,
+                                                   check_every, (8, 0))
+    it_s = {d["tag"]: d["iters"] for d in done}
+    # capacity: the same requests all arriving at once (the stream above is arrival-bound at ~20 req/s)
+    print("[bench] serve: capacity run, all requests at t = 0", file=sys.stderr, flush=True)
+    done_c, wall_c, _, cu_c = run_server(built, [(r[0], r[1], r[2], 0.0) for r in reqs], res_drop, max_iters, check_every, 3,
+                                         arena_bytes=64 << 30)
     return {"requests": len(done), "sample": f"first {n_requests} of the 10k-request stream (seed 20261204)",
+            "capacity_solves_per_s": len(done_c) / wall_c, "capacity_cell_updates_per_s": cu_c / wall_c,
+            "capacity_speedup_vs_plain": wall_p / wall_c,
             "solves_per_s": len(done) / wall, "wall_s": wall, "p50_s": pct(50), "p95_s": pct(95), "p99_s": pct(99),
             "gpu_busy_frac": busy / wall, "cell_updates_per_s": cu / wall, "converged": sum(d["converged"] for d in done),
+            "plain_solves_per_s": len(reqs) / wall_p, "plain_wall_s": wall_p, "speedup_vs_plain": wall_p / wall,
+            "same_iterations_as_plain": all(it_s[k] == iters_p[k] for k in range(len(reqs))),
+            "cl_cd_max_rel_diff_vs_plain": coef_rel_diff(done, coefs_p),
             "criterion": f"res/res0 <= {res_drop} or {max_iters} iterations, residual every {check_every}"}
 
 
