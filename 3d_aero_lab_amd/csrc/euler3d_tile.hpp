@@ -234,6 +234,36 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
     }
   };
   halo_load(k0);
+  // apertures written to LDS each plane (own x / y faces, the tile-edge face
+  // of lanes 0..2TY-1 and 64..64+2TX-1) and the far-face aperture of this
+  // lane's tile-edge slope, loaded one plane ahead
+  const int et = (tid + TX * TY - 2 * TX) % (TX * TY);
+  float aX = 1.f, aY = 1.f, aE = 1.f, aS = 1.f;
+  auto ap_load = [&](int kk) {
+    const unsigned qq = (unsigned)(kk * NXY) + qc;
+    aX = own ? gld(ap0, qq) : 1.f;
+    aY = own ? gld(ap1, qq) : 1.f;
+    if (tid < 2 * TY) {
+      const int r = tid >> 1, ii = (tid & 1) - 2, gi = i0 + ii, gj = j0 + r;
+      aE = (gi < 0 || gi >= NX || gj >= NY) ? 1.f : gld(ap0, (unsigned)((kk * NY + gj) * NX + gi));
+    } else if (tid >= 64 && tid < 64 + 2 * TX) {
+      const int u = tid - 64, c = u >> 1, jj = (u & 1) - 2, gi = i0 + c, gj = j0 + jj;
+      aE = (gj < 0 || gj >= NY || gi >= NX) ? 1.f : gld(ap1, (unsigned)((kk * NY + gj) * NX + gi));
+    }
+    if (et < 2 * TY) {
+      if (et & 1) {
+        const int gi = i0 + TX, gj = j0 + (et >> 1);
+        aS = (gi < 0 || gi >= NX || gj >= NY) ? 1.f : gld(ap0, (unsigned)((kk * NY + gj) * NX + gi));
+      }
+    } else if (et < 2 * TY + 2 * TX) {
+      const int u = et - 2 * TY;
+      if (u & 1) {
+        const int gi = i0 + (u >> 1), gj = j0 + TY;
+        aS = (gj < 0 || gj >= NY || gi >= NX) ? 1.f : gld(ap1, (unsigned)((kk * NY + gj) * NX + gi));
+      }
+    }
+  };
+  ap_load(k0);
 
   for (int k = k0; k < k1; k++) {
     // ---------------- load phase
@@ -260,15 +290,12 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
     if (k + 1 < k1) halo_load(k + 1);
     // apertures: own x / y faces, plus the two faces left of / below the tile
     const unsigned qk = (unsigned)(k * NXY) + qc;
-    sm.A0[ty][tx + 2] = own ? gld(ap0, qk) : 1.f;
-    sm.A1[ty + 2][tx] = own ? gld(ap1, qk) : 1.f;
-    if (tid < 2 * TY) {
-      const int r = tid >> 1, ii = (tid & 1) - 2, gi = i0 + ii, gj = j0 + r;
-      sm.A0[r][ii + 2] = (gi < 0 || gi >= NX || gj >= NY) ? 1.f : gld(ap0, (unsigned)((k * NY + gj) * NX + gi));
-    } else if (tid >= 64 && tid < 64 + 2 * TX) {
-      const int u = tid - 64, c = u >> 1, jj = (u & 1) - 2, gi = i0 + c, gj = j0 + jj;
-      sm.A1[jj + 2][c] = (gj < 0 || gj >= NY || gi >= NX) ? 1.f : gld(ap1, (unsigned)((k * NY + gj) * NX + gi));
-    }
+    sm.A0[ty][tx + 2] = aX;
+    sm.A1[ty + 2][tx] = aY;
+    if (tid < 2 * TY) sm.A0[tid >> 1][(tid & 1)] = aE;
+    else if (tid >= 64 && tid < 64 + 2 * TX) sm.A1[(tid - 64) & 1][(tid - 64) >> 1] = aE;
+    const float aSk = aS;
+    if (k + 1 < k1) ap_load(k + 1);
     __syncthreads();
 
     // ---------------- slopes: own x and y, then the tile-edge extras
@@ -289,7 +316,6 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       for (int v = 0; v < 5; v++) sm.Sy[v][ty + 1][tx] = s[v];
     }
     // the tile-edge slopes go to waves 2.. (waves 0 and 1 take the tile-edge fluxes)
-    const int et = (tid + TX * TY - 2 * TX) % (TX * TY);
     for (int e = et; e < 2 * TY + 2 * TX; e += TX * TY) {
       float P0[5], Pm[5], Pp[5], s[5];
       if (e < 2 * TY) {                                 // x slope of cell -1 or TX in row r
@@ -299,7 +325,8 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         // faces of cell cx: ap0 of cells cx-1 and cx; cell TX's + face lies outside the tile
         const int gi = i0 + cx, gj = j0 + r;
         const float am = sm.A0[r][cx + 1];
-        const float apf = (cx == -1) ? sm.A0[r][cx + 2] : ((gi < 0 || gi >= NX || gj >= NY) ? 1.f : gld(ap0, (unsigned)((k * NY + gj) * NX + gi)));
+        const float apf = (cx == -1) ? sm.A0[r][cx + 2] : aSk;
+        (void)gi; (void)gj;
         const bool ok = o2 && sm.T[yj][xi] == 0 && sm.T[yj][xi - 1] == 0 && sm.T[yj][xi + 1] == 0 && am == 1.f && apf == 1.f;
         slope5(Pm, P0, Pp, ok, s);
 #pragma unroll
@@ -310,7 +337,8 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         for (int v = 0; v < 5; v++) { P0[v] = sm.P[v][yj][xi]; Pm[v] = sm.P[v][yj - 1][xi]; Pp[v] = sm.P[v][yj + 1][xi]; }
         const int gi = i0 + c, gj = j0 + ry;
         const float am = sm.A1[ry + 1][c];
-        const float apf = (ry == -1) ? sm.A1[ry + 2][c] : ((gj < 0 || gj >= NY || gi >= NX) ? 1.f : gld(ap1, (unsigned)((k * NY + gj) * NX + gi)));
+        const float apf = (ry == -1) ? sm.A1[ry + 2][c] : aSk;
+        (void)gi; (void)gj;
         const bool ok = o2 && sm.T[yj][xi] == 0 && sm.T[yj - 1][xi] == 0 && sm.T[yj + 1][xi] == 0 && am == 1.f && apf == 1.f;
         slope5(Pm, P0, Pp, ok, s);
 #pragma unroll
