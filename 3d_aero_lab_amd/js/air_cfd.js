@@ -107,12 +107,17 @@ function classifyCells(g, solidAt) {
   const NX = g.NX, NY = g.NY, NZ = g.NZ, N = g.N, X = g.X, Y = g.Y, Z = g.Z;
   const sx = NX + 1, sxy = sx * (NY + 1);
   const corner = new Uint8Array(sxy * (NZ + 1));
-  for (let k = 0; k <= NZ; k++) for (let j = 0; j <= NY; j++) for (let i = 0; i <= NX; i++)
-    corner[k * sxy + j * sx + i] = solidAt(X.f[i], Y.f[j], Z.f[k]) ? 1 : 0;
+  // one plane per call: the engine compiles the per-plane loop after the first
+  // few planes instead of interpreting one long loop
+  const cornerPlane = k => {
+    for (let j = 0; j <= NY; j++) for (let i = 0; i <= NX; i++) corner[k * sxy + j * sx + i] = solidAt(X.f[i], Y.f[j], Z.f[k]) ? 1 : 0;
+  };
+  for (let k = 0; k <= NZ; k++) cornerPlane(k);
   const cn = (i, j, k) => corner[k * sxy + j * sx + i];
   const phi = new Float32Array(N);
   const ap = [new Float32Array(N).fill(1), new Float32Array(N).fill(1), new Float32Array(N).fill(1), new Float32Array(N)];
-  for (let k = 0; k < NZ; k++) for (let j = 0; j < NY; j++) for (let i = 0; i < NX; i++) {
+  const cellPlane = k => { for (let j = 0; j < NY; j++) for (let i = 0; i < NX; i++) cell(i, j, k); };
+  const cell = (i, j, k) => {
     const q = (k * NY + j) * NX + i;
     let s = 0;
     for (let c = 0; c < 8; c++) s += cn(i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2));
@@ -141,7 +146,8 @@ function classifyCells(g, solidAt) {
       }
       ap[d][q] = n / 9;
     }
-  }
+  };
+  for (let k = 0; k < NZ; k++) cellPlane(k);
   return { phi, ap };
 }
 
@@ -149,16 +155,19 @@ function classifyCells(g, solidAt) {
  * (src/air_cfd.js:114, 132-139). `fixed` holds [q, axis, side] records. */
 function finishMasks(g, phi, ap, t, fixed) {
   const NX = g.NX, NY = g.NY, NZ = g.NZ, N = g.N, NXY = NX * NY;
-  for (let k = 0; k < NZ; k++) for (let j = 0; j < NY; j++) for (let i = 0; i < NX; i++) {
-    const q = (k * NY + j) * NX + i;
-    for (let d = 0; d < 3; d++) {
-      const inside = d === 0 ? i + 1 < NX : d === 1 ? j + 1 < NY : k + 1 < NZ;
-      if (!inside) { if (d === 2) ap[3][q] = 0; continue; }
-      const q2 = q + (d === 0 ? 1 : d === 1 ? NX : NXY);
-      if (t[q] === 1 || t[q2] === 1) { ap[d][q] = 0; if (d === 2) ap[3][q] = 0; }
+  const plane = k => {                 // one plane per call (compiled after the first few)
+    for (let j = 0; j < NY; j++) for (let i = 0; i < NX; i++) {
+      const q = (k * NY + j) * NX + i;
+      for (let d = 0; d < 3; d++) {
+        const inside = d === 0 ? i + 1 < NX : d === 1 ? j + 1 < NY : k + 1 < NZ;
+        if (!inside) { if (d === 2) ap[3][q] = 0; continue; }
+        const q2 = q + (d === 0 ? 1 : d === 1 ? NX : NXY);
+        if (t[q] === 1 || t[q2] === 1) { ap[d][q] = 0; if (d === 2) ap[3][q] = 0; }
+      }
+      if (t[q] >= 3 || (k + 1 < NZ && t[q + NXY] >= 3)) ap[3][q] = 0;
     }
-    if (t[q] >= 3 || (k + 1 < NZ && t[q + NXY] >= 3)) ap[3][q] = 0;
-  }
+  };
+  for (let k = 0; k < NZ; k++) plane(k);
   for (const [q, d, s] of fixed) {
     if (s > 0) { const q2 = q + (d === 0 ? 1 : d === 1 ? NX : NXY); if (t[q2] === 0) ap[d][q] = 1; }
     else if (t[q - 1] === 0) ap[0][q - 1] = 1;
@@ -237,6 +246,11 @@ function cfdGeom(ac, G, g, deltaDeg) {
   const baffle = (S, sA, rot) => {
     for (let j = 0; j < NY; j++) {
       if (Y.f[j] > S.semi || Y.f[j + 1] < sA) continue;
+      baffleRow(S, sA, rot, j);
+    }
+  };
+  const baffleRow = (S, sA, rot, j) => {
+    {
       for (let i = 0; i < NX; i++) {
         let cov = 0, zs = 0, ss = 0;
         for (const a of SUB) for (const b of SUB) {
@@ -276,10 +290,13 @@ function cfdGeom(ac, G, g, deltaDeg) {
       const ty = ex.key === 'bypass' ? 4 : 5, i = layerAt(e.x0 + ex.x - 1e-3);
       const mark = pad => {
         let n = 0;
-        for (let k = 0; k < NZ; k++) for (let j = 0; j < NY; j++) {
-          const r = Math.hypot(Y.c[j] - e.y, Z.c[k] - e.z);
-          if (r <= ex.R + pad && r >= (ex.Rin || 0) - pad) { const q = (k * NY + j) * NX + i; t[q] = ty; phi[q] = 1; fixed.push([q, 0, 1]); n++; }
-        }
+        const plane = k => {
+          for (let j = 0; j < NY; j++) {
+            const r = Math.hypot(Y.c[j] - e.y, Z.c[k] - e.z);
+            if (r <= ex.R + pad && r >= (ex.Rin || 0) - pad) { const q = (k * NY + j) * NX + i; t[q] = ty; phi[q] = 1; fixed.push([q, 0, 1]); n++; }
+          }
+        };
+        for (let k = 0; k < NZ; k++) plane(k);
         return n;
       };
       if (!mark(0)) mark(0.55 * Math.max(Y.d[0], Z.d[0], g.h));
@@ -288,10 +305,13 @@ function cfdGeom(ac, G, g, deltaDeg) {
       const i = layerAt(e.x0 + sp.lip.x + 1e-3);
       const mark = pad => {
         let n = 0;
-        for (let k = 0; k < NZ; k++) for (let j = 0; j < NY; j++) {
-          const r = Math.hypot(Y.c[j] - e.y, Z.c[k] - e.z);
-          if (r <= 0.9 * sp.lip.r + pad) { const q = (k * NY + j) * NX + i; t[q] = 3; phi[q] = 1; fixed.push([q, 0, -1]); n++; }
-        }
+        const plane = k => {
+          for (let j = 0; j < NY; j++) {
+            const r = Math.hypot(Y.c[j] - e.y, Z.c[k] - e.z);
+            if (r <= 0.9 * sp.lip.r + pad) { const q = (k * NY + j) * NX + i; t[q] = 3; phi[q] = 1; fixed.push([q, 0, -1]); n++; }
+          }
+        };
+        for (let k = 0; k < NZ; k++) plane(k);
         return n;
       };
       if (!mark(0)) mark(0.55 * g.h);
