@@ -59,11 +59,39 @@ function insidePoly(x, r, poly) {
   return inside;
 }
 
+/* Polygon edges [xi, ri, xj, rj] (in insidePoly's order, so the crossing
+ * abscissa is computed from the same operands) in NB bins of r over the
+ * bounding box; an edge sits in every bin its r-range touches. An edge whose
+ * ends straddle r has min(ri, rj) <= r < max(ri, rj), so it is in r's bin and
+ * the even-odd count over a bin equals the count over the whole polygon. */
+const NB = 64;
+function binEdges(poly, bb) {
+  const r0 = bb[2], sc = NB / Math.max(1e-30, bb[3] - bb[2]);
+  const bin = v => Math.min(NB - 1, Math.max(0, Math.floor((v - r0) * sc)));
+  const bins = [];
+  for (let b = 0; b < NB; b++) bins.push([]);
+  for (let i = 0, j = poly.length - 1; i < poly.length; j = i++) {
+    const xi = poly[i][0], ri = poly[i][1], xj = poly[j][0], rj = poly[j][1];
+    for (let b = bin(Math.min(ri, rj)); b <= bin(Math.max(ri, rj)); b++) bins[b].push(xi, ri, xj, rj);
+  }
+  return { r0, sc, e: bins.map(a => Float64Array.from(a)) };
+}
+/* insidePoly through the binned edges: the same tests on the edges that can toggle */
+function insideBinned(x, r, B) {
+  const e = B.e[Math.min(NB - 1, Math.max(0, Math.floor((r - B.r0) * B.sc)))];
+  let inside = false;
+  for (let t = 0; t < e.length; t += 4) {
+    const xi = e[t], ri = e[t + 1], xj = e[t + 2], rj = e[t + 3];
+    if ((ri > r) !== (rj > r) && x < (xj - xi) * (r - ri) / (rj - ri) + xi) inside = !inside;
+  }
+  return inside;
+}
+
 /* The engine description with its walls as functions and body bounding boxes. */
 function engine(spec) {
   const streams = {};
   for (const k of spec.order) { const st = spec.streams[k]; streams[k] = { x0: st.x0, x1: st.x1, lo: wall(st.lo), hi: wall(st.hi) }; }
-  const bodies = spec.bodies.map(b => ({ key: b.key, mat: b.mat, poly: b.poly, bb: bbox(b.poly) }));
+  const bodies = spec.bodies.map(b => { const bb = bbox(b.poly); return { key: b.key, mat: b.mat, poly: b.poly, bb, bins: binEdges(b.poly, bb) }; });
   const comb = Object.assign({}, spec.comb, { linerIn: wall(spec.comb.linerIn), linerOut: wall(spec.comb.linerOut) });
   return Object.assign({}, spec, { streams, bodies, comb });
 }
@@ -83,6 +111,28 @@ function classify(E, x, r) {
     if (insidePoly(x, r, b.poly)) return { kind: 'solid', mat: b.mat };
   }
   return { kind: 'ext' };
+}
+/* classify on one grid column x, as a code: stream index >= 0, -1 exterior,
+ * -2 - b for body b. Wall radii and the bodies whose x-range holds x are taken
+ * once per column; the tests and their order are classify's. */
+function columnClassifier(E, x) {
+  const lo = [], hi = [], id = [];
+  E.order.forEach((key, s) => {
+    const st = E.streams[key];
+    if (x < st.x0 || x > st.x1) return;
+    lo.push(st.lo.at(x)); hi.push(st.hi.at(x)); id.push(s);
+  });
+  const bs = [], bi = [];
+  E.bodies.forEach((b, n) => { if (!(x < b.bb[0] || x > b.bb[1])) { bs.push(b); bi.push(n); } });
+  return r => {
+    for (let s = 0; s < id.length; s++) if (r >= lo[s] && r <= hi[s]) return id[s];
+    for (let n = 0; n < bs.length; n++) {
+      const bb = bs[n].bb;
+      if (r < bb[2] || r > bb[3]) continue;
+      if (insideBinned(x, r, bs[n].bins)) return -2 - bi[n];
+    }
+    return -1;
+  };
 }
 
 /* Faces of one axis: refinement bands [a, b, h], growth g outside them, cap
@@ -221,11 +271,16 @@ function buildZones(grid, spec, op) {
   const phi = new Float32Array(N), ae = new Float32Array(N), an = new Float32Array(N);
   const bodies = E.bodies.filter(b => b.mat !== 'liner' && b.mat !== 'fh');
   const NS = 8, split = new Uint8Array(N), splitFr = new Float32Array(N);
-  for (let i = 0; i < W; i++) {
+  const matOf = E.bodies.map(b => MAT_ID[b.mat] || 1);
+  // Each pass is a function of its own, and the cut-cell pass runs one grid
+  // column per call: the engine then optimises every pass from its own type
+  // feedback instead of deoptimising one long function at each new section.
+  const column = i => {
     const ivF = solidIntervals(E, bodies, xf[i + 1]);
     const ivs = [];
     for (let q = 0; q < NS; q++) ivs.push(solidIntervals(E, bodies, xf[i] + dx[i] * (q + 0.5) / NS));
     const ivC = solidIntervals(E, bodies, xc[i]);
+    const cls = columnClassifier(E, xc[i]);
     for (let j = 0; j < H; j++) {
       const k = j * W + i, r0 = rf[j], r1 = rf[j + 1], vr = 0.5 * (r1 * r1 - r0 * r0);
       let f = 0;
@@ -251,28 +306,35 @@ function buildZones(grid, spec, op) {
         ae[k] = Math.min(ae[k], splitFr[k]);
         rs = keepLow ? 0.5 * (r0 + a) : 0.5 * (c + r1);
       }
-      const cl = classify(E, xc[i], rs);
-      stream[k] = cl.kind === 'stream' ? E.order.indexOf(cl.key) : -1;
-      const cc = classify(E, xc[i], rc[j]);
-      mat[k] = cc.kind === 'solid' ? (MAT_ID[cc.mat] || 1) : 1;
+      const cc = cls(rc[j]), cl = rs === rc[j] ? cc : cls(rs);
+      stream[k] = cl >= 0 ? cl : -1;
+      mat[k] = cc <= -2 ? matOf[-2 - cc] : 1;
     }
-  }
-  for (let k = 0; k < N; k++) if (split[k]) {
-    const i = k % W, j = (k / W) | 0;
-    if (split[k] === 1) an[k] = 0; else if (j > 0) an[k - W] = 0;
-    if (i > 0) ae[k - 1] = Math.min(ae[k - 1], splitFr[k]);
-  }
-  for (let k = 0; k < N; k++) if (phi[k] < 0.08) { phi[k] = 0; solid[k] = 1; stream[k] = -2; }
-  for (let j = 0; j < H; j++) for (let i = 0; i < W; i++) {
-    const k = j * W + i;
-    if (solid[k] || i === W - 1 || solid[k + 1]) ae[k] = i === W - 1 && !solid[k] ? 1 : 0;
-    if (solid[k] || j === H - 1 || solid[k + W]) an[k] = j === H - 1 && !solid[k] ? 1 : 0;
-    ae[k] = Math.min(ae[k], 1); an[k] = Math.min(an[k], 1);
-  }
+  };
+  for (let i = 0; i < W; i++) column(i);
+  const splitFaces = () => {
+    for (let k = 0; k < N; k++) if (split[k]) {
+      const i = k % W, j = (k / W) | 0;
+      if (split[k] === 1) an[k] = 0; else if (j > 0) an[k - W] = 0;
+      if (i > 0) ae[k - 1] = Math.min(ae[k - 1], splitFr[k]);
+    }
+  };
+  splitFaces();
+  const merge = () => { for (let k = 0; k < N; k++) if (phi[k] < 0.08) { phi[k] = 0; solid[k] = 1; stream[k] = -2; } };
+  merge();
+  const wallFaces = () => {
+    for (let j = 0; j < H; j++) for (let i = 0; i < W; i++) {
+      const k = j * W + i;
+      if (solid[k] || i === W - 1 || solid[k + 1]) ae[k] = i === W - 1 && !solid[k] ? 1 : 0;
+      if (solid[k] || j === H - 1 || solid[k + W]) an[k] = j === H - 1 && !solid[k] ? 1 : 0;
+      ae[k] = Math.min(ae[k], 1); an[k] = Math.min(an[k], 1);
+    }
+  };
+  wallFaces();
   // fluid cells in no stream: those connected to the outer boundary are
   // exterior; the rest take the stream of the nearest stream cell through open
   // faces (two multi-source breadth-first sweeps, directions E W N S)
-  {
+  const fill = () => {
     const open = (k, d) => d === 0 ? ae[k] > 0 : d === 1 ? ae[k - 1] > 0 : d === 2 ? an[k] > 0 : an[k - W] > 0;
     const nb = (k, d) => {
       const i = k % W, j = (k / W) | 0;
@@ -282,79 +344,77 @@ function buildZones(grid, spec, op) {
       return j > 0 ? k - W : -1;
     };
     const ext = new Uint8Array(N);
+    const sweep = (front, mark) => {
+      while (front.length) {
+        const next = [];
+        for (const k of front) for (let d = 0; d < 4; d++) {
+          const m = nb(k, d);
+          if (m < 0 || solid[m] || ext[m] || stream[m] >= 0 || !open(k, d)) continue;
+          mark(m, k); next.push(m);
+        }
+        front = next;
+      }
+    };
     let front = [];
     for (let k = 0; k < N; k++) {
       const i = k % W, j = (k / W) | 0;
       if (!solid[k] && stream[k] < 0 && (i === 0 || i === W - 1 || j === H - 1)) { ext[k] = 1; front.push(k); }
     }
-    while (front.length) {
-      const next = [];
-      for (const k of front) for (let d = 0; d < 4; d++) {
-        const m = nb(k, d);
-        if (m < 0 || solid[m] || ext[m] || stream[m] >= 0 || !open(k, d)) continue;
-        ext[m] = 1; next.push(m);
-      }
-      front = next;
-    }
+    sweep(front, m => { ext[m] = 1; });
     front = [];
     for (let k = 0; k < N; k++) if (!solid[k] && stream[k] >= 0) front.push(k);
-    while (front.length) {
-      const next = [];
-      for (const k of front) for (let d = 0; d < 4; d++) {
-        const m = nb(k, d);
-        if (m < 0 || solid[m] || ext[m] || stream[m] >= 0 || !open(k, d)) continue;
-        stream[m] = stream[k]; next.push(m);
-      }
-      front = next;
-    }
-  }
+    sweep(front, (m, k) => { stream[m] = stream[k]; });
+  };
+  fill();
   // blade rows: rotors only do work; weight = axial overlap / (row length x rotors of the component)
   const rotors = E.rows.filter(r => r.rotor);
   const nRot = {};
   for (const r of rotors) nRot[r.comp] = (nRot[r.comp] || 0) + 1;
-  for (const r of rotors) {
-    const id = COMP_ID[r.comp];
-    if (!id) continue;
-    const si = E.order.indexOf(r.stream);
+  const row = (r, id, si) => {
     for (let i = 0; i < W; i++) {
       const ov = Math.min(r.x1, xf[i + 1]) - Math.max(r.x0, xf[i]);
       if (ov <= 0) continue;
       const w = (ov / dx[i]) / ((r.x1 - r.x0) * nRot[r.comp]);
       for (let j = 0; j < H; j++) { const k = j * W + i; if (!solid[k] && stream[k] === si) { comp[k] = id; wgt[k] += w; } }
     }
+  };
+  for (const r of rotors) {
+    const id = COMP_ID[r.comp];
+    if (!id) continue;
+    row(r, id, E.order.indexOf(r.stream));
   }
   // combustor and afterburner cells
-  const cb = E.comb, ci = E.order.indexOf(cb.stream);
-  for (let i = 0; i < W; i++) {
-    if (xc[i] < cb.xd || xc[i] > cb.x1) continue;
-    for (let j = 0; j < H; j++) { const k = j * W + i; if (!solid[k] && stream[k] === ci) comp[k] = 4; }
-  }
-  if (E.ab) {
-    const ab = E.ab, ai = E.order.indexOf(ab.stream);
+  const span = (xa, xb, si, id) => {
     for (let i = 0; i < W; i++) {
-      if (xc[i] < ab.xFH || xc[i] > ab.x1) continue;
-      for (let j = 0; j < H; j++) { const k = j * W + i; if (!solid[k] && stream[k] === ai) comp[k] = 7; }
+      if (xc[i] < xa || xc[i] > xb) continue;
+      for (let j = 0; j < H; j++) { const k = j * W + i; if (!solid[k] && stream[k] === si) comp[k] = id; }
     }
-  }
+  };
+  const cb = E.comb;
+  span(cb.xd, cb.x1, E.order.indexOf(cb.stream), 4);
+  if (E.ab) span(E.ab.xFH, E.ab.x1, E.order.indexOf(E.ab.stream), 7);
   const zones = { solid, mat, stream, comp, wgt, lmix, tgt: new Float32Array(N), rate: new Float32Array(N), yrate: new Float32Array(N),
     mfd: new Float32Array(N), phi, ae, an };
   updateHeatZones(grid, zones, E, op);
   // Prandtl mixing length: 0.025 / 0.07 of the duct height in streams, a jet
   // shear layer growing from the first exit outside
   const xe0 = Math.min(...E.exitsGeom.map(e => e.x)), Rj = Math.max(...E.exitsGeom.map(e => e.R));
-  for (let j = 0; j < H; j++) for (let i = 0; i < W; i++) {
-    const k = j * W + i;
-    if (solid[k]) continue;
-    const x = xc[i];
-    if (stream[k] >= 0) {
-      const st = E.streams[E.order[stream[k]]];
-      const h = Math.max(1e-4, st.hi.at(x) - st.lo.at(x));
-      lmix[k] = (comp[k] === 4 || comp[k] === 7 ? 0.07 : 0.025) * h;
-    } else {
-      const dxe = x - xe0;
-      lmix[k] = dxe > 0 ? 0.075 * (0.05 * Rj + 0.12 * dxe) : 0.004 * s;
+  const mixRow = j => {
+    for (let i = 0; i < W; i++) {
+      const k = j * W + i;
+      if (solid[k]) continue;
+      const x = xc[i];
+      if (stream[k] >= 0) {
+        const st = E.streams[E.order[stream[k]]];
+        const h = Math.max(1e-4, st.hi.at(x) - st.lo.at(x));
+        lmix[k] = (comp[k] === 4 || comp[k] === 7 ? 0.07 : 0.025) * h;
+      } else {
+        const dxe = x - xe0;
+        lmix[k] = dxe > 0 ? 0.075 * (0.05 * Rj + 0.12 * dxe) : 0.004 * s;
+      }
     }
-  }
+  };
+  for (let j = 0; j < H; j++) mixRow(j);
   return zones;
 }
 
