@@ -103,24 +103,30 @@ const cidx = (g, i, j, k) => (k * g.NY + j) * g.NX + i;              // src/air_
  *   -> + face apertures ap[0..2] (9 sub-points on mixed faces)
  * (src/air_cfd.js:68-90). phi and ap are exact multiples of 1/27 and 1/9.
  * ------------------------------------------------------------------------- */
-function classifyCells(g, solidAt) {
+function classifyCells(g, solidAt, part) {
   const NX = g.NX, NY = g.NY, NZ = g.NZ, N = g.N, X = g.X, Y = g.Y, Z = g.Z;
   const sx = NX + 1, sxy = sx * (NY + 1);
-  const corner = new Uint8Array(sxy * (NZ + 1));
+  // planes k0..k1-1 (all by default) into part.phi / part.ap when given; the
+  // corner planes k0..k1 are computed locally, so slabs are independent
+  const k0 = part ? part.k0 : 0, k1 = part ? part.k1 : NZ;
+  const corner = new Uint8Array(sxy * (k1 - k0 + 1));
   // one plane per call: the engine compiles the per-plane loop after the first
   // few planes instead of interpreting one long loop
   const cornerPlane = k => {
-    for (let j = 0; j <= NY; j++) for (let i = 0; i <= NX; i++) corner[k * sxy + j * sx + i] = solidAt(X.f[i], Y.f[j], Z.f[k]) ? 1 : 0;
+    const o = (k - k0) * sxy;
+    for (let j = 0; j <= NY; j++) for (let i = 0; i <= NX; i++) corner[o + j * sx + i] = solidAt(X.f[i], Y.f[j], Z.f[k]) ? 1 : 0;
   };
-  for (let k = 0; k <= NZ; k++) cornerPlane(k);
-  const cn = (i, j, k) => corner[k * sxy + j * sx + i];
-  const phi = new Float32Array(N);
-  const ap = [new Float32Array(N).fill(1), new Float32Array(N).fill(1), new Float32Array(N).fill(1), new Float32Array(N)];
+  for (let k = k0; k <= k1; k++) cornerPlane(k);
+  const phi = part ? part.phi : new Float32Array(N);
+  const ap = part ? part.ap : [new Float32Array(N).fill(1), new Float32Array(N).fill(1), new Float32Array(N).fill(1), new Float32Array(N)];
   const cellPlane = k => { for (let j = 0; j < NY; j++) for (let i = 0; i < NX; i++) cell(i, j, k); };
   const cell = (i, j, k) => {
     const q = (k * NY + j) * NX + i;
-    let s = 0;
-    for (let c = 0; c < 8; c++) s += cn(i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2));
+    // corners (i + a, j + b, k + c) at b0 + a + b sx + c sxy
+    const b0 = (k - k0) * sxy + j * sx + i, b1 = b0 + sxy;
+    const c0 = corner[b0], c1 = corner[b0 + 1], c2 = corner[b0 + sx], c3 = corner[b0 + sx + 1];
+    const c4 = corner[b1], c5 = corner[b1 + 1], c6 = corner[b1 + sx], c7 = corner[b1 + sx + 1];
+    const s = c0 + c1 + c2 + c3 + c4 + c5 + c6 + c7;
     if (s === 0) phi[q] = 1;
     else if (s === 8) phi[q] = 0;
     else {
@@ -132,9 +138,9 @@ function classifyCells(g, solidAt) {
     for (let d = 0; d < 3; d++) {
       // the four corners of the + face normal to axis d
       let m;
-      if (d === 0) m = cn(i + 1, j, k) + cn(i + 1, j + 1, k) + cn(i + 1, j, k + 1) + cn(i + 1, j + 1, k + 1);
-      else if (d === 1) m = cn(i, j + 1, k) + cn(i + 1, j + 1, k) + cn(i, j + 1, k + 1) + cn(i + 1, j + 1, k + 1);
-      else m = cn(i, j, k + 1) + cn(i + 1, j, k + 1) + cn(i, j + 1, k + 1) + cn(i + 1, j + 1, k + 1);
+      if (d === 0) m = c1 + c3 + c5 + c7;
+      else if (d === 1) m = c2 + c3 + c6 + c7;
+      else m = c4 + c5 + c6 + c7;
       if (m === 0) continue;
       if (m === 4) { ap[d][q] = 0; continue; }
       let n = 0;
@@ -147,7 +153,7 @@ function classifyCells(g, solidAt) {
       ap[d][q] = n / 9;
     }
   };
-  for (let k = 0; k < NZ; k++) cellPlane(k);
+  for (let k = k0; k < k1; k++) cellPlane(k);
   return { phi, ap };
 }
 
@@ -155,16 +161,22 @@ function classifyCells(g, solidAt) {
  * (src/air_cfd.js:114, 132-139). `fixed` holds [q, axis, side] records. */
 function finishMasks(g, phi, ap, t, fixed) {
   const NX = g.NX, NY = g.NY, NZ = g.NZ, N = g.N, NXY = NX * NY;
-  const plane = k => {                 // one plane per call (compiled after the first few)
-    for (let j = 0; j < NY; j++) for (let i = 0; i < NX; i++) {
-      const q = (k * NY + j) * NX + i;
-      for (let d = 0; d < 3; d++) {
-        const inside = d === 0 ? i + 1 < NX : d === 1 ? j + 1 < NY : k + 1 < NZ;
-        if (!inside) { if (d === 2) ap[3][q] = 0; continue; }
-        const q2 = q + (d === 0 ? 1 : d === 1 ? NX : NXY);
-        if (t[q] === 1 || t[q2] === 1) { ap[d][q] = 0; if (d === 2) ap[3][q] = 0; }
+  const a0 = ap[0], a1 = ap[1], a2 = ap[2], a3 = ap[3];
+  // one plane per call (compiled after the first few); the + face of axis d
+  // closes when either cell is solid, the z one also drops its baffle slope,
+  // as does a z face on the top boundary or next to a fixed-state cell
+  const plane = k => {
+    const kin = k + 1 < NZ;
+    for (let j = 0; j < NY; j++) {
+      const jin = j + 1 < NY;
+      for (let i = 0; i < NX; i++) {
+        const q = (k * NY + j) * NX + i, s1 = t[q] === 1;
+        if (i + 1 < NX && (s1 || t[q + 1] === 1)) a0[q] = 0;
+        if (jin && (s1 || t[q + NX] === 1)) a1[q] = 0;
+        if (!kin) a3[q] = 0;
+        else if (s1 || t[q + NXY] === 1) { a2[q] = 0; a3[q] = 0; }
+        if (t[q] >= 3 || (kin && t[q + NXY] >= 3)) a3[q] = 0;
       }
-      if (t[q] >= 3 || (k + 1 < NZ && t[q + NXY] >= 3)) ap[3][q] = 0;
     }
   };
   for (let k = 0; k < NZ; k++) plane(k);
@@ -173,10 +185,13 @@ function finishMasks(g, phi, ap, t, fixed) {
     else if (t[q - 1] === 0) ap[0][q - 1] = 1;
   }
   let nSolid = 0, nCut = 0;
-  for (let q = 0; q < N; q++) {
-    if (t[q] === 1) nSolid++;
-    else if (t[q] === 0 && (phi[q] < 1 || ap[0][q] < 1 || ap[1][q] < 1 || ap[2][q] < 1)) nCut++;
-  }
+  const count = (qa, qb) => {
+    for (let q = qa; q < qb; q++) {
+      if (t[q] === 1) nSolid++;
+      else if (t[q] === 0 && (phi[q] < 1 || a0[q] < 1 || a1[q] < 1 || a2[q] < 1)) nCut++;
+    }
+  };
+  for (let k = 0; k < NZ; k++) count(k * NXY, (k + 1) * NXY);
   return { nSolid, nCut };
 }
 
@@ -189,12 +204,10 @@ function cfdGeomBody(g, body) {
   return { t, phi, ap, nSolid: c.nSolid, nCut: c.nCut, volW: false, volH: false };
 }
 
-/* Cut-cell masks of an aircraft (src/air_cfd.js:50-141): nacelle envelopes,
- * fuselage, thick wing / tail / fin, thin-surface baffles on the mean camber
- * surface with their slope dz/dx in ap[3], fixed-state inlet (t=3) and nozzle
- * (t=4 bypass, t=5 core) cells, wall-face cleanup. */
-function cfdGeom(ac, G, g, deltaDeg) {
-  const NX = g.NX, NY = g.NY, NZ = g.NZ, X = g.X, Y = g.Y, Z = g.Z, sp = ac.spec;
+/* Solid test of an aircraft at a point (src/air_cfd.js:50-66): nacelle
+ * envelopes (max meridian radius at 301 stations), fuselage, and the wing,
+ * deflected tail and fin where they are at least 1.5 cells thick. */
+function aircraftSolid(ac, G, g, deltaDeg) {
   // nacelle envelope: max meridian radius sampled at 301 stations
   const envs = [];
   for (const e of ac.engines) {
@@ -233,7 +246,18 @@ function cfdGeom(ac, G, g, deltaDeg) {
     }
     return volV && G.inVt(x, ay, z);
   };
-  const { phi, ap } = classifyCells(g, solidAt);
+  return { solidAt, volW, volH, ht, dR, hinge };
+}
+
+/* Cut-cell masks of an aircraft (src/air_cfd.js:50-141): nacelle envelopes,
+ * fuselage, thick wing / tail / fin, thin-surface baffles on the mean camber
+ * surface with their slope dz/dx in ap[3], fixed-state inlet (t=3) and nozzle
+ * (t=4 bypass, t=5 core) cells, wall-face cleanup. `pre` ({phi, ap}) takes the
+ * classification of every plane from elsewhere (the parallel build). */
+function cfdGeom(ac, G, g, deltaDeg, pre) {
+  const NX = g.NX, NY = g.NY, NZ = g.NZ, X = g.X, Y = g.Y, Z = g.Z, sp = ac.spec;
+  const { solidAt, volW, volH, ht, dR, hinge } = aircraftSolid(ac, G, g, deltaDeg);
+  const { phi, ap } = pre || classifyCells(g, solidAt);
 
   // thin-surface baffle: close the z face nearest the mean camber surface by
   // the planform coverage, store the camber slope (incl. incidence, deflection)
@@ -495,6 +519,6 @@ function freestream(M, alphaDeg) {
 
 module.exports = {
   CG, CFD_GRIDS, SUB, cfdAxis, axisFromFaces, gridFromFaces, cfdGrid, cidx, classifyCells, finishMasks,
-  cfdGeomBody, cfdGeom, cfdForceList, forceListArrays, cfdForces, cfdLocate, cfdTrefftz, trefftzX, cfdResidual, freestream,
+  cfdGeomBody, aircraftSolid, cfdGeom, cfdForceList, forceListArrays, cfdForces, cfdLocate, cfdTrefftz, trefftzX, cfdResidual, freestream,
   cfdStencil, cfdSample, cfdState, cfdVelAt,
 };

@@ -2,7 +2,8 @@
 
 CPU: grid faces, cut-cell masks (t, phi, ap0..ap3) and force lists built by
 the JS host are byte-identical to the reference's cfdGrid / cfdGeom /
-cfdForceList outputs (tests/golden), and the Node-API addon loads and throws
+cfdForceList outputs (tests/golden), the worker-thread build (GeomPool) gives
+the same masks whatever the slab split, and the Node-API addon loads and throws
 a JS Error when no device is usable (the reference's constructor contract,
 src/air_cfd.js:371-377, 506-512).
 GPU: the reference driver flow (tools/test_cfd3.js) through CfdHIP reproduces
@@ -35,6 +36,25 @@ def test_js_masks_byte_identical(case, spec):
     for k, a in arrs.items():
         assert sha256(a) == g.sha(k), f"{case}: {k} differs from the reference"
     assert meta["nSolid"] == g.s["nSolid"] and meta["nCut"] == g.s["nCut"]
+
+
+def test_geom_pool_masks_byte_identical():
+    """The worker-thread build (3 workers + the caller, slabs of 1, 5 and the default
+    plane count) gives the reference's masks for every aircraft case."""
+    cases = [("fighter30k", "fighter:30000:3"), ("interceptor", "interceptor:45000:-4"),
+             ("transport60k", "airliner:60000:0:transport"), ("transport1m", "airliner:1000000:0:transport")]
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "geom_pool_check.js"), "3", *[s for _, s in cases]],
+                       cwd=ROOT, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    assert len(res) == 3 * len(cases)
+    for (case, spec), k in zip(cases, range(0, len(res), 3)):
+        g = Golden(case)
+        for b in res[k:k + 3]:
+            assert b["spec"] == spec
+            for name in ("t", "phi", "ap0", "ap1", "ap2", "ap3"):
+                assert b[name] == g.sha(name), f"{case} slab {b['slab']}: {name} differs from the reference"
+            assert b["nSolid"] == g.s["nSolid"] and b["nCut"] == g.s["nCut"]
 
 
 def test_display_sampling_bit_exact():
