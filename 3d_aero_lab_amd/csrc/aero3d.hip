@@ -250,8 +250,10 @@ int aero3d_create(const aero3d_grid *g, const uint8_t *t, const float *phi, cons
   *out = nullptr;
   if (!g || !g->fx || !g->fy || !g->fz || !t || !phi || !ap0 || !ap1 || !ap2 || !ap3)
     return fail(nullptr, AERO3D_E_ARG, "aero3d_create: null argument");
-  if (g->NX < 1 || g->NY < 1 || g->NZ < 1 || (int64_t)g->NX * g->NY * g->NZ > (int64_t)1 << 30)
-    return fail(nullptr, AERO3D_E_ARG, "aero3d_create: grid size out of range");
+  // the y = 0 mirror ghosts of rows -1 and -2 are rows 0 and 1, so NY >= 2;
+  // element offsets are formed in 32 bits, so N <= 2^30
+  if (g->NX < 1 || g->NY < 2 || g->NZ < 1 || (int64_t)g->NX * g->NY * g->NZ > (int64_t)1 << 30)
+    return fail(nullptr, AERO3D_E_ARG, "aero3d_create: grid size out of range (NX, NZ >= 1, NY >= 2, NX*NY*NZ <= 2^30)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, AERO3D_E_NODEV, "aero3d_create: no HIP device");
   aero3d_solver *s = new aero3d_solver();

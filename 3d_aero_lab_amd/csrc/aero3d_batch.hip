@@ -366,7 +366,9 @@ int aero3d_batch_add_geometry(aero3d_batch *b, const aero3d_grid *g, const uint8
                               const double *fcz, const double *fx, const double *fz, int nf, int *gid) {
   if (!b || !g || !t || !phi || !ap0 || !ap1 || !ap2 || !ap3 || !gid || nf < 0 || (nf > 0 && (!fq || !fcx || !fcz || !fx || !fz)))
     return bfail(b, AERO3D_E_ARG, "add_geometry: null argument");
-  if (g->NX < 1 || g->NY < 1 || g->NZ < 1) return bfail(b, AERO3D_E_ARG, "add_geometry: bad grid");
+  // as aero3d_create: two rows for the y = 0 mirror, 32-bit element offsets
+  if (g->NX < 1 || g->NY < 2 || g->NZ < 1 || (int64_t)g->NX * g->NY * g->NZ > (int64_t)1 << 30)
+    return bfail(b, AERO3D_E_ARG, "add_geometry: grid size out of range (NX, NZ >= 1, NY >= 2, NX*NY*NZ <= 2^30)");
   Geometry G;
   G.NX = g->NX; G.NY = g->NY; G.NZ = g->NZ; G.N = g->NX * g->NY * g->NZ;
   for (int k = 0; k < nf; k++) if (fq[k] < 0 || fq[k] >= G.N) return bfail(b, AERO3D_E_ARG, "add_geometry: force-list index out of range");

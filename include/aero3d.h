@@ -32,7 +32,9 @@ typedef struct aero3d_solver aero3d_solver;
 /* Grid of the half domain y >= 0: face coordinates of each axis (Float64,
  * NX+1 / NY+1 / NZ+1 entries), as the reference's g.X.f / g.Y.f / g.Z.f
  * (src/air_cfd.js:36). Cell widths and centres are rounded to FP32 exactly as
- * the reference grid stores them. */
+ * the reference grid stores them. NX, NZ >= 1, NY >= 2 (the mirror ghosts of
+ * rows -1, -2 are rows 0, 1) and NX*NY*NZ <= 2^30; other sizes are rejected
+ * with AERO3D_E_ARG. */
 typedef struct {
   int NX, NY, NZ;
   const double *fx, *fy, *fz;

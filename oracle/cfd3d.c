@@ -188,6 +188,8 @@ static float *dupf(const float *p, int n) { float *d = (float *)malloc(sizeof(fl
 
 O3 *o3_create(int NX, int NY, int NZ, const float *dx, const float *dy, const float *dz, const uint8_t *t,
               const float *phi, const float *ap0, const float *ap1, const float *ap2, const float *ap3) {
+  /* the reference's y = 0 mirror reads rows 0 and 1 (src/air_cfd.js:205-210) */
+  if (NX < 1 || NY < 2 || NZ < 1 || (long long)NX * NY * NZ > (1LL << 30)) return NULL;
   O3 *s = (O3 *)calloc(1, sizeof(O3));
   s->NX = NX; s->NY = NY; s->NZ = NZ; s->N = NX * NY * NZ;
   const int N = s->N;

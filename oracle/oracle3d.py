@@ -62,6 +62,8 @@ class Oracle3D:
         self._keep = [a(grid["dx"], np.float32), a(grid["dy"], np.float32), a(grid["dz"], np.float32),
                       a(geo["t"], np.uint8), a(geo["phi"], np.float32)] + [a(geo["ap"][d], np.float32) for d in range(4)]
         self.h = L.o3_create(self.NX, self.NY, self.NZ, *self._keep)
+        if not self.h:
+            raise ValueError(f"grid {self.NX}x{self.NY}x{self.NZ} out of range (NX, NZ >= 1, NY >= 2)")
         L.o3_set_threads(self.h, threads)
 
     def __del__(self):

@@ -46,4 +46,11 @@ def test_create_rejects_bad_arguments():
     h = C.c_void_p()
     rc = L.aero3d_create(None, np.zeros(1, np.uint8), *[np.zeros(1, np.float32)] * 5, C.byref(h))
     assert rc == -1 and b"null" in L.aero3d_last_error(None)
+    # sizes the stencil cannot take are rejected before any device work: one row
+    # (the y = 0 mirror reads rows 0 and 1), an empty axis, more than 2^30 cells
+    for n in ((4, 1, 3), (0, 4, 3), (1 << 11, 1 << 10, 1 << 10)):
+        f = [np.arange(m + 1.0) for m in n]
+        g = aero3d.Grid(*n, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in f])
+        rc = L.aero3d_create(C.byref(g), np.zeros(1, np.uint8), *[np.zeros(1, np.float32)] * 5, C.byref(h))
+        assert rc == -1 and b"out of range" in L.aero3d_last_error(None), n
     assert L.aero3d_destroy(None) == 0

@@ -32,6 +32,28 @@ def test_tile_kernel_cpu_build_matches_oracle(emul_lib, case, iters, ty, zc):
     assert max(d.values()) < 1e-5, d
 
 
+# one OS thread per emulated GPU thread makes each run cost seconds: a subset of
+# tests/synth_cases.py SIZES (single cells, sub-tile, just past a tile, tall and
+# thin), each with a z chunk shorter and one longer than the grid
+SYNTH = [("synth:1:2:1", 8, 3), ("synth:2:2:3", 4, 1), ("synth:3:5:2", 8, 1), ("synth:65:9:3", 8, 2),
+         ("synth:7:40:2", 4, 3), ("synth:200:2:1", 8, 3)]
+
+
+@pytest.mark.parametrize("case,ty,zc", SYNTH)
+def test_tile_kernel_cpu_build_small_grids(emul_lib, case, ty, zc):
+    """Grids of one to a few tiles, narrower and shorter than a tile and a z
+    chunk (tests/synth_cases.py), under AddressSanitizer: every access in
+    bounds and the oracle reproduced after 3 iterations."""
+    asan = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True, text=True).stdout.strip()
+    env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0")
+    r = subprocess.run([sys.executable, os.path.join(EMUL, "run_emul.py"), emul_lib, case, "3", str(ty), str(zc)],
+                       capture_output=True, text=True, env=env, timeout=900)
+    assert r.returncode == 0, r.stderr[-3000:]
+    import json
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    assert max(d.values()) < 1e-5, d
+
+
 def test_axi_kernels_cpu_build_matches_oracle(tmp_path):
     out = str(tmp_path / "libaxi_emul.so")
     subprocess.run(["g++", "-std=c++20", "-O1", "-g", "-fPIC", "-shared", "-fsanitize=address", "-fno-omit-frame-pointer",
