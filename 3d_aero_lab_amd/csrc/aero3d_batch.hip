@@ -609,6 +609,14 @@ double clock_s(const aero3d_server *s) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->epoch).count();
 }
 
+// arena bytes free with no case resident (the registered geometries stay)
+int64_t free_when_idle(const aero3d_batch *b) {
+  size_t geo = 0;
+  for (const auto &G : b->geos)
+    if (!G.dead && G.mem) geo += (G.bytes + kAlign - 1) / kAlign * kAlign;
+  return (int64_t)b->arena.cap - (int64_t)geo;
+}
+
 }  // namespace
 
 extern "C" {
@@ -644,6 +652,13 @@ int aero3d_server_submit(aero3d_server *s, const aero3d_request *r) {
   if (!s || !r) return AERO3D_E_ARG;
   if (r->geometry < 0 || r->geometry >= (int)s->b->geos.size() || s->b->geos[r->geometry].dead) return AERO3D_E_ARG;
   if (r->max_iters < 1 || !(r->res_drop > 0)) return AERO3D_E_ARG;
+  int64_t need;
+  int rc = aero3d_batch_case_bytes(s->b, r->geometry, &need);
+  if (rc) return rc;
+  if (free_when_idle(s->b) < need + (int64_t)kAlign) {
+    return bfail(s->b, AERO3D_E_STATE, "submit: request " + std::to_string(r->tag) + " needs " + std::to_string(need) +
+                                           " bytes, more than the arena holds beside the registered geometries");
+  }
   s->pending.push_back(Pending{*r, r->arrival});
   return 0;
 }
@@ -658,11 +673,26 @@ int aero3d_server_tick(aero3d_server *s, int *n_active, int *n_pending) {
     Pending &p = s->pending.front();
     int64_t need;
     if ((rc = aero3d_batch_case_bytes(b, p.r.geometry, &need))) return rc;
-    if ((int64_t)(b->arena.cap - b->arena.used) < need + (int64_t)kAlign) break;     // admission by HBM footprint
-    int slot;
-    rc = aero3d_batch_add(b, p.r.geometry, p.r.F, p.r.fix, p.r.cfl, p.r.order, &slot);
-    if (rc == AERO3D_E_STATE) break;                                                  // no slot / fragmented: wait
-    if (rc) return rc;
+    int slot = -1;
+    if ((int64_t)(b->arena.cap - b->arena.used) >= need + (int64_t)kAlign) {      // admission by HBM footprint
+      rc = aero3d_batch_add(b, p.r.geometry, p.r.F, p.r.fix, p.r.cfl, p.r.order, &slot);
+      if (rc && rc != AERO3D_E_STATE) return rc;
+      if (rc) slot = -1;                                                          // no slot / fragmented
+    }
+    if (slot < 0) {
+      if (!s->active.empty()) break;                 // wait: a resident case will free room
+      // nothing resident will free room (geometries registered after the
+      // submit, or slots taken on the batch directly): return the request as
+      // failed so the queue does not stall
+      aero3d_result o;
+      std::memset(&o, 0, sizeof o);
+      o.tag = p.r.tag; o.res0 = o.res = o.CL = o.CD = o.Cm = NAN;
+      o.arrival = p.arrival; o.t_admit = o.t_done = clock_s(s);
+      s->done.push_back(o);
+      b->err = "tick: request " + std::to_string(p.r.tag) + " cannot be admitted with no case resident";
+      s->pending.pop_front();
+      continue;
+    }
     Active a;
     a.r = p.r; a.slot = slot; a.arrival = p.arrival; a.t_admit = clock_s(s);
     s->active.push_back(a);

@@ -195,8 +195,12 @@ int aero3d_server_destroy(aero3d_server *s);
 aero3d_batch *aero3d_server_batch(aero3d_server *s);
 /* Seconds since the server was created (steady clock). */
 int aero3d_server_clock(const aero3d_server *s, double *now);
+/* Queue a request (FIFO). AERO3D_E_STATE when its case cannot fit the arena
+ * even with no case resident (message on aero3d_server_batch(s)). */
 int aero3d_server_submit(aero3d_server *s, const aero3d_request *r);
-/* Admit, step check_every iterations, retire; returns resident / queued counts. */
+/* Admit, step check_every iterations, retire; returns resident / queued counts.
+ * A queued request that still cannot be admitted when no case is resident is
+ * returned at once as a result with iters 0 and NaN residuals / coefficients. */
 int aero3d_server_tick(aero3d_server *s, int *n_active, int *n_pending);
 /* Move up to max finished results to out; *n = number moved. */
 int aero3d_server_poll(aero3d_server *s, aero3d_result *out, int max, int *n);

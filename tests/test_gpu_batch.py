@@ -149,3 +149,39 @@ def test_server_alone_equals_mixed_and_admission_by_footprint():
     admits = sorted(o["t_admit"] for o in mixed.values())
     dones = sorted(o["t_done"] for o in mixed.values())
     assert admits[-1] >= dones[0]
+
+
+def test_server_requests_that_cannot_fit_do_not_stall():
+    """A request larger than the arena beside the registered geometries is
+    refused at submit; one that a later geometry leaves without room is
+    returned as failed (iters 0, NaN coefficients) instead of waiting forever,
+    and the queue behind it is served."""
+    import aero3d
+    g = Golden("sphere")
+    F = g.s["F"]
+    probe = aero3d.Batch(arena_bytes=1 << 28, max_slots=2)
+    gi = probe.add_geometry(g.grid(), g.geo())
+    need, geo_bytes = probe.case_bytes(gi), probe.footprint()[0]
+    probe.dispose()
+    say("sphere case bytes", need, "geometry bytes", geo_bytes)
+    ref = (0.0, 0.4, 1.0, 0.0, 0.0, 1.0)
+    # arena smaller than geometry + one case: refused at submit
+    srv = aero3d.Server(arena_bytes=geo_bytes + need // 2, max_slots=4, check_every=10)
+    gid = srv.batch.add_geometry(g.grid(), g.geo())
+    with pytest.raises(aero3d.Aero3DError, match="more than the arena"):
+        srv.submit(gid, F, [F] * 3, 0.5, 2, ref, 1e-4, 20, srv.clock(), 7)
+    assert srv.tick() == (0, 0)
+    srv.dispose()
+    # room for one case; a second geometry registered after the submit takes it
+    srv = aero3d.Server(arena_bytes=geo_bytes + need + 8192, max_slots=4, check_every=10)
+    gid = srv.batch.add_geometry(g.grid(), g.geo())
+    srv.submit(gid, F, [F] * 3, 0.5, 2, ref, 1e-4, 20, srv.clock(), 1)
+    gid2 = srv.batch.add_geometry(g.grid(), g.geo())
+    a, p = srv.tick()
+    out = srv.poll()
+    say("stranded request", out)
+    assert (a, p) == (0, 0) and len(out) == 1 and out[0]["tag"] == 1 and out[0]["iters"] == 0
+    assert np.isnan(out[0]["CL"]) and out[0]["converged"] == 0
+    with pytest.raises(aero3d.Aero3DError, match="more than the arena"):
+        srv.submit(gid2, F, [F] * 3, 0.5, 2, ref, 1e-4, 20, srv.clock(), 2)
+    srv.dispose()
