@@ -6,7 +6,9 @@
 // ============================================================================
 #include <node_api.h>
 
+#include <mutex>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "aero3d.h"
@@ -55,13 +57,56 @@ bool typed(napi_env env, napi_value v, napi_typedarray_type want, T **data, size
   return true;
 }
 
-aero3d_solver *handle(napi_env env, napi_value v) {
+// A JS handle is an external pointing at one of these records: the solver and
+// its cell count (every array argument is checked against it). The addon keeps
+// the set of live records, so a foreign external, a handle of the other solver
+// kind or a handle used after dispose is a JS error, not a stray pointer. The
+// external's finalizer frees the record and, for a handle never disposed, the
+// solver and its device memory.
+struct Rec3 { aero3d_solver *s; size_t N; };
+struct RecAxi { axi_solver *s; size_t N; };
+std::mutex g_mu;
+std::unordered_set<const void *> g_rec3, g_recaxi;
+
+void fin3(napi_env, void *d, void *) {
+  Rec3 *r = static_cast<Rec3 *>(d);
+  { std::lock_guard<std::mutex> l(g_mu); g_rec3.erase(r); }
+  if (r->s) aero3d_destroy(r->s);
+  delete r;
+}
+void finaxi(napi_env, void *d, void *) {
+  RecAxi *r = static_cast<RecAxi *>(d);
+  { std::lock_guard<std::mutex> l(g_mu); g_recaxi.erase(r); }
+  if (r->s) axi_destroy(r->s);
+  delete r;
+}
+
+template <class R>
+R *record(napi_env env, napi_value v, const std::unordered_set<const void *> &live, const char *what) {
   void *p = nullptr;
-  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
-    napi_throw_type_error(env, nullptr, "aero3d addon: not a solver handle (disposed?)");
+  bool ok = napi_get_value_external(env, v, &p) == napi_ok && p;
+  if (ok) { std::lock_guard<std::mutex> l(g_mu); ok = live.count(p) > 0; }
+  if (!ok) {
+    std::string m = std::string("aero3d addon: not ") + what;
+    napi_throw_type_error(env, nullptr, m.c_str());
     return nullptr;
   }
-  return static_cast<aero3d_solver *>(p);
+  R *r = static_cast<R *>(p);
+  if (!r->s) { napi_throw_error(env, nullptr, "aero3d addon: solver already disposed"); return nullptr; }
+  return r;
+}
+Rec3 *rec3(napi_env env, napi_value v) { return record<Rec3>(env, v, g_rec3, "a 3D solver handle"); }
+
+aero3d_solver *handle(napi_env env, napi_value v) {
+  Rec3 *r = rec3(env, v);
+  return r ? r->s : nullptr;
+}
+
+bool check_len(napi_env env, size_t n, size_t want, const char *what) {
+  if (n == want) return true;
+  std::string m = std::string("aero3d addon: ") + what + " has " + std::to_string(n) + " elements, needs " + std::to_string(want);
+  napi_throw_range_error(env, nullptr, m.c_str());
+  return false;
 }
 
 double num(napi_env env, napi_value v) { double d = 0; napi_get_value_double(env, v, &d); return d; }
@@ -72,32 +117,42 @@ napi_value undef(napi_env env) { napi_value u; napi_get_undefined(env, &u); retu
 napi_value Create(napi_env env, napi_callback_info info) {
   napi_value a[9];
   if (!args(env, info, 9, a)) return nullptr;
-  double *fx, *fy, *fz; size_t nx, ny, nz, n;
+  double *fx, *fy, *fz; size_t nx, ny, nz, n, nphi, nap[4];
   uint8_t *t; float *phi, *ap[4];
   if (!typed(env, a[0], napi_float64_array, &fx, &nx) || !typed(env, a[1], napi_float64_array, &fy, &ny) ||
       !typed(env, a[2], napi_float64_array, &fz, &nz) || !typed(env, a[3], napi_uint8_array, &t, &n) ||
-      !typed(env, a[4], napi_float32_array, &phi, nullptr))
+      !typed(env, a[4], napi_float32_array, &phi, &nphi))
     return nullptr;
-  for (int d = 0; d < 4; d++) if (!typed(env, a[5 + d], napi_float32_array, &ap[d], nullptr)) return nullptr;
-  if (nx < 2 || ny < 2 || nz < 2 || n != (nx - 1) * (ny - 1) * (nz - 1)) {
-    napi_throw_range_error(env, nullptr, "aero3d addon: mask length does not match the grid");
+  for (int d = 0; d < 4; d++) if (!typed(env, a[5 + d], napi_float32_array, &ap[d], &nap[d])) return nullptr;
+  if (nx < 2 || ny < 2 || nz < 2 || nx > (1u << 30) || ny > (1u << 30) || nz > (1u << 30)) {
+    napi_throw_range_error(env, nullptr, "aero3d addon: each axis needs at least two faces");
     return nullptr;
   }
+  const size_t N = (nx - 1) * (ny - 1) * (nz - 1);
+  if (!check_len(env, n, N, "t") || !check_len(env, nphi, N, "phi")) return nullptr;
+  for (int d = 0; d < 4; d++) if (!check_len(env, nap[d], N, "ap")) return nullptr;
   aero3d_grid g{(int)nx - 1, (int)ny - 1, (int)nz - 1, fx, fy, fz};
   aero3d_solver *s = nullptr;
   int rc = aero3d_create(&g, t, phi, ap[0], ap[1], ap[2], ap[3], &s);
   if (rc) return throw_rc(env, nullptr, rc);
+  Rec3 *r = new Rec3{s, N};
+  { std::lock_guard<std::mutex> l(g_mu); g_rec3.insert(r); }
   napi_value ext;
-  NAPI_OK(napi_create_external(env, s, nullptr, nullptr, &ext));
+  if (napi_create_external(env, r, fin3, nullptr, &ext) != napi_ok) {
+    fin3(env, r, nullptr);
+    napi_throw_error(env, nullptr, "aero3d addon: Node-API call failed");
+    return nullptr;
+  }
   return ext;
 }
 
 napi_value Dispose(napi_env env, napi_callback_info info) {
   napi_value a[1];
   if (!args(env, info, 1, a)) return nullptr;
-  aero3d_solver *s = handle(env, a[0]);
-  if (!s) return nullptr;
-  aero3d_destroy(s);
+  Rec3 *r = rec3(env, a[0]);
+  if (!r) return nullptr;
+  aero3d_destroy(r->s);
+  r->s = nullptr;                // the record lives until the external is collected
   return undef(env);
 }
 
@@ -167,14 +222,16 @@ napi_value Iter(napi_env env, napi_callback_info info) {
 napi_value Read(napi_env env, napi_callback_info info) {
   napi_value a[7];
   if (!args(env, info, 7, a)) return nullptr;
-  aero3d_solver *s = handle(env, a[0]);
-  if (!s) return nullptr;
+  Rec3 *r = rec3(env, a[0]);
+  if (!r) return nullptr;
+  aero3d_solver *s = r->s;
   float *p[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   for (int c = 0; c < 6; c++) {
     napi_valuetype vt;
     napi_typeof(env, a[1 + c], &vt);
     if (vt == napi_null || vt == napi_undefined) continue;
-    if (!typed(env, a[1 + c], napi_float32_array, &p[c], nullptr)) return nullptr;
+    size_t n;
+    if (!typed(env, a[1 + c], napi_float32_array, &p[c], &n) || !check_len(env, n, r->N, "read output")) return nullptr;
   }
   int rc = aero3d_read(s, p[0], p[1], p[2], p[3], p[4], p[5]);
   return rc ? throw_rc(env, s, rc) : undef(env);
@@ -183,10 +240,14 @@ napi_value Read(napi_env env, napi_callback_info info) {
 napi_value Write(napi_env env, napi_callback_info info) {
   napi_value a[6];
   if (!args(env, info, 6, a)) return nullptr;
-  aero3d_solver *s = handle(env, a[0]);
-  if (!s) return nullptr;
+  Rec3 *r = rec3(env, a[0]);
+  if (!r) return nullptr;
+  aero3d_solver *s = r->s;
   float *p[5];
-  for (int c = 0; c < 5; c++) if (!typed(env, a[1 + c], napi_float32_array, &p[c], nullptr)) return nullptr;
+  for (int c = 0; c < 5; c++) {
+    size_t n;
+    if (!typed(env, a[1 + c], napi_float32_array, &p[c], &n) || !check_len(env, n, r->N, "write input")) return nullptr;
+  }
   int rc = aero3d_write(s, p[0], p[1], p[2], p[3], p[4]);
   return rc ? throw_rc(env, s, rc) : undef(env);
 }
@@ -207,7 +268,10 @@ napi_value SetForceList(napi_env env, napi_callback_info info) {
   aero3d_solver *s = handle(env, a[0]);
   int32_t *q; size_t n; double *d[4];
   if (!s || !typed(env, a[1], napi_int32_array, &q, &n)) return nullptr;
-  for (int c = 0; c < 4; c++) if (!typed(env, a[2 + c], napi_float64_array, &d[c], nullptr)) return nullptr;
+  for (int c = 0; c < 4; c++) {
+    size_t m;
+    if (!typed(env, a[2 + c], napi_float64_array, &d[c], &m) || !check_len(env, m, n, "force-list column")) return nullptr;
+  }
   int rc = aero3d_set_force_list(s, q, d[0], d[1], d[2], d[3], (int)n);
   return rc ? throw_rc(env, s, rc) : undef(env);
 }
@@ -292,13 +356,10 @@ napi_value axi_throw(napi_env env, axi_solver *s, int rc) {
   napi_throw_error(env, nullptr, msg.c_str());
   return nullptr;
 }
+RecAxi *recaxi(napi_env env, napi_value v) { return record<RecAxi>(env, v, g_recaxi, "an axisymmetric solver handle"); }
 axi_solver *axi_handle(napi_env env, napi_value v) {
-  void *p = nullptr;
-  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
-    napi_throw_type_error(env, nullptr, "aero3d addon: not an axisymmetric solver handle (disposed?)");
-    return nullptr;
-  }
-  return static_cast<axi_solver *>(p);
+  RecAxi *r = recaxi(env, v);
+  return r ? r->s : nullptr;
 }
 bool get_prop(napi_env env, napi_value o, const char *k, napi_value *v) {
   bool has = false;
@@ -339,24 +400,32 @@ napi_value AxiCreate(napi_env env, napi_callback_info info) {
       !typed(env, a[3], napi_float64_array, &comps, &nc) || !typed(env, a[4], napi_float64_array, &fs, &nf) ||
       !typed(env, a[5], napi_float64_array, &Q0, &nq))
     return nullptr;
+  if (nx < 3 || nr < 3 || nx > (1u << 20) || nr > (1u << 20)) { napi_throw_range_error(env, nullptr, "axiCreate: sizes"); return nullptr; }
   const size_t N = (nx - 1) * (nr - 1);
-  if (nx < 3 || nr < 3 || nc != 32 || nf != 3 || nq != 5 * N) { napi_throw_range_error(env, nullptr, "axiCreate: sizes"); return nullptr; }
+  if (nc != 32 || nf != 3 || nq != 5 * N) { napi_throw_range_error(env, nullptr, "axiCreate: sizes"); return nullptr; }
   axi_zones z;
   if (!zones_of(env, a[2], N, &z)) return nullptr;
   axi_grid g{(int)nx - 1, (int)nr - 1, xf, rf};
   axi_solver *s = nullptr;
   int rc = axi_create(&g, &z, comps, fs, Q0, &s);
   if (rc) return axi_throw(env, nullptr, rc);
+  RecAxi *r = new RecAxi{s, N};
+  { std::lock_guard<std::mutex> l(g_mu); g_recaxi.insert(r); }
   napi_value ext;
-  NAPI_OK(napi_create_external(env, s, nullptr, nullptr, &ext));
+  if (napi_create_external(env, r, finaxi, nullptr, &ext) != napi_ok) {
+    finaxi(env, r, nullptr);
+    napi_throw_error(env, nullptr, "aero3d addon: Node-API call failed");
+    return nullptr;
+  }
   return ext;
 }
 napi_value AxiDispose(napi_env env, napi_callback_info info) {
   napi_value a[1];
   if (!args(env, info, 1, a)) return nullptr;
-  axi_solver *s = axi_handle(env, a[0]);
-  if (!s) return nullptr;
-  axi_destroy(s);
+  RecAxi *r = recaxi(env, a[0]);
+  if (!r) return nullptr;
+  axi_destroy(r->s);
+  r->s = nullptr;                // the record lives until the external is collected
   return undef(env);
 }
 // axiSetOpt(h, cfl, order, visc, hf, global, dtRatio)
@@ -381,10 +450,12 @@ napi_value AxiSetComps(napi_env env, napi_callback_info info) {
 napi_value AxiSetZones(napi_env env, napi_callback_info info) {
   napi_value a[3];
   if (!args(env, info, 3, a)) return nullptr;
-  axi_solver *s = axi_handle(env, a[0]);
-  if (!s) return nullptr;
+  RecAxi *r = recaxi(env, a[0]);
+  if (!r) return nullptr;
+  axi_solver *s = r->s;
+  if ((size_t)num(env, a[2]) != r->N) { napi_throw_range_error(env, nullptr, "setZones: cell count differs from the solver's"); return nullptr; }
   axi_zones z;
-  if (!zones_of(env, a[1], (size_t)num(env, a[2]), &z)) return nullptr;
+  if (!zones_of(env, a[1], r->N, &z)) return nullptr;
   int rc = axi_set_zones(s, &z);
   return rc ? axi_throw(env, s, rc) : undef(env);
 }
@@ -401,9 +472,10 @@ napi_value AxiSetFs(napi_env env, napi_callback_info info) {
 napi_value AxiSetState(napi_env env, napi_callback_info info) {
   napi_value a[2];
   if (!args(env, info, 2, a)) return nullptr;
-  axi_solver *s = axi_handle(env, a[0]);
+  RecAxi *r = recaxi(env, a[0]);
   double *q; size_t n;
-  if (!s || !typed(env, a[1], napi_float64_array, &q, &n)) return nullptr;
+  if (!r || !typed(env, a[1], napi_float64_array, &q, &n) || !check_len(env, n, 5 * r->N, "setState")) return nullptr;
+  axi_solver *s = r->s;
   int rc = axi_set_state(s, q);
   return rc ? axi_throw(env, s, rc) : undef(env);
 }
@@ -419,11 +491,14 @@ napi_value AxiStep(napi_env env, napi_callback_info info) {
 napi_value AxiRead(napi_env env, napi_callback_info info) {
   napi_value a[4];
   if (!args(env, info, 4, a)) return nullptr;
-  axi_solver *s = axi_handle(env, a[0]);
+  RecAxi *r = recaxi(env, a[0]);
   float *q, *sr, *mf;
-  if (!s || !typed(env, a[1], napi_float32_array, &q, nullptr) || !typed(env, a[2], napi_float32_array, &sr, nullptr) ||
-      !typed(env, a[3], napi_float32_array, &mf, nullptr))
+  size_t nq, ns, nm;
+  if (!r || !typed(env, a[1], napi_float32_array, &q, &nq) || !typed(env, a[2], napi_float32_array, &sr, &ns) ||
+      !typed(env, a[3], napi_float32_array, &mf, &nm) || !check_len(env, nq, 5 * r->N, "axiRead Q") ||
+      !check_len(env, ns, r->N, "axiRead src") || !check_len(env, nm, r->N, "axiRead mfx"))
     return nullptr;
+  axi_solver *s = r->s;
   int rc = axi_read(s, q, sr, mf, nullptr);
   return rc ? axi_throw(env, s, rc) : undef(env);
 }

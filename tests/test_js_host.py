@@ -92,6 +92,30 @@ def test_addon_exports():
         assert k in keys
 
 
+def test_addon_rejects_foreign_handles():
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "addon_guard_check.js"), "cpu"], cwd=ROOT,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    assert all(v.startswith("TypeError") and "handle" in v for v in res.values()), res
+
+
+@pytest.mark.gpu
+def test_addon_argument_guards():
+    """Short arrays, a handle of the other solver kind, use after dispose and a
+    second dispose are JS errors, never a read or write past a buffer."""
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "addon_guard_check.js"), "gpu"], cwd=ROOT,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    print("[addon-guards]", res, flush=True)
+    for k in ("create_short_phi", "read_short_output", "write_short_input", "force_list_short_column"):
+        assert res[k].startswith("RangeError"), (k, res[k])
+    assert res["axi_call_with_3d_handle"].startswith("TypeError"), res
+    assert res["read_full_size"] == "no error", res
+    assert "disposed" in res["step_after_dispose"] and "disposed" in res["dispose_twice"], res
+
+
 @pytest.mark.gpu
 def test_js_dropin_fighter_readme_row():
     g = Golden("fighter30k")
