@@ -460,17 +460,15 @@ int aero3d_residual(aero3d_solver *s, double *res) {
 
 int aero3d_time_steps(aero3d_solver *s, int n, float *ms) {
   if (!s || !ms || n < 0) return AERO3D_E_ARG;
-  hipEvent_t e0, e1;
-  HIPCHK(s, hipEventCreate(&e0));
-  HIPCHK(s, hipEventCreate(&e1));
-  HIPCHK(s, hipEventRecord(e0, s->stream));
+  struct Ev { hipEvent_t e = nullptr; ~Ev() { if (e) (void)hipEventDestroy(e); } } e0, e1;   // released on every path
+  HIPCHK(s, hipEventCreate(&e0.e));
+  HIPCHK(s, hipEventCreate(&e1.e));
+  HIPCHK(s, hipEventRecord(e0.e, s->stream));
   int rc = enqueue_steps(s, n);
-  if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
-  HIPCHK(s, hipEventRecord(e1, s->stream));
-  HIPCHK(s, hipEventSynchronize(e1));
-  HIPCHK(s, hipEventElapsedTime(ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  if (rc) return rc;
+  HIPCHK(s, hipEventRecord(e1.e, s->stream));
+  HIPCHK(s, hipEventSynchronize(e1.e));
+  HIPCHK(s, hipEventElapsedTime(ms, e0.e, e1.e));
   return 0;
 }
 

@@ -557,18 +557,17 @@ int aero3d_batch_footprint(const aero3d_batch *b, int64_t *used, int64_t *capaci
 }
 
 int aero3d_batch_time_steps(aero3d_batch *b, int n, float *ms) {
-  if (!b || !ms) return AERO3D_E_ARG;
+  if (!b || !ms || n < 0) return AERO3D_E_ARG;
   int rc;
   if (b->table_dirty && (rc = upload_table(b))) return rc;
-  hipEvent_t e0, e1;
-  BCHK(b, hipEventCreate(&e0));
-  BCHK(b, hipEventCreate(&e1));
-  BCHK(b, hipEventRecord(e0, b->stream));
+  struct Ev { hipEvent_t e = nullptr; ~Ev() { if (e) (void)hipEventDestroy(e); } } e0, e1;   // released on every path
+  BCHK(b, hipEventCreate(&e0.e));
+  BCHK(b, hipEventCreate(&e1.e));
+  BCHK(b, hipEventRecord(e0.e, b->stream));
   if ((rc = batch_steps(b, n))) return rc;
-  BCHK(b, hipEventRecord(e1, b->stream));
-  BCHK(b, hipEventSynchronize(e1));
-  BCHK(b, hipEventElapsedTime(ms, e0, e1));
-  hipEventDestroy(e0); hipEventDestroy(e1);
+  BCHK(b, hipEventRecord(e1.e, b->stream));
+  BCHK(b, hipEventSynchronize(e1.e));
+  BCHK(b, hipEventElapsedTime(ms, e0.e, e1.e));
   return 0;
 }
 

@@ -79,12 +79,13 @@ int upload_zones(axi_solver *s, const axi_zones *z) {
   if (!z || !z->solid || !z->comp || !z->phi || !z->ae || !z->an || !z->wgt || !z->lmix || !z->tgt || !z->rate || !z->yrate || !z->mfd)
     return afail(s, -1, "zones: null array");
   const size_t N = s->N;
+  // component ids index the constant table: checked before anything reaches the device
+  for (size_t k = 0; k < N; k++)
+    if (z->comp[k] >= 8) return afail(s, -1, "zones: component id >= 8");
   ACHK(s, hipMemcpy(s->solid, z->solid, N, hipMemcpyHostToDevice));
   ACHK(s, hipMemcpy(s->comp, z->comp, N, hipMemcpyHostToDevice));
   const float *planes[9] = {z->phi, z->ae, z->an, z->wgt, z->lmix, z->tgt, z->rate, z->yrate, z->mfd};
   for (int p = 0; p < 9; p++) ACHK(s, hipMemcpy(s->zf + p * N, planes[p], 4 * N, hipMemcpyHostToDevice));
-  for (size_t k = 0; k < N; k++)
-    if (z->comp[k] >= 8) return afail(s, -1, "zones: component id >= 8");
   return 0;
 }
 
@@ -151,7 +152,8 @@ extern "C" {
 int axi_create(const axi_grid *g, const axi_zones *z, const double comps[32], const double fs[3], const double *Q0, axi_solver **out) {
   if (!out) return afail(nullptr, -1, "axi_create: null output");
   *out = nullptr;
-  if (!g || !g->xf || !g->rf || !z || !comps || !fs || !Q0 || g->W < 2 || g->H < 2) return afail(nullptr, -1, "axi_create: bad argument");
+  if (!g || !g->xf || !g->rf || !z || !comps || !fs || !Q0 || g->W < 2 || g->H < 2 || (int64_t)g->W * g->H > (int64_t)1 << 28)
+    return afail(nullptr, -1, "axi_create: bad argument (W, H >= 2, W*H <= 2^28)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return afail(nullptr, -2, "axi_create: no HIP device");
   axi_solver *s = new axi_solver();
@@ -255,18 +257,16 @@ int axi_sync(axi_solver *s) {
   return 0;
 }
 int axi_time_steps(axi_solver *s, int n, float *ms) {
-  if (!s || !ms) return -1;
-  hipEvent_t e0, e1;
-  ACHK(s, hipEventCreate(&e0));
-  ACHK(s, hipEventCreate(&e1));
-  ACHK(s, hipEventRecord(e0, s->stream));
+  if (!s || !ms || n < 0) return -1;
+  struct Ev { hipEvent_t e = nullptr; ~Ev() { if (e) (void)hipEventDestroy(e); } } e0, e1;   // released on every path
+  ACHK(s, hipEventCreate(&e0.e));
+  ACHK(s, hipEventCreate(&e1.e));
+  ACHK(s, hipEventRecord(e0.e, s->stream));
   int rc = steps(s, n);
   if (rc) return rc;
-  ACHK(s, hipEventRecord(e1, s->stream));
-  ACHK(s, hipEventSynchronize(e1));
-  ACHK(s, hipEventElapsedTime(ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  ACHK(s, hipEventRecord(e1.e, s->stream));
+  ACHK(s, hipEventSynchronize(e1.e));
+  ACHK(s, hipEventElapsedTime(ms, e0.e, e1.e));
   return 0;
 }
 

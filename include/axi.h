@@ -18,7 +18,8 @@ extern "C" {
 
 typedef struct axi_solver axi_solver;
 
-/* Grid of buildCFDGrid (src/cfd_grid.js:28-43): W x H cells, Float64 faces. */
+/* Grid of buildCFDGrid (src/cfd_grid.js:28-43): W x H cells, Float64 faces;
+ * W, H >= 2 and W*H <= 2^28. */
 typedef struct {
   int W, H;
   const double *xf, *rf;
