@@ -433,7 +433,14 @@ class AxiZones(C.Structure):
 
 
 AXI_EXPORTS = ["axi_create", "axi_destroy", "axi_last_error", "axi_set_opt", "axi_set_comps", "axi_set_zones", "axi_set_fs",
-               "axi_set_state", "axi_step", "axi_iter", "axi_read", "axi_sync", "axi_time_steps"]
+               "axi_set_state", "axi_step", "axi_iter", "axi_read", "axi_sync", "axi_time_steps", "axi_monitor",
+               "axi_monitor_sections"]
+
+
+class AxiMonitorOut(C.Structure):
+    _fields_ = [("F", C.c_double), ("Fb", C.c_double), ("Fw", C.c_double), ("mfuel", C.c_double), ("pw", C.c_double * 8),
+                ("res_sum2", C.c_double), ("res_n", C.c_double), ("res_valid", C.c_int), ("nsec", C.c_int),
+                ("sec_mdot", C.c_double * 4), ("sec_w", C.c_double * 4), ("sec_wTt", C.c_double * 4)]
 _alib = None
 
 
@@ -456,6 +463,8 @@ def alib():
         L.axi_read.argtypes = [vp] + [C.c_void_p] * 4
         L.axi_sync.argtypes = [vp]
         L.axi_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_float)]
+        L.axi_monitor.argtypes = [vp, f64p, C.c_double, C.POINTER(AxiMonitorOut)]
+        L.axi_monitor_sections.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p]
         _alib = L
     return _alib
 
@@ -548,5 +557,20 @@ class AxiHIP:
         ms = C.c_float()
         self._chk(alib().axi_time_steps(self.h, n, C.byref(ms)))
         return ms.value
+
+    def monitor_sections(self, cols, rows):
+        """cols: column per section; rows: (nsec, H) mask of the rows each section counts."""
+        cols = np.ascontiguousarray(cols, np.int32)
+        rows = np.ascontiguousarray(rows, np.uint8).reshape(-1)
+        assert rows.size == cols.size * self.H
+        self._chk(alib().axi_monitor_sections(self.h, int(cols.size), cols.ctypes.data, rows.ctypes.data))
+
+    def monitor(self, comps, P0):
+        o = AxiMonitorOut()
+        self._chk(alib().axi_monitor(self.h, np.ascontiguousarray(comps, np.float64), P0, C.byref(o)))
+        n = o.nsec
+        return {"F": o.F, "Fb": o.Fb, "Fw": o.Fw, "mfuel": o.mfuel, "pw": list(o.pw), "res_sum2": o.res_sum2,
+                "res_n": o.res_n, "res_valid": bool(o.res_valid),
+                "sec": [{"mdot": o.sec_mdot[q], "w": o.sec_w[q], "wTt": o.sec_wTt[q]} for q in range(n)]}
 
 EXPORTS = EXPORTS + AXI_EXPORTS

@@ -64,7 +64,7 @@ bool typed(napi_env env, napi_value v, napi_typedarray_type want, T **data, size
 // external's finalizer frees the record and, for a handle never disposed, the
 // solver and its device memory.
 struct Rec3 { aero3d_solver *s; size_t N; };
-struct RecAxi { axi_solver *s; size_t N; };
+struct RecAxi { axi_solver *s; size_t N; int W, H; };
 std::mutex g_mu;
 std::unordered_set<const void *> g_rec3, g_recaxi;
 
@@ -409,7 +409,7 @@ napi_value AxiCreate(napi_env env, napi_callback_info info) {
   axi_solver *s = nullptr;
   int rc = axi_create(&g, &z, comps, fs, Q0, &s);
   if (rc) return axi_throw(env, nullptr, rc);
-  RecAxi *r = new RecAxi{s, N};
+  RecAxi *r = new RecAxi{s, N, (int)nx - 1, (int)nr - 1};
   { std::lock_guard<std::mutex> l(g_mu); g_recaxi.insert(r); }
   napi_value ext;
   if (napi_create_external(env, r, finaxi, nullptr, &ext) != napi_ok) {
@@ -502,6 +502,37 @@ napi_value AxiRead(napi_env env, napi_callback_info info) {
   int rc = axi_read(s, q, sr, mf, nullptr);
   return rc ? axi_throw(env, s, rc) : undef(env);
 }
+// axiMonitorSections(h, cols Int32Array(n), rows Uint8Array(n H))
+napi_value AxiMonitorSections(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!args(env, info, 3, a)) return nullptr;
+  RecAxi *r = recaxi(env, a[0]);
+  int32_t *cols; uint8_t *rows; size_t n, m;
+  if (!r || !typed(env, a[1], napi_int32_array, &cols, &n) || !typed(env, a[2], napi_uint8_array, &rows, &m) ||
+      !check_len(env, m, n * (size_t)r->H, "section rows"))
+    return nullptr;
+  int rc = axi_monitor_sections(r->s, (int)n, cols, rows);
+  return rc ? axi_throw(env, r->s, rc) : undef(env);
+}
+// axiMonitor(h, comps Float64Array(32), P0, out Float64Array(28)):
+// F Fb Fw mfuel pw[8] res_sum2 res_n res_valid nsec (mdot w wTt) x 4
+napi_value AxiMonitor(napi_env env, napi_callback_info info) {
+  napi_value a[4];
+  if (!args(env, info, 4, a)) return nullptr;
+  RecAxi *r = recaxi(env, a[0]);
+  double *c, *o; size_t nc, no;
+  if (!r || !typed(env, a[1], napi_float64_array, &c, &nc) || !check_len(env, nc, 32, "comps") ||
+      !typed(env, a[3], napi_float64_array, &o, &no) || !check_len(env, no, 28, "monitor output"))
+    return nullptr;
+  axi_monitor_out m;
+  int rc = axi_monitor(r->s, c, num(env, a[2]), &m);
+  if (rc) return axi_throw(env, r->s, rc);
+  o[0] = m.F; o[1] = m.Fb; o[2] = m.Fw; o[3] = m.mfuel;
+  for (int q = 0; q < 8; q++) o[4 + q] = m.pw[q];
+  o[12] = m.res_sum2; o[13] = m.res_n; o[14] = m.res_valid; o[15] = m.nsec;
+  for (int q = 0; q < 4; q++) { o[16 + 3 * q] = m.sec_mdot[q]; o[17 + 3 * q] = m.sec_w[q]; o[18 + 3 * q] = m.sec_wTt[q]; }
+  return undef(env);
+}
 napi_value AxiSync(napi_env env, napi_callback_info info) {
   napi_value a[1];
   if (!args(env, info, 1, a)) return nullptr;
@@ -519,7 +550,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"timeSteps", TimeSteps}, {"autotune", Autotune},
       {"axiCreate", AxiCreate}, {"axiDispose", AxiDispose}, {"axiSetOpt", AxiSetOpt}, {"axiSetComps", AxiSetComps},
       {"axiSetZones", AxiSetZones}, {"axiSetFs", AxiSetFs}, {"axiSetState", AxiSetState}, {"axiStep", AxiStep},
-      {"axiRead", AxiRead}, {"axiSync", AxiSync}};
+      {"axiRead", AxiRead}, {"axiSync", AxiSync}, {"axiMonitorSections", AxiMonitorSections}, {"axiMonitor", AxiMonitor}};
   for (auto &f : fns) {
     napi_value fn;
     NAPI_OK(napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));

@@ -52,6 +52,16 @@ struct axi_solver {
   bool graph_capped = false;
   int global = 0;
   double dtRatio = 0;
+  // readback monitors (axi_monitor): FP64 grid widths, the density of the last
+  // monitored state, section columns and row masks, partial and final sums
+  double *rcD = nullptr, *drD = nullptr, *dxD = nullptr;
+  float *rsnap = nullptr;
+  bool snap_valid = false;
+  int nsec = 0;
+  int *sec_col = nullptr;
+  uint8_t *sec_rows = nullptr;
+  double *mon_part = nullptr, *mon_out = nullptr;
+  MonIn *mon_in = nullptr;
   std::string err;
 };
 
@@ -172,12 +182,23 @@ int axi_create(const axi_grid *g, const axi_zones *z, const double comps[32], co
   auto al = [&](auto **p, size_t n) { return hipMalloc((void **)p, sizeof(**p) * (n ? n : 1)); };
   if (al(&s->rc, s->H) || al(&s->dx, s->W) || al(&s->dr, s->H) || al(&s->rf, s->H + 1) || al(&s->solid, N) || al(&s->comp, N) ||
       al(&s->zf, 9 * N) || al(&s->Q, 5 * N) || al(&s->Q1, 5 * N) || al(&s->Fx, 5 * NXF) || al(&s->Fr, 5 * NRF) || al(&s->nutF, NRF) ||
-      al(&s->dt, N) || al(&s->src, N) || al(&s->mfx, N) || al(&s->partial, 1024) || al(&s->cap, 1) || al(&s->dc, 1))
+      al(&s->dt, N) || al(&s->src, N) || al(&s->mfx, N) || al(&s->partial, 1024) || al(&s->cap, 1) || al(&s->dc, 1) ||
+      al(&s->rcD, s->H) || al(&s->drD, s->H) || al(&s->dxD, s->W) || al(&s->rsnap, N) || al(&s->sec_col, kMonMaxSec) ||
+      al(&s->sec_rows, (size_t)kMonMaxSec * s->H) || al(&s->mon_part, (size_t)kMonBlocks * kMonK) ||
+      al(&s->mon_out, kMonK + 3 * kMonMaxSec) || al(&s->mon_in, 1))
     return bail(afail(s, -1002, "axi_create: hipMalloc failed"));
   if (hipMemcpy(s->rc, rc.data(), 4 * rc.size(), hipMemcpyHostToDevice) || hipMemcpy(s->dx, dx.data(), 4 * dx.size(), hipMemcpyHostToDevice) ||
       hipMemcpy(s->dr, dr.data(), 4 * dr.size(), hipMemcpyHostToDevice) || hipMemcpy(s->rf, rf.data(), 4 * rf.size(), hipMemcpyHostToDevice) ||
       hipMemset(s->src, 0, 4 * N) || hipMemset(s->mfx, 0, 4 * N) || hipMemset(s->dt, 0, 4 * N))
     return bail(afail(s, -1001, "axi_create: upload failed"));
+  {  // the host grid's double centres and widths (buildCFDGrid, src/cfd_grid.js:36-41)
+    std::vector<double> rcD(s->H), drD(s->H), dxD(s->W);
+    for (int i = 0; i < s->W; i++) dxD[i] = g->xf[i + 1] - g->xf[i];
+    for (int j = 0; j < s->H; j++) { rcD[j] = 0.5 * (g->rf[j] + g->rf[j + 1]); drD[j] = g->rf[j + 1] - g->rf[j]; }
+    if (hipMemcpy(s->rcD, rcD.data(), 8 * rcD.size(), hipMemcpyHostToDevice) || hipMemcpy(s->drD, drD.data(), 8 * drD.size(), hipMemcpyHostToDevice) ||
+        hipMemcpy(s->dxD, dxD.data(), 8 * dxD.size(), hipMemcpyHostToDevice))
+      return bail(afail(s, -1001, "axi_create: upload failed"));
+  }
   int rc_;
   if ((rc_ = upload_zones(s, z))) return bail(rc_);
   if ((rc_ = axi_set_state(s, Q0))) return bail(rc_);
@@ -190,7 +211,8 @@ int axi_destroy(axi_solver *s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   if (s->exec) (void)hipGraphExecDestroy(s->exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
-  void *p[] = {s->rc, s->dx, s->dr, s->rf, s->solid, s->comp, s->zf, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, s->partial, s->cap, s->dc};
+  void *p[] = {s->rc, s->dx, s->dr, s->rf, s->solid, s->comp, s->zf, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, s->partial, s->cap, s->dc,
+               s->rcD, s->drD, s->dxD, s->rsnap, s->sec_col, s->sec_rows, s->mon_part, s->mon_out, s->mon_in};
   for (void *x : p) if (x) (void)hipFree(x);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -230,6 +252,7 @@ int axi_set_state(axi_solver *s, const double *Q) {
   for (size_t k = 0; k < f.size(); k++) f[k] = (float)Q[k];
   ACHK(s, hipStreamSynchronize(s->stream));
   ACHK(s, hipMemcpy(s->Q, f.data(), 4 * f.size(), hipMemcpyHostToDevice));
+  s->snap_valid = false;          // the next monitor has no previous readback
   return 0;
 }
 int axi_step(axi_solver *s, int n) {
@@ -251,6 +274,48 @@ int axi_read(axi_solver *s, float *Q, float *src, float *mfx, float *dt) {
   ACHK(s, hipStreamSynchronize(s->stream));
   return 0;
 }
+int axi_monitor_sections(axi_solver *s, int nsec, const int *col, const uint8_t *rows) {
+  if (!s || nsec < 0 || nsec > kMonMaxSec || (nsec > 0 && (!col || !rows))) return afail(s, -1, "monitor_sections: bad argument");
+  for (int q = 0; q < nsec; q++)
+    if (col[q] < 0 || col[q] >= s->W) return afail(s, -1, "monitor_sections: column out of range");
+  ACHK(s, hipStreamSynchronize(s->stream));
+  if (nsec > 0) {
+    ACHK(s, hipMemcpy(s->sec_col, col, sizeof(int) * nsec, hipMemcpyHostToDevice));
+    ACHK(s, hipMemcpy(s->sec_rows, rows, (size_t)nsec * s->H, hipMemcpyHostToDevice));
+  }
+  s->nsec = nsec;
+  return 0;
+}
+
+int axi_monitor(axi_solver *s, const double comps[32], double P0, axi_monitor_out *out) {
+  if (!s || !comps || !out) return afail(s, -1, "monitor: null argument");
+  MonIn in;
+  std::memcpy(in.comps, comps, sizeof in.comps);
+  in.P0 = P0;
+  in.prev_valid = s->snap_valid ? 1 : 0;
+  ACHK(s, hipMemcpyAsync(s->mon_in, &in, sizeof in, hipMemcpyHostToDevice, s->stream));
+  const AxiView v = view(s);
+  const int nb = std::max(1, std::min(kMonBlocks, (s->N + kB - 1) / kB));
+  monitor_kernel<<<nb, kB, 0, s->stream>>>(v, s->rcD, s->drD, s->dxD, s->Q, s->src, s->mfx, s->rsnap, s->mon_in, s->mon_part);
+  monitor_finish<<<1, 64, 0, s->stream>>>(s->mon_part, nb, s->mon_out);
+  if (s->nsec > 0) section_kernel<<<1, 64, 0, s->stream>>>(v, s->rcD, s->drD, s->Q, s->mfx, s->nsec, s->sec_col, s->sec_rows, s->mon_out);
+  ACHK(s, hipGetLastError());
+  double o[kMonK + 3 * kMonMaxSec];
+  ACHK(s, hipMemcpyAsync(o, s->mon_out, sizeof(double) * (kMonK + 3 * s->nsec), hipMemcpyDeviceToHost, s->stream));
+  ACHK(s, hipMemcpyAsync(s->rsnap, s->Q, 4 * (size_t)s->N, hipMemcpyDeviceToDevice, s->stream));   // density of this readback
+  ACHK(s, hipStreamSynchronize(s->stream));
+  s->snap_valid = true;
+  std::memset(out, 0, sizeof *out);
+  out->Fb = o[0]; out->Fw = o[1]; out->F = o[0] + o[1]; out->mfuel = o[2];
+  for (int q = 0; q < 8; q++) out->pw[q] = o[3 + q];
+  out->res_sum2 = o[11]; out->res_n = o[12]; out->res_valid = in.prev_valid;
+  out->nsec = s->nsec;
+  for (int q = 0; q < s->nsec; q++) {
+    out->sec_mdot[q] = o[kMonK + 3 * q]; out->sec_w[q] = o[kMonK + 3 * q + 1]; out->sec_wTt[q] = o[kMonK + 3 * q + 2];
+  }
+  return 0;
+}
+
 int axi_sync(axi_solver *s) {
   if (!s) return -1;
   ACHK(s, hipStreamSynchronize(s->stream));

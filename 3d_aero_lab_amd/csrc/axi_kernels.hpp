@@ -275,4 +275,133 @@ __global__ void dtcap_kernel(const float *__restrict__ partial, int nb, float fa
   *cap = a * factor;
 }
 
+
+// ---------------------------------------------------------------------------
+// Readback monitors in FP64: what the application's convergence loop reads
+// back after each chunk of iterations (src/app_cfd.js:361-404).
+//   monitor_kernel  thrust parts of cfdForces with the solver's own sources
+//                   (src/cfd_post.js:44-75) and the density residual of the
+//                   readback (src/app_cfd.js:368-371): block partial sums.
+//   section_kernel  mass flow and mass-weighted Tt sums of cfdSection
+//                   (src/cfd_post.js:83-102) at the control stations, one
+//                   thread per section, rows in the host's order.
+// Per-cell terms are the host's double operations in the host's order
+// (contraction off, NaN-propagating max / min / clamp as in JavaScript); only
+// the order of the grid-wide sums differs from the host's sequential loop.
+// ---------------------------------------------------------------------------
+constexpr int kMonK = 13;         // Fb, Fw, mfuel, pw[0..7], sum of d^2, count
+constexpr int kMonMaxSec = 4;
+constexpr int kMonBlocks = 512;
+constexpr double kPi = 3.141592653589793;
+constexpr double kRGas = 287.05;
+
+struct MonIn {
+  double comps[32];
+  double P0;
+  int prev_valid;                 // the density snapshot holds the previous readback
+};
+
+__device__ __forceinline__ double jmax(double a, double b) { return (a != a || b != b) ? (a != a ? a : b) : (a > b ? a : b); }
+__device__ __forceinline__ double jmin(double a, double b) { return (a != a || b != b) ? (a != a ? a : b) : (a < b ? a : b); }
+__device__ __forceinline__ double jclamp(double v, double a, double b) { return v < a ? a : v > b ? b : v; }
+
+struct PrimD { double r, u, v, p, Y, g; };
+// cfdPrim (src/cfd_post.js:4-8) on the FP32 state, in doubles
+__device__ __forceinline__ PrimD prim_d(const float *__restrict__ Q, int N, int k) {
+#pragma clang fp contract(off)
+  PrimD q;
+  q.r = jmax((double)Q[k], 1e-4);
+  q.u = (double)Q[N + k] / q.r;
+  q.v = (double)Q[2 * N + k] / q.r;
+  q.Y = jclamp((double)Q[4 * N + k] / q.r, 0.0, 1.0);
+  q.g = 1.4 - 0.07 * q.Y;
+  q.p = jmax(1.0, (q.g - 1.0) * ((double)Q[3 * N + k] - 0.5 * q.r * (q.u * q.u + q.v * q.v)));
+  return q;
+}
+
+__global__ void __launch_bounds__(kB) monitor_kernel(AxiView v, const double *__restrict__ rcD, const double *__restrict__ drD,
+                                                     const double *__restrict__ dxD, const float *__restrict__ Q,
+                                                     const float *__restrict__ src, const float *__restrict__ mfx,
+                                                     const float *__restrict__ prev, const MonIn *__restrict__ in,
+                                                     double *__restrict__ partials) {
+#pragma clang fp contract(off)
+  __shared__ double lds[kB / 64][kMonK];
+  double acc[kMonK];
+#pragma unroll
+  for (int c = 0; c < kMonK; c++) acc[c] = 0.0;
+  const int N = v.N, W = v.W;
+  const bool pv = in->prev_valid != 0;
+  const double P0 = in->P0;
+  for (int k = blockIdx.x * kB + threadIdx.x; k < N; k += gridDim.x * kB) {
+    if (v.solid[k]) continue;
+    const int i = k % W, j = k / W;
+    const double A = 2.0 * kPi * rcD[j] * drD[j], V = A * dxD[i] * (double)v.phi[k];
+    const int id = v.comp[k];
+    if (id) {
+      const double ty = in->comps[id * 4 + 2];
+      if (ty == 1.0 || ty == 2.0) {
+        const PrimD q = prim_d(Q, N, k);
+        const double e = in->comps[id * 4 + 1], mf = (double)mfx[k] / (double)v.phi[k];
+        const double w = fabs(mf) > 1e-9 ? (double)src[k] / mf : 0.0;
+        acc[0] += (ty == 1.0 ? e * q.r * w : q.r * w / e) * V;
+        acc[3 + id] += (double)src[k] * V;
+      } else if (v.rate[k] > 0.f) {
+        acc[2] += (double)src[k] * V;
+      }
+    }
+    const double aw = i > 0 ? (double)v.ae[k - 1] : 1.0, dA = (double)v.ae[k] - aw;
+    if (dA != 0.0) acc[1] += (prim_d(Q, N, k).p - P0) * dA * A;
+    if (pv && (k & 1) == 0) {
+      const double d = ((double)Q[k] - (double)prev[k]) / (double)prev[k];
+      if (d == d) { acc[11] += d * d; acc[12] += 1.0; }
+    }
+  }
+  // block sum: lanes, then the four waves in order
+#pragma unroll
+  for (int c = 0; c < kMonK; c++)
+    for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_down(acc[c], off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < kMonK; c++) lds[wave][c] = acc[c];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int c = 0; c < kMonK; c++) {
+      double a = 0.0;
+      for (int w = 0; w < kB / 64; w++) a += lds[w][c];
+      partials[blockIdx.x * kMonK + c] = a;
+    }
+}
+
+// partials of nb blocks -> out[0..kMonK), block order
+__global__ void monitor_finish(const double *__restrict__ partials, int nb, double *__restrict__ out) {
+  const int c = threadIdx.x;
+  if (c >= kMonK) return;
+  double a = 0.0;
+  for (int b = 0; b < nb; b++) a += partials[b * kMonK + c];
+  out[c] = a;
+}
+
+// out[kMonK + 3 s + {0, 1, 2}] = mass flow, sum of max(mf, 0), sum of max(mf, 0) Tt at column col[s] over rows[s H + j] != 0
+__global__ void section_kernel(AxiView v, const double *__restrict__ rcD, const double *__restrict__ drD, const float *__restrict__ Q,
+                               const float *__restrict__ mfx, int nsec, const int *__restrict__ col, const uint8_t *__restrict__ rows,
+                               double *__restrict__ out) {
+#pragma clang fp contract(off)
+  const int s = threadIdx.x;
+  if (s >= nsec) return;
+  const int W = v.W, H = v.H, N = v.N, i = col[s];
+  double m = 0.0, w = 0.0, wTt = 0.0;
+  for (int j = 0; j < H; j++) {
+    if (!rows[s * H + j]) continue;
+    const int k = j * W + i;
+    const PrimD q = prim_d(Q, N, k);
+    const double T = q.p / (q.r * kRGas), cp = q.g * kRGas / (q.g - 1.0), V2 = q.u * q.u + q.v * q.v, Tt = T + V2 / (2.0 * cp);
+    const double mf = (double)mfx[k] * 2.0 * kPi * rcD[j] * drD[j], wk = jmax(mf, 0.0);
+    m += mf; w += wk; wTt += wk * Tt;
+  }
+  out[kMonK + 3 * s] = m;
+  out[kMonK + 3 * s + 1] = w;
+  out[kMonK + 3 * s + 2] = wTt;
+}
+
 }  // namespace axik

@@ -10,11 +10,15 @@
  * moves to the target grid (interpState, control gains carried) once the
  * coarse grid has converged after 2500 iterations, or after 9000.
  *
- *   node axi_seq.js <coarse case dir> <target case dir> [every] [cap]   (sequenced)
- *   node axi_seq.js - <target case dir> [every] [cap]                   (target grid only)
+ *   node axi_seq.js <coarse case dir> <target case dir> [every] [cap] [host]   (sequenced)
+ *   node axi_seq.js - <target case dir> [every] [cap] [host]                   (target grid only)
  * Case directories as written by tests/golden/make_golden_axi.js (setup arrays
  * and initial state). Prints one JSON line: per-grid iterations, residual and
  * thrust histories, final thrust, wall time.
+ * By default each readback is the device monitor (AxiHIP.monitor: thrust,
+ * blade-row powers, residual sums and the control sections in FP64, 28
+ * numbers back); `host` reads the whole state back and computes the same
+ * quantities with cfdForces / cfdSection on the host, as the application does.
  * ========================================================================== */
 const P = require('./cfd_post.js');
 const { interpState } = require('./cfd_grid.js');
@@ -34,23 +38,36 @@ function residual(zones, Q0, prev, dIt) {
 }
 
 /* Runs one grid until the reference's criteria; returns the final state. */
-function runGrid(C, comps, Q0, seq, every, cap, log) {
+function runGrid(C, comps, Q0, seq, every, cap, log, host) {
   const s = C.s;
   const S = new AxiHIP(C.grid, C.zones, comps, s.fs, Q0, Object.assign({}, s.opt, { hf: s.hf }));
   const minIt = seq ? 2500 : 3000;
   let it = 0, prev = null, res = NaN, Q = null, f = null;
   const lastF = [];
+  if (!host) S.monitorSections(P.controlStations(s.ctl).map(st => P.sectionRows(C.grid, C.zones, st.x, st.si)));
   const t0 = process.hrtime.bigint();
   for (;;) {
     S.step(every); it += every;
-    Q = S.read(Q);
-    if (prev) res = residual(C.zones, Q[0], prev, every);
-    prev = Float32Array.from(Q[0]);
-    f = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, s.hf, { S: S.src, F: S.mfx });
-    if (log.itTotal + it > 1500 && it > 300) {
-      const f0 = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, 0, { S: S.src, F: S.mfx });
-      P.cfdControl(s.type || 'tj', s.ctl, C.grid, C.zones, Q, S.mfx, comps, f0.pw, s.ctl.etam, every);
-      S.setComps(comps);
+    const gate = log.itTotal + it > 1500 && it > 300;
+    if (host) {
+      Q = S.read(Q);
+      if (prev) res = residual(C.zones, Q[0], prev, every);
+      prev = Float32Array.from(Q[0]);
+      f = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, s.hf, { S: S.src, F: S.mfx });
+      if (gate) {
+        const f0 = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, 0, { S: S.src, F: S.mfx });
+        P.cfdControl(s.type || 'tj', s.ctl, C.grid, C.zones, Q, S.mfx, comps, f0.pw, s.ctl.etam, every);
+        S.setComps(comps);
+      }
+    } else {
+      // with the solver's sources the forces do not depend on the heating value
+      const m = S.monitor(comps, s.ctl.P0);
+      if (m.resValid) res = Math.sqrt(m.resSum2 / Math.max(1, m.resN)) / every;
+      f = m;
+      if (gate) {
+        P.controlFromSections(s.type || 'tj', s.ctl, st => P.sectionFromSums(m.sec[st.id === 4 ? 0 : 1]), comps, m.pw, s.ctl.etam, every);
+        S.setComps(comps);
+      }
     }
     log.hist.push({ grid: C.grid.W + 'x' + C.grid.H, it: log.itTotal + it, res, F: f.F });
     lastF.push([it, f.F]);
@@ -64,13 +81,14 @@ function runGrid(C, comps, Q0, seq, every, cap, log) {
     if (!seq && it >= cap) break;
   }
   const wall = Number(process.hrtime.bigint() - t0) / 1e9;
+  if (!host) Q = S.read(Q);                       // the final state, once
   S.dispose();
   log.itTotal += it;
   log.grids.push({ W: C.grid.W, H: C.grid.H, iters: it, res, F: f.F, wall_s: wall });
   return Q;
 }
 
-function run(coarseDir, targetDir, every, cap) {
+function run(coarseDir, targetDir, every, cap, host) {
   const log = { itTotal: 0, hist: [], grids: [] };
   const T = loadCase(targetDir);
   const comps = Float64Array.from(T.comps);
@@ -78,19 +96,19 @@ function run(coarseDir, targetDir, every, cap) {
   const t0 = process.hrtime.bigint();
   if (coarseDir && coarseDir !== '-') {
     const Cc = loadCase(coarseDir), cc = Float64Array.from(Cc.comps);
-    const Qc = runGrid(Cc, cc, Cc.Q0, true, every, cap, log);
+    const Qc = runGrid(Cc, cc, Cc.Q0, true, every, cap, log, host);
     const Q64 = [0, 1, 2, 3, 4].map(q => Float64Array.from(Qc[q]));
     Q0 = interpState(Cc.grid, Cc.zones, Q64, T.grid, T.zones, T.s.fl);
     comps[17] = cc[17]; comps[29] = cc[29];                 // control gains carried (src/app_cfd.js:281)
   }
-  runGrid(T, comps, Q0, false, every, cap, log);
+  runGrid(T, comps, Q0, false, every, cap, log, host);
   const wall = Number(process.hrtime.bigint() - t0) / 1e9;
   const last = log.grids[log.grids.length - 1];
-  return { grids: log.grids, iters: log.itTotal, F: last.F, res: last.res, wall_s: wall, hist: log.hist };
+  return { mode: host ? 'host' : 'device', grids: log.grids, iters: log.itTotal, F: last.F, res: last.res, wall_s: wall, hist: log.hist };
 }
 
 if (require.main === module) {
-  const [coarse, target, every, cap] = process.argv.slice(2);
-  console.log(JSON.stringify(run(coarse, target, +(every || 250), +(cap || 60000))));
+  const [coarse, target, every, cap, mode] = process.argv.slice(2);
+  console.log(JSON.stringify(run(coarse, target, +(every || 250), +(cap || 60000), mode === 'host')));
 }
 module.exports = { run, residual };

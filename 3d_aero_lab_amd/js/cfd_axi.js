@@ -58,6 +58,25 @@ class AxiHIP {
   }
   finish() { loadAddon().axiSync(this.h); }
   dispose() { if (this.h) { loadAddon().axiDispose(this.h); this.h = null; } }
+  /* Sections the device monitor sums (at most 4): [{col, rows: Uint8Array(H)}]
+   * as cfd_post.js sectionRows builds them. */
+  monitorSections(list) {
+    const cols = Int32Array.from(list.map(x => x.col)), rows = new Uint8Array(list.length * this.H);
+    list.forEach((x, q) => rows.set(x.rows, q * this.H));
+    loadAddon().axiMonitorSections(this.h, cols, rows);
+    this.nsec = list.length;
+  }
+  /* The readback of the convergence loop on the device (FP64): thrust parts
+   * and blade-row powers of cfdForces with the solver's own sources, the
+   * density residual sums against the previous monitor call, and the
+   * section sums {mdot, w, wTt}. Waits for the GPU; reads back 28 numbers. */
+  monitor(comps, P0) {
+    const o = this._mon || (this._mon = new Float64Array(28));
+    loadAddon().axiMonitor(this.h, comps instanceof Float64Array ? comps : Float64Array.from(comps), P0, o);
+    const sec = [];
+    for (let q = 0; q < o[15]; q++) sec.push({ mdot: o[16 + 3 * q], w: o[17 + 3 * q], wTt: o[18 + 3 * q] });
+    return { F: o[0], Fb: o[1], Fw: o[2], mfuel: o[3], pw: o.slice(4, 12), resSum2: o[12], resN: o[13], resValid: o[14] !== 0, sec };
+  }
 }
 
 /* Checkpoint of a long run in the reference's format (save and setup's load,

@@ -96,18 +96,45 @@ function spoolRatios(type, pw, etam) {
  * iterations. ctl = {s, comb: {x1, stream}, ab: {x1, stream} | null, order,
  * Tt4, Tt7, abOn} — the spec / operating-point fields the reference reads. */
 function cfdControl(type, ctl, grid, zones, Q, mfx, comps, pw, etam, dIt) {
-  const lim = 0.03 * Math.min(1, (dIt || 100) / 100);
-  const out = { ratios: spoolRatios(type, pw, etam), Tt4: NaN, Tt7: NaN };
-  const step = (id, x, si, target) => {
-    const sec = cfdSection(grid, zones, Q, x, si, mfx);
-    const T0 = comps[id * 4], K = comps[id * 4 + 1] || 1;
-    if (sec.mdot > 0 && sec.Tt > T0 + 20 && target > T0 + 20) comps[id * 4 + 1] = clamp(K * clamp(Math.pow((target - T0) / (sec.Tt - T0), 0.3), 1 - lim, 1 + lim), 0.5, 2.5);
-    return sec.Tt;
-  };
-  out.Tt4 = step(4, ctl.comb.x1 - 0.004 * ctl.s, ctl.order.indexOf(ctl.comb.stream), ctl.Tt4);
-  if (ctl.ab && ctl.abOn) out.Tt7 = step(7, ctl.ab.x1 - 0.004 * ctl.s, ctl.order.indexOf(ctl.ab.stream), ctl.Tt7);
+  return controlFromSections(type, ctl, st => cfdSection(grid, zones, Q, st.x, st.si, mfx), comps, pw, etam, dIt);
+}
+
+/* The control stations of cfdControl: heat source id, section x and stream,
+ * target exit temperature (combustor; afterburner when lit). */
+function controlStations(ctl) {
+  const out = [{ id: 4, x: ctl.comb.x1 - 0.004 * ctl.s, si: ctl.order.indexOf(ctl.comb.stream), target: ctl.Tt4 }];
+  if (ctl.ab && ctl.abOn) out.push({ id: 7, x: ctl.ab.x1 - 0.004 * ctl.s, si: ctl.order.indexOf(ctl.ab.stream), target: ctl.Tt7 });
   return out;
 }
+
+/* cfdControl's gain update with the section of each station from secOf(st)
+ * ({mdot, Tt}: cfdSection on the host, or the device monitor's sums). */
+function controlFromSections(type, ctl, secOf, comps, pw, etam, dIt) {
+  const lim = 0.03 * Math.min(1, (dIt || 100) / 100);
+  const out = { ratios: spoolRatios(type, pw, etam), Tt4: NaN, Tt7: NaN };
+  for (const st of controlStations(ctl)) {
+    const sec = secOf(st), id = st.id, target = st.target;
+    const T0 = comps[id * 4], K = comps[id * 4 + 1] || 1;
+    if (sec.mdot > 0 && sec.Tt > T0 + 20 && target > T0 + 20) comps[id * 4 + 1] = clamp(K * clamp(Math.pow((target - T0) / (sec.Tt - T0), 0.3), 1 - lim, 1 + lim), 0.5, 2.5);
+    out[id === 4 ? 'Tt4' : 'Tt7'] = sec.Tt;
+  }
+  return out;
+}
+
+/* Column and row mask of cfdSection at x over stream si (null: all fluid),
+ * for the device monitor. */
+function sectionRows(grid, zones, x, si) {
+  const { W, H } = grid, i = colAt(grid, x), rows = new Uint8Array(H);
+  for (let j = 0; j < H; j++) {
+    const k = j * W + i;
+    rows[j] = !zones.solid[k] && (si === null || zones.stream[k] === si) ? 1 : 0;
+  }
+  return { col: i, rows };
+}
+
+/* {mdot, Tt} of a section from the device monitor's sums (cfdSection's
+ * division by the positive mass flow, 1 when there is none). */
+const sectionFromSums = s => ({ mdot: s.mdot, Tt: s.wTt / (s.w || 1) });
 
 /* Display fields of an axisymmetric state (cfdDerived, src/cfd_post.js:10-40):
  * region (0 exterior, 2 + stream), material, T, Tt, P, Pt, |V|, M, rho, Vx,
@@ -183,4 +210,5 @@ function stations(grid, zones, Q, mfx, order, list) {
   return out;
 }
 
-module.exports = { RGAS, clamp, cfdPrim, cfdDerived, DERIVED_KEYS, cfdForces, colAt, cfdSection, spoolRatios, cfdControl, stations, STATIONS, TJ_STATIONS };
+module.exports = { RGAS, clamp, cfdPrim, cfdDerived, DERIVED_KEYS, cfdForces, colAt, cfdSection, spoolRatios, cfdControl, controlStations,
+  controlFromSections, sectionRows, sectionFromSums, stations, STATIONS, TJ_STATIONS };

@@ -58,6 +58,31 @@ int axi_sync(axi_solver *s);
 /* n iterations bracketed by HIP events on the solver's stream. */
 int axi_time_steps(axi_solver *s, int n, float *ms);
 
+/* Readback monitors of the application's convergence loop on the device, in
+ * FP64 (src/app_cfd.js:361-404): what the loop otherwise computes on the host
+ * from a full read() after each chunk of iterations.
+ *   F, Fb, Fw, mfuel, pw  cfdForces with the solver's own sources
+ *                         (src/cfd_post.js:44-75; F = Fb + Fw)
+ *   res_sum2, res_n       sum of ((rho - rho_prev) / rho_prev)^2 and the count
+ *                         over every second non-solid cell, against the state
+ *                         of the previous axi_monitor call (src/app_cfd.js:368-371);
+ *                         res_valid = 0 on the first call and after setState
+ *   sec_*                 per section of axi_monitor_sections: mass flow, sum
+ *                         of max(mf, 0) and of max(mf, 0) Tt (cfdSection with
+ *                         the solver's mass flux, src/cfd_post.js:83-102)
+ * Per-cell terms are the host's double operations; the grid sums differ from
+ * the host's sequential ones only in their order. Waits. */
+typedef struct {
+  double F, Fb, Fw, mfuel, pw[8];
+  double res_sum2, res_n;
+  int res_valid, nsec;
+  double sec_mdot[4], sec_w[4], sec_wTt[4];
+} axi_monitor_out;
+int axi_monitor(axi_solver *s, const double comps[32], double P0, axi_monitor_out *out);
+/* Sections for axi_monitor (at most 4): column col[q] and row mask rows[q*H + j]
+ * (1 where cfdSection counts row j: non-solid and in the section's stream). */
+int axi_monitor_sections(axi_solver *s, int nsec, const int *col, const uint8_t *rows);
+
 #ifdef __cplusplus
 }
 #endif
