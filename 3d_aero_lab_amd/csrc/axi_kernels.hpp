@@ -10,6 +10,13 @@
 
 namespace axik {
 
+// No multiply-add contraction in any axisymmetric kernel: each result is the
+// rounded product / sum the source states, whatever the code's layout (the
+// HIP default fuses across statements, and which pairs fuse changes with how
+// the code is arranged). Against the oracle this keeps TJ, HBTF and LBTF
+// within 7.7e-6 per variable after 100 iterations.
+#pragma clang fp contract(off)
+
 constexpr int kB = 256;
 constexpr float kR = 287.05f;
 
@@ -84,19 +91,32 @@ __device__ __forceinline__ void sel(const Prim &P, int dir, float *o) {
   o[0] = P.r; o[1] = dir ? P.v : P.u; o[2] = dir ? P.u : P.v; o[3] = P.p; o[4] = P.Y;
 }
 
-// MUSCL face states (recon, src/cfd_cpu.js:86-94)
-__device__ void recon(const Prim &A, const Prim *Am, const Prim &B, const Prim *Bp, int dir, bool second, float *L, float *R) {
-  float A5[5], B5[5], T5[5];
-  sel(A, dir, A5); sel(B, dir, B5);
-  for (int q = 0; q < 5; q++) { L[q] = A5[q]; R[q] = B5[q]; }
+// MUSCL face states (recon, src/cfd_cpu.js:86-94). hm / hp: the outer cell on
+// that side takes part (both full cells and the face inside the grid); the
+// outer primitives are always passed in, and every choice is a value select,
+// so the caller's loads of them are issued with all the others.
+__device__ __forceinline__ void recon(const Prim &A, const Prim &Am, bool hm, const Prim &B, const Prim &Bp, bool hp, int dir,
+                                      bool second, float *L, float *R) {
+  float A5[5], B5[5], M5[5], P5[5];
+  sel(A, dir, A5); sel(B, dir, B5); sel(Am, dir, M5); sel(Bp, dir, P5);
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const float l = A5[q] + 0.5f * minmod(A5[q] - M5[q], B5[q] - A5[q]);
+    const float r = B5[q] - 0.5f * minmod(B5[q] - A5[q], P5[q] - B5[q]);
+    L[q] = second && hm ? l : A5[q];
+    R[q] = second && hp ? r : B5[q];
+  }
   if (!second) return;
-  if (Am) { sel(*Am, dir, T5); for (int q = 0; q < 5; q++) L[q] = A5[q] + 0.5f * minmod(A5[q] - T5[q], B5[q] - A5[q]); }
-  if (Bp) { sel(*Bp, dir, T5); for (int q = 0; q < 5; q++) R[q] = B5[q] - 0.5f * minmod(B5[q] - A5[q], T5[q] - B5[q]); }
-  if (L[0] <= 0.f || L[3] <= 0.f) for (int q = 0; q < 5; q++) L[q] = A5[q];
-  if (R[0] <= 0.f || R[3] <= 0.f) for (int q = 0; q < 5; q++) R[q] = B5[q];
+  const bool badL = L[0] <= 0.f || L[3] <= 0.f, badR = R[0] <= 0.f || R[3] <= 0.f;
+#pragma unroll
+  for (int q = 0; q < 5; q++) { L[q] = badL ? A5[q] : L[q]; R[q] = badR ? B5[q] : R[q]; }
 }
 
-// One thread per face (src/cfd_cpu.js:97-143).
+// One thread per face (src/cfd_cpu.js:97-143). Every load of a face (the four
+// cells' states and volume fractions, the face's aperture, the solid flags) is
+// issued before any of them is tested, at indices clamped into the grid: one
+// memory round trip per face instead of one per decision. Closed faces compute
+// a flux too and store 0 through a select.
 __global__ void __launch_bounds__(kB) face_kernel(AxiView v, const AxiConsts *__restrict__ cp, const float *__restrict__ Q, float *__restrict__ Fx,
                                                   float *__restrict__ Fr, float *__restrict__ nutF) {
   const AxiConsts &c = *cp;
@@ -105,57 +125,56 @@ __global__ void __launch_bounds__(kB) face_kernel(AxiView v, const AxiConsts *__
   const int t = blockIdx.x * kB + threadIdx.x;
   const bool second = c.order == 2;
   float F[5], G[5], L[5], R[5];
-  auto full = [&](int k) { return v.phi[k] > 0.999f; };
   if (t < NXF) {
-    const int j = t / (W + 1), f = t % (W + 1), a = f - 1, b = f, ka = j * W + a, kb = j * W + b;
-    float al;
-    if (a < 0) al = v.solid[kb] ? 0.f : 1.f; else if (b >= W) al = v.solid[ka] ? 0.f : 1.f; else al = v.ae[ka];
-    if (al <= 0.f) { for (int q = 0; q < 5; q++) Fx[q * NXF + t] = 0.f; return; }
-    if (a < 0) {
-      const Prim Pb = prim(Q, N, kb);
-      farfield(c, Pb, -1.f, 0.f, G);
-      hllc(G[0], G[1], G[2], G[3], G[4], 1.4f - 0.07f * G[4], Pb.r, Pb.u, Pb.v, Pb.p, Pb.Y, Pb.g, F);
-    } else if (b >= W) {
-      const Prim Pa = prim(Q, N, ka);
-      farfield(c, Pa, 1.f, 0.f, G);
-      hllc(Pa.r, Pa.u, Pa.v, Pa.p, Pa.Y, Pa.g, G[0], G[1], G[2], G[3], G[4], 1.4f - 0.07f * G[4], F);
+    const int j = t / (W + 1), f = t % (W + 1), a = f - 1, b = f;
+    // cells a-1, a, b, b+1 of row j, clamped into the row
+    const int ia = a < 0 ? 0 : a, ib = b >= W ? W - 1 : b, im = ia > 0 ? ia - 1 : 0, ip = ib + 1 < W ? ib + 1 : W - 1;
+    const int ka = j * W + ia, kb = j * W + ib, km = j * W + im, kp = j * W + ip;
+    const uint8_t sa = v.solid[ka], sb = v.solid[kb];
+    const float aea = v.ae[ka], pa = v.phi[ka], pb = v.phi[kb], pm = v.phi[km], pp = v.phi[kp];
+    const Prim Pa = prim(Q, N, ka), Pb = prim(Q, N, kb), Pm = prim(Q, N, km), Pp = prim(Q, N, kp);
+    const float al = a < 0 ? (sb ? 0.f : 1.f) : b >= W ? (sa ? 0.f : 1.f) : aea;
+    if (a < 0 || b >= W) {                 // farfield ghost at x = lo / x = hi (two faces per row)
+      const Prim &P = a < 0 ? Pb : Pa;
+      farfield(c, P, a < 0 ? -1.f : 1.f, 0.f, G);
+      if (a < 0) hllc(G[0], G[1], G[2], G[3], G[4], 1.4f - 0.07f * G[4], Pb.r, Pb.u, Pb.v, Pb.p, Pb.Y, Pb.g, F);
+      else hllc(Pa.r, Pa.u, Pa.v, Pa.p, Pa.Y, Pa.g, G[0], G[1], G[2], G[3], G[4], 1.4f - 0.07f * G[4], F);
     } else {
-      const Prim Pa = prim(Q, N, ka), Pb = prim(Q, N, kb);
-      const bool fa = full(ka) && full(kb);
-      Prim Pm, Pp;
-      const bool hm = second && fa && a - 1 >= 0 && full(ka - 1), hp = second && fa && b + 1 < W && full(kb + 1);
-      if (hm) Pm = prim(Q, N, ka - 1);
-      if (hp) Pp = prim(Q, N, kb + 1);
-      recon(Pa, hm ? &Pm : nullptr, Pb, hp ? &Pp : nullptr, 0, second, L, R);
+      const bool fa = pa > 0.999f && pb > 0.999f;
+      const bool hm = fa && a - 1 >= 0 && pm > 0.999f, hp = fa && b + 1 < W && pp > 0.999f;
+      recon(Pa, Pm, hm, Pb, Pp, hp, 0, second, L, R);
       hllc(L[0], L[1], L[2], L[3], L[4], 1.4f - 0.07f * L[4], R[0], R[1], R[2], R[3], R[4], 1.4f - 0.07f * R[4], F);
     }
-    for (int q = 0; q < 5; q++) Fx[q * NXF + t] = al * F[q];
+    const bool open = al > 0.f;
+#pragma unroll
+    for (int q = 0; q < 5; q++) Fx[q * NXF + t] = open ? al * F[q] : 0.f;
     return;
   }
   const int u = t - NXF;
   if (u >= NRF) return;
-  const int f = u / W, i = u % W, a = f - 1, b = f, ka = a * W + i, kb = b * W + i;
+  const int f = u / W, i = u % W, a = f - 1, b = f;
+  // cells a-1, a, b, b+1 of column i, clamped into the column
+  const int ja = a < 0 ? 0 : a, jb = b >= H ? H - 1 : b, jm = ja > 0 ? ja - 1 : 0, jp = jb + 1 < H ? jb + 1 : H - 1;
+  const int ka = ja * W + i, kb = jb * W + i, km = jm * W + i, kp = jp * W + i;
+  const uint8_t sa = v.solid[ka];
+  const float ana = v.an[ka], pa = v.phi[ka], pb = v.phi[kb], pm = v.phi[km], pp = v.phi[kp];
+  const float la = v.lmix[ka], lb = v.lmix[kb], rca = v.rc[ja], rcb = v.rc[jb];
+  const Prim Pa = prim(Q, N, ka), Pb = prim(Q, N, kb), Pm = prim(Q, N, km), Pp = prim(Q, N, kp);
   float G2[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
   float nut = 0.f;
-  const float al = a < 0 ? 0.f : b >= H ? (v.solid[ka] ? 0.f : 1.f) : v.an[ka];
-  if (al <= 0.f) { for (int q = 0; q < 5; q++) Fr[q * NRF + u] = 0.f; nutF[u] = 0.f; return; }
-  if (b >= H) {
-    const Prim Pa = prim(Q, N, ka);
+  const float al = a < 0 ? 0.f : b >= H ? (sa ? 0.f : 1.f) : ana;
+  if (b >= H) {                            // farfield ghost at r = top
     farfield(c, Pa, 0.f, 1.f, G);
     hllc(Pa.r, Pa.v, Pa.u, Pa.p, Pa.Y, Pa.g, G[0], G[2], G[1], G[3], G[4], 1.4f - 0.07f * G[4], F);
   } else {
-    const Prim Pa = prim(Q, N, ka), Pb = prim(Q, N, kb);
-    const bool fa = full(ka) && full(kb);
-    Prim Pm, Pp;
-    const bool hm = second && fa && a - 1 >= 0 && full(ka - W), hp = second && fa && b + 1 < H && full(kb + W);
-    if (hm) Pm = prim(Q, N, ka - W);
-    if (hp) Pp = prim(Q, N, kb + W);
-    recon(Pa, hm ? &Pm : nullptr, Pb, hp ? &Pp : nullptr, 1, second, L, R);
+    const bool fa = pa > 0.999f && pb > 0.999f;
+    const bool hm = fa && a - 1 >= 0 && pm > 0.999f, hp = fa && b + 1 < H && pp > 0.999f;
+    recon(Pa, Pm, hm, Pb, Pp, hp, 1, second, L, R);
     hllc(L[0], L[1], L[2], L[3], L[4], 1.4f - 0.07f * L[4], R[0], R[1], R[2], R[3], R[4], 1.4f - 0.07f * R[4], F);
     if (c.visc) {            // mixing-length shear, heat and species fluxes (src/cfd_cpu.js:129-140)
-      const float drc = v.rc[b] - v.rc[a];
+      const float drc = rcb - rca;
       const float dudr = (Pb.u - Pa.u) / drc;
-      const float l = 0.5f * (v.lmix[ka] + v.lmix[kb]);
+      const float l = 0.5f * (la + lb);
       nut = l * l * fabsf(dudr);
       const float rho = 0.5f * (Pa.r + Pb.r), mu = rho * nut;
       const float Ta = Pa.p / (Pa.r * kR), Tb = Pb.p / (Pb.r * kR);
@@ -166,18 +185,20 @@ __global__ void __launch_bounds__(kB) face_kernel(AxiView v, const AxiConsts *__
       G2[4] = mu / 0.9f * (Pb.Y - Pa.Y) / drc;
     }
   }
-  Fr[u] = al * F[0];
-  Fr[NRF + u] = al * (F[2] - G2[1]);
-  Fr[2 * NRF + u] = al * F[1];
-  Fr[3 * NRF + u] = al * (F[3] - G2[3]);
-  Fr[4 * NRF + u] = al * (F[4] - G2[4]);
-  nutF[u] = nut;
+  const bool open = al > 0.f;
+  Fr[u] = open ? al * F[0] : 0.f;
+  Fr[NRF + u] = open ? al * (F[2] - G2[1]) : 0.f;
+  Fr[2 * NRF + u] = open ? al * F[1] : 0.f;
+  Fr[3 * NRF + u] = open ? al * (F[3] - G2[3]) : 0.f;
+  Fr[4 * NRF + u] = open ? al * (F[4] - G2[4]) : 0.f;
+  nutF[u] = open ? nut : 0.f;
 }
 
-// One thread per cell (src/cfd_cpu.js:145-185, 187-207). mode 0: stage 1
-// (dt from the state, Qout = Qin + dt R); mode 1: stage 2 (Qout = (Qb + Qin +
-// dt R) / 2); mode 2: stage 1, dt only (for the global / ratio cap); mode 3:
-// stage 1 update with the stored dt capped at *cap.
+// One thread per cell (src/cfd_cpu.js:145-192). mode 0: stage 1 with dt,
+// 1: stage 2, 2: dt only, 3: stage 1 with dt capped at *cap. The cell's loads
+// (state, face fluxes, geometry, zone data, the stored dt) are issued before
+// the solid flag is tested; solid cells keep their state (modes 0 and 3 copy
+// it to the output) and write nothing else.
 __global__ void __launch_bounds__(kB) cell_kernel(AxiView v, const AxiConsts *__restrict__ cp, const float *__restrict__ Qin, const float *Qb,
                                                   float *Qout, const float *__restrict__ Fx, const float *__restrict__ Fr,
                                                   const float *__restrict__ nutF, float *__restrict__ dt, float *__restrict__ src,
@@ -188,60 +209,79 @@ __global__ void __launch_bounds__(kB) cell_kernel(AxiView v, const AxiConsts *__
   if (k >= N) return;
   const int NXF = (W + 1) * v.H, NRF = W * (v.H + 1);
   const int i = k % W, j = k / W;
-  if (v.solid[k]) {
+  const int fx0 = j * (W + 1) + i, fx1 = fx0 + 1, fr0 = j * W + i, fr1 = fr0 + W;
+  // every load of the cell
+  const bool sol = v.solid[k] != 0;
+  const float *Qbb = mode == 1 ? Qb : Qin;
+  float q0[5], qb[5], fxa[5], fxb[5], fra[5], frb[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    q0[q] = Qin[q * N + k]; qb[q] = Qbb[q * N + k];
+    fxa[q] = Fx[q * NXF + fx0]; fxb[q] = Fx[q * NXF + fx1]; fra[q] = Fr[q * NRF + fr0]; frb[q] = Fr[q * NRF + fr1];
+  }
+  const float dx = v.dx[i], dr = v.dr[j], ax0 = v.rf[j], ax1 = v.rf[j + 1], rcj = v.rc[j], ph = v.phi[k];
+  const float nu0 = nutF[j * W + i], nu1 = nutF[(j + 1) * W + i];
+  const float aek = v.ae[k], aem = v.ae[i > 0 ? k - 1 : k], ank = v.an[k], anm = v.an[j > 0 ? k - W : k];
+  const int id = v.comp[k];
+  const float mfd = v.mfd[k], wgt = v.wgt[k], tgtk = v.tgt[k], rate = v.rate[k], yrate = v.yrate[k];
+  const float dts = dt[k];
+  if (sol) {
     if (mode == 0 || mode == 3)
-      for (int q = 0; q < 5; q++) Qout[q * N + k] = Qin[q * N + k];
+#pragma unroll
+      for (int q = 0; q < 5; q++) Qout[q * N + k] = q0[q];
     return;
   }
-  const Prim P = prim(Qin, N, k);
-  const float dx = v.dx[i], dr = v.dr[j];
+  Prim P;
+  P.r = fmaxf(q0[0], 1e-4f);
+  P.u = q0[1] / P.r; P.v = q0[2] / P.r;
+  P.Y = clampf(q0[4] / P.r, 0.f, 1.f);
+  P.g = 1.4f - 0.07f * P.Y;
+  P.p = fmaxf(1.f, (P.g - 1.f) * (q0[3] - 0.5f * P.r * (P.u * P.u + P.v * P.v)));
+  float d = dts;
   if (mode == 0 || mode == 2) {
-    const float nu = fmaxf(nutF[j * W + i], nutF[(j + 1) * W + i]);
+    const float nu = fmaxf(nu0, nu1);
     const float cs = sqrtf(P.g * P.p / P.r);
-    dt[k] = c.cfl * fmaxf(v.phi[k], 0.25f) / ((fabsf(P.u) + cs) / dx + (fabsf(P.v) + cs) / dr + 2.2f * nu / (dr * dr));
+    d = c.cfl * fmaxf(ph, 0.25f) / ((fabsf(P.u) + cs) / dx + (fabsf(P.v) + cs) / dr + 2.2f * nu / (dr * dr));
+    dt[k] = d;
     if (mode == 2) return;
   }
-  const int fx0 = j * (W + 1) + i, fx1 = fx0 + 1, fr0 = j * W + i, fr1 = fr0 + W;
-  const float ax0 = v.rf[j], ax1 = v.rf[j + 1], vol = v.rc[j] * dr, ph = v.phi[k];
+  const float vol = rcj * dr;
   float R[5];
-  for (int q = 0; q < 5; q++)
-    R[q] = (-(Fx[q * NXF + fx1] - Fx[q * NXF + fx0]) / dx - (Fr[q * NRF + fr1] * ax1 - Fr[q * NRF + fr0] * ax0) / vol) / ph;
-  const float aw = i > 0 ? v.ae[k - 1] : 1.f, as = j > 0 ? v.an[k - W] : 0.f;
-  R[1] += P.p * (v.ae[k] - aw) / (dx * ph);
-  R[2] += P.p * (v.an[k] * ax1 - as * ax0) / (vol * ph);
-  const float mfx = 0.5f * (Fx[fx0] + Fx[fx1]);
+#pragma unroll
+  for (int q = 0; q < 5; q++) R[q] = (-(fxb[q] - fxa[q]) / dx - (frb[q] * ax1 - fra[q] * ax0) / vol) / ph;
+  const float aw = i > 0 ? aem : 1.f, as = j > 0 ? anm : 0.f;
+  R[1] += P.p * (aek - aw) / (dx * ph);
+  R[2] += P.p * (ank * ax1 - as * ax0) / (vol * ph);
+  const float mfx = 0.5f * (fxa[0] + fxb[0]);
   float sk = 0.f;
-  const int id = v.comp[k];
   if (id) {
-    const float rho = Qin[k], g = P.g, cp = g * kR / (g - 1.f);
+    const float rho = q0[0], g = P.g, cp = g * kR / (g - 1.f);
     const float T = P.p / (rho * kR), Tt = T + 0.5f * (P.u * P.u + P.v * P.v) / cp;
     const float ty = c.comps[id * 4 + 2], mf = mfx / ph;
     if (ty == 1.f || ty == 2.f) {     // blade row: tau^(wgt dx) on the upstream face enthalpy
-      const float e = c.comps[id * 4 + 1], fac = ty == 1.f ? 1.f : clampf(mf / (0.3f * v.mfd[k]), 0.f, 1.f);
-      const float mw = Fx[fx0], HU = mw > 1e-9f ? Fx[3 * NXF + fx0] / mw : cp * Tt;
-      const float w = (HU > 0.f ? HU : cp * Tt) * expm1f(c.comps[id * 4] * v.wgt[k] * fac * dx) / dx;
+      const float e = c.comps[id * 4 + 1], fac = ty == 1.f ? 1.f : clampf(mf / (0.3f * mfd), 0.f, 1.f);
+      const float mw = fxa[0], HU = mw > 1e-9f ? fxa[3] / mw : cp * Tt;
+      const float w = (HU > 0.f ? HU : cp * Tt) * expm1f(c.comps[id * 4] * wgt * fac * dx) / dx;
       R[1] += ty == 1.f ? e * rho * w : rho * w / e;
       R[3] += mf * w;
       sk = mf * w;
-    } else if (v.rate[k] > 0.f) {      // heat source relaxed to the target total temperature
+    } else if (rate > 0.f) {           // heat source relaxed to the target total temperature
       const float K = c.comps[id * 4 + 1] != 0.f ? c.comps[id * 4 + 1] : 1.f;
-      const float tgt = c.comps[id * 4] + (v.tgt[k] - c.comps[id * 4]) * K;
+      const float tgt = c.comps[id * 4] + (tgtk - c.comps[id * 4]) * K;
       const float m0 = fmaxf(mf, 0.f);
-      const float q = m0 * cp * fmaxf(0.f, tgt - Tt) * v.rate[k];
+      const float q = m0 * cp * fmaxf(0.f, tgt - Tt) * rate;
       R[3] += q;
       const float mfuel = q / c.hf;
       R[0] += mfuel;
       sk = mfuel;
-      R[4] += m0 * (1.f - P.Y) * v.yrate[k] + mfuel;
+      R[4] += m0 * (1.f - P.Y) * yrate + mfuel;
     }
   }
   src[k] = sk; mfxo[k] = mfx;
-  float d = dt[k];
-  if (mode == 3) d = fminf(d, *cap);
-  if (mode == 3) dt[k] = d;
+  if (mode == 3) { d = fminf(d, *cap); dt[k] = d; }
   float n[5];
-  if (mode == 1) for (int q = 0; q < 5; q++) n[q] = 0.5f * Qb[q * N + k] + 0.5f * (Qin[q * N + k] + d * R[q]);
-  else for (int q = 0; q < 5; q++) n[q] = Qin[q * N + k] + d * R[q];
+  if (mode == 1) for (int q = 0; q < 5; q++) n[q] = 0.5f * qb[q] + 0.5f * (q0[q] + d * R[q]);
+  else for (int q = 0; q < 5; q++) n[q] = q0[q] + d * R[q];
   // floors (CPUSolver.fix, src/cfd_cpu.js:187-192)
   const float r = fmaxf(n[0], 1e-3f);
   n[0] = r;
