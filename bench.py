@@ -297,8 +297,8 @@ def main():
                     help="run only a plain device copy of known size (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this build measures one GPU only; the solve would shard by z-slabs with a 2-cell halo "
-              "(see DESIGN.md) but no multi-GPU path is built", file=sys.stderr)
+        print("bench.py: one GPU only; a solve would shard by z-slabs with a 2-plane halo exchange per stage, "
+              "but no multi-GPU path is built", file=sys.stderr)
         sys.exit(2)
     if a.copy_probe:
         return copy_probe()
