@@ -20,14 +20,18 @@ def _faces(rng, n, lo):
     return np.concatenate([[0.0], np.cumsum(1.0 + 0.15 * rng.random(n))]) + lo
 
 
-def synth_case(NX, NY, NZ, seed=0):
-    """-> (grid, geo, F, fix) in the layouts the solver, the oracle and the batch take."""
+def synth_case(NX, NY, NZ, seed=0, body=True, quiescent=False):
+    """-> (grid, geo, F, fix) in the layouts the solver, the oracle and the batch take.
+    body=False: no body, no baffles, no fixed cells (every cell fluid, every face
+    open). quiescent=True: gas at rest (M = 0) with fixed states equal to it."""
     rng = np.random.default_rng(seed)
     fx, fy, fz = _faces(rng, NX, -0.5 * NX), _faces(rng, NY, 0.0), _faces(rng, NZ, -0.5 * NZ)
     cx, cz = 0.5 * (fx[0] + fx[-1]), 0.5 * (fz[0] + fz[-1])
     ax, ay, az = 0.3 * (fx[-1] - fx[0]) + 0.3, 0.55 * fy[-1] + 0.3, 0.3 * (fz[-1] - fz[0]) + 0.3
 
     def solid(x, y, z):
+        if not body:
+            return np.zeros(np.broadcast(x, y, z).shape, bool)
         return ((x - cx) / ax) ** 2 + (y / ay) ** 2 + ((z - cz) / az) ** 2 < 1.0
 
     dx, dy, dz = np.diff(fx), np.diff(fy), np.diff(fz)
@@ -53,13 +57,13 @@ def synth_case(NX, NY, NZ, seed=0):
     ap[2][:-1][sol[:-1] | sol[1:]] = 0
     ap3 = np.zeros((NZ, NY, NX))
     fluid = np.argwhere((t == 0) & (np.arange(NZ)[:, None, None] + 1 < NZ))
-    if len(fluid):
+    if len(fluid) and body:
         pick = fluid[rng.random(len(fluid)) < 0.05]
         for k, j, i in pick:
             ap[2][k, j, i] = min(ap[2][k, j, i], 1 - 3 / 9)
             ap3[k, j, i] = rng.uniform(-0.12, 0.12)
     full = np.argwhere((t == 0) & (phi == 1.0))
-    if len(full) >= 6:
+    if len(full) >= 6 and body:
         for flag, idx in zip((3, 4, 5), rng.choice(len(full), 3, replace=False)):
             k, j, i = full[idx]
             t[k, j, i] = flag
@@ -74,14 +78,14 @@ def synth_case(NX, NY, NZ, seed=0):
         grid[a + "c"] = (0.5 * (f[:-1] + f[1:])).astype(np.float32)
     geo = {"t": t.reshape(-1), "phi": phi.reshape(-1).astype(np.float32),
            "ap": [a.reshape(-1).astype(np.float32) for a in ap] + [ap3.reshape(-1).astype(np.float32)]}
-    M, al = 0.6, math.radians(4.0)
+    M, al = (0.0 if quiescent else 0.6), math.radians(4.0)
     p0 = 1 / 1.4
     F = [1.0, M * math.cos(al), 0.0, M * math.sin(al), p0 / 0.4 + 0.5 * M * M]
 
     def state(r, u, p):
         return [r, r * u * M, 0.0, 0.0, p * p0 / 0.4 + 0.5 * r * (u * M) ** 2]
 
-    fix = [state(1.05, 0.45, 1.1), state(0.95, 1.0, 1.3), state(0.5, 1.6, 1.2)]
+    fix = [list(F)] * 3 if quiescent else [state(1.05, 0.45, 1.1), state(0.95, 1.0, 1.3), state(0.5, 1.6, 1.2)]
     return grid, geo, F, fix
 
 
