@@ -83,6 +83,42 @@ def test_fighter_readme_coefficients():
     assert rel(cdi, g.s["CDi"]) < COEF_TOL
 
 
+def test_transport_1m_matches_oracle():
+    """The headline workload at full size: the 1.0M-cell transport (M 0.78,
+    alpha 2, CFL 0.5, fixed nacelle states) after 200 iterations on the GPU
+    against the oracle on the host cores: every cell within 1e-5 and CL / CD /
+    Cm within 1e-4. The oracle takes about 8 s on 16 cores."""
+    import math
+    import os
+    import aero3d
+    import geometry
+    say("transport 1M: grid and masks")
+    grid, geo, fl, meta = geometry.build_case("transport:1000000")
+    M, alpha = 0.78, 2.0
+    a = math.radians(alpha)
+    F = [1.0, M * math.cos(a), 0.0, M * math.sin(a), 1 / (1.4 * 0.4) + 0.5 * M * M]
+    fix = meta["fix"]
+    r = meta["ref"]
+    fref = (alpha, M, r["S"], r["xcg"], 0.0, r["c"])
+    s = aero3d.CfdHIP(grid, geo)
+    s.opt["cfl"] = 0.5
+    s.set_free(F); s.set_fix(fix); s.reset(); s.set_force_list(fl)
+    s.step(200)
+    Q = s.read()
+    fd = s.forces(*fref)
+    s.dispose()
+    say("transport 1M: GPU done", fd, "; oracle 200 iterations")
+    o = O.Oracle3D(grid, geo, threads=min(16, os.cpu_count() or 1))
+    o.set_free(F); o.set_fix(fix); o.set_opt(0.5, 2); o.reset(); o.step(200)
+    Qo = o.read()
+    fo = O.forces(Qo, fl, *fref)
+    d = rel_diff(Q, Qo, geo["t"])
+    say("transport 1M state", d, "coefficients GPU", fd, "oracle", fo)
+    assert max(d.values()) < STATE_TOL, d
+    for k in ("CL", "CD", "Cm"):
+        assert rel(fd[k], fo[k]) < COEF_TOL, (k, fd[k], fo[k])
+
+
 def test_device_reductions_match_oracle_on_same_state():
     """Forces, Trefftz and residual in FP64 on the device equal the oracle's
     sums over the identical FP32 state to summation-order rounding."""
