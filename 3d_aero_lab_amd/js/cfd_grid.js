@@ -258,23 +258,17 @@ function updateHeatZones(grid, zones, E, op) {
   }
 }
 
-/* Zones (src/cfd_grid.js:81-191): cut cells (volume fraction phi, east / north
- * face open fractions ae / an from 8 axial sub-lines of solid intervals),
- * thin-wall split cells, small cut cells merged into solids, stream index per
- * cell (with a breadth-first fill of fluid cells no stream contains), blade-row
- * weights, combustor / afterburner cells, heat targets, mixing length. */
-function buildZones(grid, spec, op) {
-  const E = engine(spec);
-  const { W, H, N, xc, rc, dx, xf, rf } = grid, s = E.s;
-  const solid = new Uint8Array(N), mat = new Uint8Array(N), stream = new Int8Array(N);
-  const comp = new Uint8Array(N), wgt = new Float32Array(N), lmix = new Float32Array(N);
-  const phi = new Float32Array(N), ae = new Float32Array(N), an = new Float32Array(N);
+/* The cut-cell pass of buildZones over columns i0..i1-1 (src/cfd_grid.js:81-128):
+ * volume fraction, east / north open fractions, split cells, stream and
+ * material per cell, into out = {phi, ae, an, split, splitFr, stream, mat}.
+ * Columns are independent, so a range can run on any thread. One column per
+ * call: the engine optimises the column function from its own type feedback. */
+function zoneColumns(E, grid, out, i0, i1) {
+  const { W, H, xc, rc, dx, xf, rf } = grid;
+  const { phi, ae, an, split, splitFr, stream, mat } = out;
   const bodies = E.bodies.filter(b => b.mat !== 'liner' && b.mat !== 'fh');
-  const NS = 8, split = new Uint8Array(N), splitFr = new Float32Array(N);
+  const NS = 8;
   const matOf = E.bodies.map(b => MAT_ID[b.mat] || 1);
-  // Each pass is a function of its own, and the cut-cell pass runs one grid
-  // column per call: the engine then optimises every pass from its own type
-  // feedback instead of deoptimising one long function at each new section.
   const column = i => {
     const ivF = solidIntervals(E, bodies, xf[i + 1]);
     const ivs = [];
@@ -311,7 +305,25 @@ function buildZones(grid, spec, op) {
       mat[k] = cc <= -2 ? matOf[-2 - cc] : 1;
     }
   };
-  for (let i = 0; i < W; i++) column(i);
+  for (let i = i0; i < i1; i++) column(i);
+}
+
+/* Zones (src/cfd_grid.js:81-191): cut cells (volume fraction phi, east / north
+ * face open fractions ae / an from 8 axial sub-lines of solid intervals),
+ * thin-wall split cells, small cut cells merged into solids, stream index per
+ * cell (with a breadth-first fill of fluid cells no stream contains), blade-row
+ * weights, combustor / afterburner cells, heat targets, mixing length.
+ * `pre`: the column pass's arrays for every column, computed elsewhere (the
+ * worker-thread build); they are completed in place. */
+function buildZones(grid, spec, op, pre) {
+  const E = engine(spec);
+  const { W, H, N, xc, dx, xf } = grid, s = E.s;
+  const out = pre || { phi: new Float32Array(N), ae: new Float32Array(N), an: new Float32Array(N), split: new Uint8Array(N),
+    splitFr: new Float32Array(N), stream: new Int8Array(N), mat: new Uint8Array(N) };
+  if (!pre) zoneColumns(E, grid, out, 0, W);
+  const { phi, ae, an, split, splitFr, stream, mat } = out;
+  const solid = new Uint8Array(N), comp = new Uint8Array(N), wgt = new Float32Array(N), lmix = new Float32Array(N);
+  // each pass below is a function of its own (optimised from its own type feedback)
   const splitFaces = () => {
     for (let k = 0; k < N; k++) if (split[k]) {
       const i = k % W, j = (k / W) | 0;
@@ -490,4 +502,4 @@ function axiSetup(spec, level) {
   return { grid, zones, comps };
 }
 
-module.exports = { GRID_LEVELS, COMP_ID, axisFaces, buildCFDGrid, buildZones, updateHeatZones, streamMassFlows, compParams, axiSetup, fracIdx, interpState };
+module.exports = { GRID_LEVELS, COMP_ID, axisFaces, buildCFDGrid, engine, zoneColumns, buildZones, updateHeatZones, streamMassFlows, compParams, axiSetup, fracIdx, interpState };

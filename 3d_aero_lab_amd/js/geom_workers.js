@@ -17,12 +17,15 @@
  *   pool.close();
  * The result has the fields of cases.js aircraftCase. A worker keeps the last
  * aircraft it rebuilt, so a pool that serves several requests of one body
- * rebuilds it once.
+ * rebuilds it once. pool.axiSetup(spec, level) does the same for the
+ * axisymmetric setup: buildZones' cut-cell pass (independent per grid
+ * column) in column slabs, the remaining passes on the calling thread.
  * ========================================================================== */
 const { Worker, isMainThread, parentPort } = require('worker_threads');
 const A = require('./air_cfd.js');
 const B = require('./bodies.js');
 const K = require('./cases.js');
+const Z = require('./cfd_grid.js');
 
 // aircraft, solid test and grid of a recipe {type, cells, delta, over}
 function rebuild(r) {
@@ -49,14 +52,37 @@ function runSlabs(S, job) {
   return done;
 }
 
+// engine description and grid of an axisymmetric setup {spec, level}
+function rebuildAxi(r) {
+  return { E: Z.engine(r.spec), grid: Z.buildCFDGrid(r.spec, r.level) };
+}
+
+// column slabs of buildZones' cut-cell pass, claimed from the shared counter
+function runColumns(S, job) {
+  const ctr = new Int32Array(job.ctr), W = S.grid.W, len = job.slab;
+  const out = {
+    phi: new Float32Array(job.bufs.phi), ae: new Float32Array(job.bufs.ae), an: new Float32Array(job.bufs.an),
+    split: new Uint8Array(job.bufs.split), splitFr: new Float32Array(job.bufs.splitFr), stream: new Int8Array(job.bufs.stream),
+    mat: new Uint8Array(job.bufs.mat),
+  };
+  let done = 0;
+  for (;;) {
+    const s = Atomics.add(ctr, 0, 1);
+    if (s >= job.nslab) break;
+    Z.zoneColumns(S.E, S.grid, out, s * len, Math.min(W, (s + 1) * len));
+    done++;
+  }
+  return done;
+}
+
 if (!isMainThread) {
   let key = null, S = null;
   parentPort.on('message', job => {
     try {
       const k = JSON.stringify(job.recipe);
-      if (k !== key) { S = rebuild(job.recipe); key = k; }
-      if (S.g.N !== job.N) throw new Error('geom_workers: grid mismatch');
-      parentPort.postMessage({ ok: true, slabs: runSlabs(S, job) });
+      if (k !== key) { S = job.axi ? rebuildAxi(job.recipe) : rebuild(job.recipe); key = k; }
+      if ((job.axi ? S.grid.N : S.g.N) !== job.N) throw new Error('geom_workers: grid mismatch');
+      parentPort.postMessage({ ok: true, slabs: job.axi ? runColumns(S, job) : runSlabs(S, job) });
     } catch (e) {
       parentPort.postMessage({ ok: false, error: String(e && e.stack || e) });
     }
@@ -100,6 +126,38 @@ class GeomPool {
   }
 
   transportCase(cells, deltaDeg = 0) { return this.aircraftCase('airliner', cells, deltaDeg, B.TRANSPORT_OVERRIDES); }
+
+  /* cfd_grid.js axiSetup(spec, level) with buildZones' cut-cell pass (the
+   * column pass, most of the work) spread over the pool: {grid, zones, comps},
+   * the zone arrays byte-identical to the one-thread build. */
+  async axiSetup(spec, level, slab = 0) {
+    if (this.busy) throw new Error('GeomPool: one build at a time');
+    if (!this.workers) throw new Error('GeomPool: closed');
+    this.busy = true;
+    try {
+      const recipe = { spec, level: level || spec.level };
+      const S = rebuildAxi(recipe), grid = S.grid, N = grid.N, W = grid.W;
+      const nt = this.workers.length + 1;
+      const len = slab > 0 ? slab : Math.max(1, Math.ceil(W / (4 * nt)));
+      const bufs = {};
+      for (const [k, b] of [['phi', 4], ['ae', 4], ['an', 4], ['split', 1], ['splitFr', 4], ['stream', 1], ['mat', 1]]) bufs[k] = new SharedArrayBuffer(b * N);
+      const job = { axi: true, recipe, N, slab: len, nslab: Math.ceil(W / len), ctr: new SharedArrayBuffer(4), bufs };
+      const replies = this.workers.map(w => new Promise(res => w.once('message', res)));
+      for (const w of this.workers) w.postMessage(job);
+      runColumns(S, job);
+      for (const r of await Promise.all(replies)) if (!r.ok) throw new Error(r.error);
+      // plain (non-shared) arrays for the caller, like the one-thread build
+      const pre = {
+        phi: new Float32Array(bufs.phi).slice(), ae: new Float32Array(bufs.ae).slice(), an: new Float32Array(bufs.an).slice(),
+        split: new Uint8Array(bufs.split).slice(), splitFr: new Float32Array(bufs.splitFr).slice(), stream: new Int8Array(bufs.stream).slice(),
+        mat: new Uint8Array(bufs.mat).slice(),
+      };
+      const zones = Z.buildZones(grid, spec, spec.op, pre);
+      return { grid, zones, comps: Z.compParams(spec.type, spec.op, spec.eff) };
+    } finally {
+      this.busy = false;
+    }
+  }
 
   close() {
     if (!this.workers) return Promise.resolve();

@@ -52,3 +52,17 @@ def test_checkpoint_format_matches_reference(tmp_path):
     assert r.returncode == 0, r.stderr[-2000:]
     res = json.loads(r.stdout)
     assert all(res.values()), res
+
+
+def test_pool_setups_byte_identical():
+    """The worker-thread setup (GeomPool.axiSetup: buildZones' column pass in
+    slabs of 1, 7 and the default column count over 3 workers + the caller)
+    gives the reference's grids, zones and constants for all 12 setups."""
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "geom_pool_axi_check.js"), "3",
+                        os.path.join(ROOT, "tests", "golden", "axi_setups")], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    assert len(res) == 36
+    bad = {k: v for k, v in res.items() if v is not True}
+    assert not bad, bad
+
