@@ -214,7 +214,7 @@ class Result(C.Structure):
 
 BATCH_EXPORTS = ["aero3d_batch_create", "aero3d_batch_destroy", "aero3d_batch_last_error", "aero3d_batch_add_geometry",
                  "aero3d_batch_remove_geometry", "aero3d_batch_case_bytes", "aero3d_batch_add", "aero3d_batch_remove",
-                 "aero3d_batch_set_zchunk", "aero3d_batch_set_group_cells", "aero3d_batch_step", "aero3d_batch_sync", "aero3d_batch_iter", "aero3d_batch_read",
+                 "aero3d_batch_set_zchunk", "aero3d_batch_set_group_cells", "aero3d_batch_set_persistent", "aero3d_batch_step", "aero3d_batch_sync", "aero3d_batch_iter", "aero3d_batch_read",
                  "aero3d_batch_residuals", "aero3d_batch_forces", "aero3d_batch_footprint", "aero3d_batch_time_steps",
                  "aero3d_server_create", "aero3d_server_destroy", "aero3d_server_batch", "aero3d_server_clock",
                  "aero3d_server_submit", "aero3d_server_tick", "aero3d_server_poll", "aero3d_server_stats"]
@@ -240,6 +240,7 @@ def blib():
         L.aero3d_batch_remove.argtypes = [vp, ci]
         L.aero3d_batch_set_zchunk.argtypes = [vp, ci]
         L.aero3d_batch_set_group_cells.argtypes = [vp, C.c_int64]
+        L.aero3d_batch_set_persistent.argtypes = [vp, C.c_int]
         L.aero3d_batch_step.argtypes = [vp, ci]
         L.aero3d_batch_sync.argtypes = [vp]
         L.aero3d_batch_iter.argtypes = [vp, ci, pi]
@@ -323,6 +324,9 @@ class Batch:
 
     def set_zchunk(self, zc):
         self._chk(blib().aero3d_batch_set_zchunk(self.h, zc))
+
+    def set_persistent(self, on):
+        self._chk(blib().aero3d_batch_set_persistent(self.h, int(bool(on))))
 
     def set_group_cells(self, cells):
         self._chk(blib().aero3d_batch_set_group_cells(self.h, int(cells)))

@@ -119,12 +119,13 @@ def coef_rel_diff(done, coefs):
     return worst
 
 
-def run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes, max_slots=256, group_cells=1 << 40):
+def run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes, max_slots=256, group_cells=1 << 40, persistent=0):
     """Submit requests (geometry index, M, alpha, arrival s) to the continuous-batching server as their
     arrival times come; tick until all are done. Returns results and the wall time."""
     import aero3d
     srv = aero3d.Server(arena_bytes=arena_bytes, max_slots=max_slots, check_every=check_every)
     srv.batch.set_zchunk(zc)
+    srv.batch.set_persistent(persistent)
     srv.batch.set_group_cells(group_cells)
     gids = [srv.batch.add_geometry(g, geo, fl) for (g, geo, fl, meta) in built]
     t_start = srv.clock()
@@ -164,20 +165,22 @@ def batched_leg(res_drop=1e-4, max_iters=2000, check_every=50):
         F, fix, _ = case_setup(meta, M, alpha)
         tb.add(tb.add_geometry(g, geo), F, fix, 0.5)
     best = None
-    for zc in (3, 4, 6, 8):
-        for gc in (1_000_000, 2_000_000, 4_000_000, 1 << 40):
-            tb.set_zchunk(zc)
-            tb.set_group_cells(gc)
-            tb.time_steps(3)
-            ms = tb.time_steps(3)
-            if best is None or ms < best[1]:
-                best = (zc, ms, gc)
+    for pers in (0, 1):
+        tb.set_persistent(pers)
+        for zc in (3, 4, 6, 8):
+            for gc in (1_000_000, 2_000_000, 4_000_000, 1 << 40):
+                tb.set_zchunk(zc)
+                tb.set_group_cells(gc)
+                tb.time_steps(3)
+                ms = tb.time_steps(3)
+                if best is None or ms < best[1]:
+                    best = (zc, ms, gc, pers)
     tb.dispose()
-    zc, group = best[0], best[2]
-    print(f"[bench] batched: {cells} cells, zchunk {zc}, group {group}; server run", file=sys.stderr, flush=True)
+    zc, group, pers = best[0], best[2], best[3]
+    print(f"[bench] batched: {cells} cells, zchunk {zc}, group {group}, persistent {pers}; server run", file=sys.stderr, flush=True)
     reqs = [(k, M, alpha, 0.0) for k, (M, alpha) in enumerate(conds)]
     done, wall_b, busy, cu = run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes=int(cells * 110 + (2 << 30)),
-                                        group_cells=group)
+                                        group_cells=group, persistent=pers)
     print("[bench] batched: plain path, one case at a time", file=sys.stderr, flush=True)
     wall_p, iters_p, coefs_p = run_converged_plain(built, conds, res_drop, max_iters, check_every, (8, 0))
     it_b = {d["tag"]: d["iters"] for d in done}
@@ -187,7 +190,7 @@ def batched_leg(res_drop=1e-4, max_iters=2000, check_every=50):
             "cell_updates_per_s": cu / wall_b, "plain_solves_per_s": len(conds) / wall_p, "plain_wall_s": wall_p,
             "speedup_vs_plain": wall_p / wall_b, "converged": sum(d["converged"] for d in done),
             "iters_mean": sum(it_b.values()) / len(it_b), "same_iterations_as_plain": same, "cl_cd_max_rel_diff_vs_plain": dcoef,
-            "zchunk": zc, "group_cells": group,
+            "zchunk": zc, "group_cells": group, "launch": "persistent" if pers else "one block per work item",
             "criterion": f"res/res0 <= {res_drop} (density residual every {check_every} iterations) or {max_iters} iterations"}
 
 
@@ -226,11 +229,18 @@ def serve_leg(n_requests=200, res_drop=1e-4, max_iters=2000, check_every=50):
     it_s = {d["tag"]: d["iters"] for d in done}
     # capacity: the same requests all arriving at once (the stream above is arrival-bound at ~20 req/s)
     print("[bench] serve: capacity run, all requests at t = 0", file=sys.stderr, flush=True)
-    done_c, wall_c, _, cu_c = run_server(built, [(r[0], r[1], r[2], 0.0) for r in reqs], res_drop, max_iters, check_every, 3,
-                                         arena_bytes=64 << 30)
+    cap = {}
+    for pers in (0, 1):
+        done_c, wall_c, _, cu_c = run_server(built, [(r[0], r[1], r[2], 0.0) for r in reqs], res_drop, max_iters, check_every, 3,
+                                             arena_bytes=64 << 30, persistent=pers)
+        cap[pers] = (done_c, wall_c, cu_c)
+    pers = min(cap, key=lambda p: cap[p][1])
+    done_c, wall_c, cu_c = cap[pers]
     return {"requests": len(done), "sample": f"first {n_requests} of the 10k-request stream (seed 20261204)",
             "capacity_solves_per_s": len(done_c) / wall_c, "capacity_cell_updates_per_s": cu_c / wall_c,
             "capacity_speedup_vs_plain": wall_p / wall_c,
+            "capacity_launch": "persistent" if pers else "one block per work item",
+            "capacity_solves_per_s_by_launch": {"one block per work item": len(cap[0][0]) / cap[0][1], "persistent": len(cap[1][0]) / cap[1][1]},
             "solves_per_s": len(done) / wall, "wall_s": wall, "p50_s": pct(50), "p95_s": pct(95), "p99_s": pct(99),
             "gpu_busy_frac": busy / wall, "cell_updates_per_s": cu / wall, "converged": sum(d["converged"] for d in done),
             "plain_solves_per_s": len(reqs) / wall_p, "plain_wall_s": wall_p, "speedup_vs_plain": wall_p / wall,

@@ -146,6 +146,11 @@ int aero3d_batch_set_zchunk(aero3d_batch *b, int zchunk);
  * one group): a group's state and masks stay in the Infinity Cache across its
  * stages and iterations before the next group runs. */
 int aero3d_batch_set_group_cells(aero3d_batch *b, int64_t cells);
+/* Launch form of the stage kernel: 0 (default) one block per work item (the
+ * grid is the group's item count; a graph is captured again when it changes),
+ * 1 the device's resident blocks walking all items (one persistent launch per
+ * stage whatever the table). Results are the same bits either way. */
+int aero3d_batch_set_persistent(aero3d_batch *b, int on);
 /* n SSP-RK2 iterations of every resident case (src/air_cfd.js:252-258), enqueued. */
 int aero3d_batch_step(aero3d_batch *b, int n);
 int aero3d_batch_sync(aero3d_batch *b);

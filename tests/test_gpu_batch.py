@@ -40,11 +40,17 @@ def plain(g, F, fix, cfl, n):
     return Q
 
 
+@pytest.mark.parametrize("persistent", [0, 1])
 @pytest.mark.parametrize("group_cells", [2_000_000, 40_000])
-def test_batch_equals_plain_and_oracle(group_cells):
+def test_batch_equals_plain_and_oracle(group_cells, persistent):
+    """Both launch forms (one block per work item, resident blocks walking the
+    items), one group and one group per case: bit-identical to the plain path,
+    before and after a case leaves (which changes the item count)."""
     import aero3d
     b = aero3d.Batch(arena_bytes=1 << 30, max_slots=16)
     b.set_group_cells(group_cells)        # 40k: every case in its own group
+    b.set_persistent(persistent)
+    say("batch", group_cells, "persistent", persistent)
     cs = cases()
     gids = {name: b.add_geometry(g.grid(), g.geo()) for name, g, *_ in cs}
     slots = {}
@@ -76,6 +82,7 @@ def test_batch_equals_plain_and_oracle(group_cells):
     d = rel_diff(Q, g.state("it100"), g.geo()["t"])
     say("transport60k it100 in batch", d)
     assert max(d.values()) < 1e-5
+    assert np.array_equal(Q.view(np.uint32), plain(g, cs[1][2], cs[1][3], cs[1][4], 100).view(np.uint32))
     b.dispose()
 
 
