@@ -18,7 +18,8 @@
 //   k+1, the z flux of face k-1/2 carried from the previous plane.
 // Per plane: load (own plane k+2 into the queue, halo of plane k into LDS
 // from registers loaded one plane earlier), slopes, fluxes, cell update;
-// three barriers.
+// three barriers. In the balanced launch each wave's issue priority falls as
+// it advances through its chunk (progress_prio).
 // Divisions use the hardware reciprocal, square root and reciprocal square
 // root (1 ulp).
 // ============================================================================
@@ -30,6 +31,20 @@ namespace aero3d {
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// Issue priority of this wave from the share of its chunk already done: 3 in
+// the first quarter down to 0 in the last. Two blocks share each CU, and at
+// equal priority the SIMDs issue the older block's waves first, so the older
+// block finishes some 15 % of the launch ahead and the younger then runs alone
+// at half the occupancy. With the priority falling as a block advances, the
+// block that is behind is issued first and both finish together.
+__device__ __forceinline__ void progress_prio(int done, int len) {
+  const int q = (4 * done) / len;
+  if (q == 0) __builtin_amdgcn_s_setprio(3);
+  else if (q == 1) __builtin_amdgcn_s_setprio(2);
+  else if (q == 2) __builtin_amdgcn_s_setprio(1);
+  else __builtin_amdgcn_s_setprio(0);
+}
 
 // Global access by a 32-bit element index from a wave-uniform base: the byte
 // offset is formed in 32 bits, so the access takes the scalar-base + vector-
@@ -168,7 +183,10 @@ __device__ __forceinline__ int load_prim(const CaseView &cv, const StageConsts &
 
 // One stage over the z chunk [k0, k1) of tile (i0, j0) of case cv.
 // Writes out[] for every cell of the chunk; dtout (if non-null) gets dt.
-template <int TX, int TY>
+// kPrio: set the wave priority from the chunk's progress (for launches whose
+// blocks are all resident at once; where blocks are dispatched over many
+// rounds, oldest-first issue finishes blocks sooner and frees their slots).
+template <int TX, int TY, bool kPrio = false>
 __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts &kc, const float *__restrict__ qin,
                                            const float *__restrict__ bse, float *__restrict__ out, float *__restrict__ dtout,
                                            size_t ns, float wa, float wb, int i0, int j0, int k0, int k1,
@@ -266,6 +284,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
   ap_load(k0);
 
   for (int k = k0; k < k1; k++) {
+    if constexpr (kPrio) progress_prio(k - k0, k1 - k0);
     // ---------------- load phase
     Tz[3] = load_prim(cv, kc, qin, ns, own ? i : -1, j, k + 2, Pz[3]);
     const float A2z = apz(k + 1);                   // face k+3/2
