@@ -272,7 +272,8 @@ int aero3d_create(const aero3d_grid *g, const uint8_t *t, const float *phi, cons
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(s, AERO3D_E_NODEV, "hipStreamCreate failed"));
   {
     int ncu = 0, o4 = 0, o8 = 0;
-    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device);
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || ncu <= 0)
+      return bail(fail(s, AERO3D_E_NODEV, "hipDeviceGetAttribute(MultiprocessorCount) failed"));
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o4, tile_kernel_bal<4, true>, 256, 0) != hipSuccess) o4 = 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o8, tile_kernel_bal<8, true>, 512, 0) != hipSuccess) o8 = 1;
     s->resident4 = std::max(1, ncu * std::max(1, o4));
@@ -303,10 +304,11 @@ int aero3d_create(const aero3d_grid *g, const uint8_t *t, const float *phi, cons
 
 int aero3d_destroy(aero3d_solver *s) {
   if (!s) return 0;
-  if (s->stream) hipStreamSynchronize(s->stream);
+  // teardown: an error here leaves nothing to recover, so return codes are not checked
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
   void *ptrs[] = {s->dx, s->dy, s->dz, s->t, s->phi, s->ap, s->state, s->dt, s->rsnap, s->fq, s->fcx, s->fcz, s->fx, s->fz, s->partials, s->sums};
-  for (void *p : ptrs) if (p) hipFree(p);
-  if (s->stream) hipStreamDestroy(s->stream);
+  for (void *p : ptrs) if (p) (void)hipFree(p);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return 0;
 }
@@ -380,8 +382,12 @@ int aero3d_set_force_list(aero3d_solver *s, const int32_t *q, const double *cx, 
     if (q[k] < 0 || q[k] >= s->N) return fail(s, AERO3D_E_ARG, "set_force_list: cell index out of range");
   HIPCHK(s, hipStreamSynchronize(s->stream));
   void *old[] = {s->fq, s->fcx, s->fcz, s->fx, s->fz};
-  for (void *p : old) if (p) hipFree(p);
+  hipError_t fe = hipSuccess;
+  for (void *p : old)
+    if (p) { const hipError_t e = hipFree(p); if (fe == hipSuccess) fe = e; }
   s->fq = nullptr; s->fcx = s->fcz = s->fx = s->fz = nullptr;
+  s->nf = 0;                 // no list until the new one is in place
+  HIPCHK(s, fe);
   int rc;
   if ((rc = dalloc(s, &s->fq, n)) || (rc = dalloc(s, &s->fcx, n)) || (rc = dalloc(s, &s->fcz, n)) ||
       (rc = dalloc(s, &s->fx, n)) || (rc = dalloc(s, &s->fz, n)))
@@ -512,14 +518,14 @@ int aero3d_autotune(aero3d_solver *s, int iters, int *ty, int *zchunk, float *ms
       float ms;
       int rc = aero3d_time_steps(s, 1, &ms);          // warm
       if (!rc) rc = aero3d_time_steps(s, iters, &ms);
-      if (rc) { hipFree(save); return rc; }
+      if (rc) { (void)hipFree(save); return rc; }
       if (ms < best) { best = ms; bty = t; bzc = z; }
     }
   s->tile_ty = bty; s->tile_zc = bzc;
   s->cur = cur0; s->it = it0;
   HIPCHK(s, hipMemcpyAsync(buf(s, s->cur), save, b, hipMemcpyDeviceToDevice, s->stream));
   HIPCHK(s, hipStreamSynchronize(s->stream));
-  hipFree(save);
+  HIPCHK(s, hipFree(save));
   if (ty) *ty = bty;
   if (zchunk) *zchunk = bzc;
   if (ms_best) *ms_best = best / iters;

@@ -371,9 +371,11 @@ int aero3d_batch_create(int64_t arena_bytes, int max_slots, aero3d_batch **out) 
     return AERO3D_E_HIP_BASE;
   }
   int dev = 0, ncu = 0, occ = 0;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_kernel<1, 1>, 64 * kTY, 0);
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_kernel<1, 1>, 64 * kTY, 0) != hipSuccess) {
+    aero3d_batch_destroy(b);
+    return AERO3D_E_HIP_BASE;
+  }
   b->grid = std::max(1, ncu * std::max(1, occ));       // resident blocks: the persistent form's grid
   *out = b;
   return 0;
@@ -381,13 +383,14 @@ int aero3d_batch_create(int64_t arena_bytes, int max_slots, aero3d_batch **out) 
 
 int aero3d_batch_destroy(aero3d_batch *b) {
   if (!b) return 0;
-  if (b->stream) hipStreamSynchronize(b->stream);
+  // teardown: an error here leaves nothing to recover, so return codes are not checked
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto e : b->execs) if (e) (void)hipGraphExecDestroy(e);
   for (auto g : b->graphs) if (g) (void)hipGraphDestroy(g);
-  if (b->arena.base) hipFree(b->arena.base);
-  if (b->d_hdr) hipFree(b->d_hdr);
-  if (b->d_table) hipFree(b->d_table);
-  if (b->stream) hipStreamDestroy(b->stream);
+  if (b->arena.base) (void)hipFree(b->arena.base);
+  if (b->d_hdr) (void)hipFree(b->d_hdr);
+  if (b->d_table) (void)hipFree(b->d_table);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return 0;
 }

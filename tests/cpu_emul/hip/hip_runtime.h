@@ -31,6 +31,8 @@ inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
 inline float __builtin_amdgcn_sqrtf(float x) { return std::sqrt(x); }
 inline float __builtin_amdgcn_rsqf(float x) { return 1.0f / std::sqrt(x); }
 inline float __builtin_amdgcn_fmed3f(float a, float b, float c) { return std::max(std::min(a, b), std::min(std::max(a, b), c)); }
+// wave issue priority: scheduling only, no effect on results
+inline void __builtin_amdgcn_s_setprio(int) {}
 #include <math.h>
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
