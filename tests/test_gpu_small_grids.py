@@ -56,6 +56,29 @@ def test_small_grid_every_shape_matches_oracle(n):
     s.dispose()
 
 
+@pytest.mark.parametrize("ty", [8, 4])
+def test_balanced_launch_priority_bit_identical(ty):
+    """A balanced launch that fits one resident round sets each wave's priority
+    from its chunk's progress; a launch of three times as many blocks (several
+    rounds) keeps the default issue order. On a 174 x 77 x 40 grid with a body
+    the second launch has more z chunks per tile than the first (up to one per
+    plane), so both forms run. Priority changes only the order of issue: after 20
+    iterations the states and the last stage's local dt are the same bits."""
+    import aero3d
+    grid, geo, F, fix = synth_case(174, 77, 40, seed=3)
+    out = []
+    for zc in (0, -3):
+        say("balanced launch", ty, zc)
+        s = aero3d.CfdHIP(grid, geo)
+        s.opt["cfl"] = 0.5
+        s.set_free(F); s.set_fix(fix); s.reset()
+        s.set_tile(ty, zc)
+        s.step(20)
+        out.append(s.read(with_dt=True))
+        s.dispose()
+    assert np.array_equal(out[0], out[1])
+
+
 def test_small_grids_batched_equal_plain():
     import aero3d
     cs = [(n, *synth_case(*n, seed=sum(n))) for n in SIZES]
