@@ -116,8 +116,8 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4)
 // levels: x-y neighbour tiles (which read each other's halo rows) and the
 // chunk below (whose top planes start this chunk's z queue) share an L2.
 // kPrio (launches of at most one resident round): wave priority from chunk
-// progress, so the two blocks of a CU finish together.
-template <int TY, bool kPrio>
+// progress, so the two blocks of a CU finish together. kMode: as tile_stage.
+template <int TY, bool kPrio, int kMode>
 __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4))) tile_kernel_bal(CaseView cv, StageConsts kc, const float *__restrict__ qin,
                                                           const float *__restrict__ bse, float *__restrict__ out,
                                                           float *__restrict__ dtout, size_t ns, float wa, float wb, int ntx, int nty) {
@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4)
   const int l = L / nt, t = L % nt;
   const int k0 = (int)(((long long)l * cv.NZ) / c), k1 = (int)(((long long)(l + 1) * cv.NZ) / c);
   const int bx = t % ntx, by = t / ntx;
-  tile_stage<64, TY, kPrio>(cv, kc, qin, bse, out, dtout, ns, wa, wb, bx * 64, by * TY, k0, k1, sm);
+  tile_stage<64, TY, kPrio, kMode>(cv, kc, qin, bse, out, dtout, ns, wa, wb, bx * 64, by * TY, k0, k1, sm);
 }
 
 // Reset: every plane of buffer `dst` := F.
@@ -211,12 +211,18 @@ void launch_stage(aero3d_solver *s, const CaseView &cv, const StageConsts &kc, c
     const long long c = std::max(1LL, std::min((long long)s->NZ, res / nt));   // z chunks per tile
     const long long G = nt * c;
     const bool one_round = G <= (s->tile_ty == 4 ? s->resident4 : s->resident8);
+    const int mode = qin == bse ? 1 : (dtout ? 3 : 2);
     if (s->tile_ty == 4) {
-      if (one_round) tile_kernel_bal<4, true><<<(int)G, dim3(64, 4), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
-      else tile_kernel_bal<4, false><<<(int)G, dim3(64, 4), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
+      if (one_round) tile_kernel_bal<4, true, 0><<<(int)G, dim3(64, 4), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
+      else tile_kernel_bal<4, false, 0><<<(int)G, dim3(64, 4), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
+    } else if (one_round) {
+      if (mode == 1) tile_kernel_bal<8, true, 1><<<(int)G, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
+      else if (mode == 2) tile_kernel_bal<8, true, 2><<<(int)G, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
+      else tile_kernel_bal<8, true, 3><<<(int)G, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
     } else {
-      if (one_round) tile_kernel_bal<8, true><<<(int)G, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
-      else tile_kernel_bal<8, false><<<(int)G, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
+      if (mode == 1) tile_kernel_bal<8, false, 1><<<(int)G, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
+      else if (mode == 2) tile_kernel_bal<8, false, 2><<<(int)G, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
+      else tile_kernel_bal<8, false, 3><<<(int)G, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
     }
     return;
   }
@@ -274,8 +280,8 @@ int aero3d_create(const aero3d_grid *g, const uint8_t *t, const float *phi, cons
     int ncu = 0, o4 = 0, o8 = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || ncu <= 0)
       return bail(fail(s, AERO3D_E_NODEV, "hipDeviceGetAttribute(MultiprocessorCount) failed"));
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o4, tile_kernel_bal<4, true>, 256, 0) != hipSuccess) o4 = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o8, tile_kernel_bal<8, true>, 512, 0) != hipSuccess) o8 = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o4, tile_kernel_bal<4, true, 0>, 256, 0) != hipSuccess) o4 = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o8, tile_kernel_bal<8, true, 2>, 512, 0) != hipSuccess) o8 = 1;
     s->resident4 = std::max(1, ncu * std::max(1, o4));
     s->resident8 = std::max(1, ncu * std::max(1, o8));
   }

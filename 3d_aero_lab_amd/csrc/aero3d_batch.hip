@@ -83,8 +83,8 @@ __global__ void __launch_bounds__(64 * kTY) __attribute__((amdgpu_waves_per_eu(4
     const size_t plane5 = 5 * (size_t)S.ns;
     float *Q = S.state + (size_t)q * plane5, *S1 = S.state + (size_t)((q + 1) % 3) * plane5,
           *Bb = S.state + (size_t)((q + 2) % 3) * plane5;
-    if constexpr (stage == 0) tile_stage<64, kTY>(S.cv, S.kc, Q, Q, S1, nullptr, S.ns, 0.f, 1.f, bx * 64, by * kTY, k0, k1, sm);
-    else tile_stage<64, kTY>(S.cv, S.kc, S1, Q, Bb, nullptr, S.ns, 0.5f, 0.5f, bx * 64, by * kTY, k0, k1, sm);
+    if constexpr (stage == 0) tile_stage<64, kTY, false, 1>(S.cv, S.kc, Q, Q, S1, nullptr, S.ns, 0.f, 1.f, bx * 64, by * kTY, k0, k1, sm);
+    else tile_stage<64, kTY, false, 2>(S.cv, S.kc, S1, Q, Bb, nullptr, S.ns, 0.5f, 0.5f, bx * 64, by * kTY, k0, k1, sm);
   };
   if constexpr (persist) {
     for (int base = 0; base < total; base += G) {

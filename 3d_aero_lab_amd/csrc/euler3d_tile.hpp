@@ -186,7 +186,11 @@ __device__ __forceinline__ int load_prim(const CaseView &cv, const StageConsts &
 // kPrio: set the wave priority from the chunk's progress (for launches whose
 // blocks are all resident at once; where blocks are dispatched over many
 // rounds, oldest-first issue finishes blocks sooner and frees their slots).
-template <int TX, int TY, bool kPrio = false>
+// kMode: 0 decides at run time (stage 1 when qin == bse, dt written when
+// dtout is non-null); 1 stage 1 (bse == qin), 2 stage 2 without dt, 3 stage 2
+// with dt. The fixed modes hold fewer base pointers in scalar registers
+// (fewer of them spilled to vector-register lanes and reloaded per plane).
+template <int TX, int TY, bool kPrio = false, int kMode = 0>
 __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts &kc, const float *__restrict__ qin,
                                            const float *__restrict__ bse, float *__restrict__ out, float *__restrict__ dtout,
                                            size_t ns, float wa, float wb, int i0, int j0, int k0, int k1,
@@ -410,7 +414,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
     if (own) {
 #pragma unroll
       for (int c = 0; c < 5; c++) Qc[c] = gld(qin + c * ns, qk);
-      if (qin == bse) {            // stage 1: the base state is the input state (one read)
+      if (kMode == 1 || (kMode == 0 && qin == bse)) {   // stage 1: the base state is the input state (one read)
 #pragma unroll
         for (int c = 0; c < 5; c++) Bq[c] = Qc[c];
       } else {
@@ -428,7 +432,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       if (tq != 0) {               // solid and fixed cells keep their state (a fixed cell holds its fixed state)
 #pragma unroll
         for (int c = 0; c < 5; c++) gst(out + c * ns, q, Bq[c]);
-        if (dtout) gst(dtout, q, 0.f);
+        if (kMode == 3 || (kMode == 0 && dtout)) gst(dtout, q, 0.f);
       } else {
         const float ihz = frcp(dzq);
         const float *C = Pz[1];
@@ -476,7 +480,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         if (kGm1 * (n[4] - ke) < 1e-5f) n[4] = 2.5e-5f + ke;
 #pragma unroll
         for (int c = 0; c < 5; c++) gst(out + c * ns, q, n[c]);
-        if (dtout) gst(dtout, q, dt);
+        if (kMode == 3 || (kMode == 0 && dtout)) gst(dtout, q, dt);
       }
       sUpPrev = sUpq;
     }
