@@ -197,6 +197,14 @@ int apply_fix(aero3d_solver *s) {
   return 0;
 }
 
+// The balanced kernels by [one resident round][stage mode - 1]; stage mode 1 =
+// stage 1 (qin == bse, which never writes dt), 2 = stage 2, 3 = stage 2 with dt.
+using BalKernel = void (*)(CaseView, StageConsts, const float *, const float *, float *, float *, size_t, float, float, int, int);
+const BalKernel kBal4[2][3] = {{tile_kernel_bal<4, false, 1>, tile_kernel_bal<4, false, 2>, tile_kernel_bal<4, false, 3>},
+                               {tile_kernel_bal<4, true, 1>, tile_kernel_bal<4, true, 2>, tile_kernel_bal<4, true, 3>}};
+const BalKernel kBal8[2][3] = {{tile_kernel_bal<8, false, 1>, tile_kernel_bal<8, false, 2>, tile_kernel_bal<8, false, 3>},
+                               {tile_kernel_bal<8, true, 1>, tile_kernel_bal<8, true, 2>, tile_kernel_bal<8, true, 3>}};
+
 void launch_stage(aero3d_solver *s, const CaseView &cv, const StageConsts &kc, const float *qin, const float *bse, float *out,
                   float *dtout, float wa, float wb) {
   if (s->tile_ty == 0) {
@@ -212,18 +220,8 @@ void launch_stage(aero3d_solver *s, const CaseView &cv, const StageConsts &kc, c
     const long long G = nt * c;
     const bool one_round = G <= (s->tile_ty == 4 ? s->resident4 : s->resident8);
     const int mode = qin == bse ? 1 : (dtout ? 3 : 2);
-    if (s->tile_ty == 4) {
-      if (one_round) tile_kernel_bal<4, true, 0><<<(int)G, dim3(64, 4), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
-      else tile_kernel_bal<4, false, 0><<<(int)G, dim3(64, 4), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
-    } else if (one_round) {
-      if (mode == 1) tile_kernel_bal<8, true, 1><<<(int)G, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
-      else if (mode == 2) tile_kernel_bal<8, true, 2><<<(int)G, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
-      else tile_kernel_bal<8, true, 3><<<(int)G, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
-    } else {
-      if (mode == 1) tile_kernel_bal<8, false, 1><<<(int)G, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
-      else if (mode == 2) tile_kernel_bal<8, false, 2><<<(int)G, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
-      else tile_kernel_bal<8, false, 3><<<(int)G, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
-    }
+    const BalKernel k = s->tile_ty == 4 ? kBal4[one_round][mode - 1] : kBal8[one_round][mode - 1];
+    k<<<(int)G, dim3(64, s->tile_ty), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
     return;
   }
   const int nzc = (s->NZ + s->tile_zc - 1) / s->tile_zc;
@@ -280,7 +278,7 @@ int aero3d_create(const aero3d_grid *g, const uint8_t *t, const float *phi, cons
     int ncu = 0, o4 = 0, o8 = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || ncu <= 0)
       return bail(fail(s, AERO3D_E_NODEV, "hipDeviceGetAttribute(MultiprocessorCount) failed"));
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o4, tile_kernel_bal<4, true, 0>, 256, 0) != hipSuccess) o4 = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o4, tile_kernel_bal<4, true, 2>, 256, 0) != hipSuccess) o4 = 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o8, tile_kernel_bal<8, true, 2>, 512, 0) != hipSuccess) o8 = 1;
     s->resident4 = std::max(1, ncu * std::max(1, o4));
     s->resident8 = std::max(1, ncu * std::max(1, o8));
