@@ -32,6 +32,23 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// Keeps a per-lane value in a vector register, opaque to the optimiser from
+// this point on. Tests of a lane's position (tile-edge lanes, halo-ring cells)
+// do not change from plane to plane; hoisted out of the plane loop they become
+// 64-bit lane masks in scalar registers, which do not fit and are spilled to
+// vector-register lanes and reloaded on every plane. Made opaque at the top of
+// each plane, they are recomputed from a few vector registers instead. Only the
+// device pass needs this (elsewhere, e.g. a CPU build of this header, it is a
+// no-op).
+template <class T>
+__device__ __forceinline__ void keep_vgpr(T &x) {
+#ifdef __AMDGCN__
+  asm volatile("" : "+v"(x));
+#else
+  (void)x;
+#endif
+}
+
 // Issue priority of this wave from the share of its chunk already done: 3 in
 // the first quarter down to 0 in the last. Two blocks share each CU, and at
 // equal priority the SIMDs issue the older block's waves first, so the older
@@ -245,11 +262,16 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
     hmir[p] = gj < 0;
     hq[p] = hin[p] ? (unsigned)((hmir[p] ? -1 - gj : gj) * NX + gi) : 0u;
   }
+  // copies of the lane's thread index and halo-ring data made opaque per plane (keep_vgpr)
+  int tidv = tid;
+  int hoffv[NHP], hinv[NHP], hmirv[NHP];
+#pragma unroll
+  for (int p = 0; p < NHP; p++) { hoffv[p] = hoff[p]; hinv[p] = hin[p]; hmirv[p] = hmir[p]; }
   // raw state of plane kk (cell 0 stands in outside the box)
   auto halo_load = [&](int kk) {
 #pragma unroll
     for (int p = 0; p < NHP; p++) {
-      const unsigned q = hin[p] ? (unsigned)(kk * NXY) + hq[p] : 0u;
+      const unsigned q = hinv[p] ? (unsigned)(kk * NXY) + hq[p] : 0u;
       ht[p] = gld(cv.t, q);
 #pragma unroll
       for (int v = 0; v < 5; v++) ha[p][v] = gld(qin + v * ns, q);
@@ -260,25 +282,26 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
   // of lanes 0..2TY-1 and 64..64+2TX-1) and the far-face aperture of this
   // lane's tile-edge slope, loaded one plane ahead
   const int et = (tid + TX * TY - 2 * TX) % (TX * TY);
+  int etv = et;
   float aX = 1.f, aY = 1.f, aE = 1.f, aS = 1.f;
   auto ap_load = [&](int kk) {
     const unsigned qq = (unsigned)(kk * NXY) + qc;
     aX = own ? gld(ap0, qq) : 1.f;
     aY = own ? gld(ap1, qq) : 1.f;
-    if (tid < 2 * TY) {
-      const int r = tid >> 1, ii = (tid & 1) - 2, gi = i0 + ii, gj = j0 + r;
+    if (tidv < 2 * TY) {
+      const int r = tidv >> 1, ii = (tidv & 1) - 2, gi = i0 + ii, gj = j0 + r;
       aE = (gi < 0 || gi >= NX || gj >= NY) ? 1.f : gld(ap0, (unsigned)((kk * NY + gj) * NX + gi));
-    } else if (tid >= 64 && tid < 64 + 2 * TX) {
-      const int u = tid - 64, c = u >> 1, jj = (u & 1) - 2, gi = i0 + c, gj = j0 + jj;
+    } else if (tidv >= 64 && tidv < 64 + 2 * TX) {
+      const int u = tidv - 64, c = u >> 1, jj = (u & 1) - 2, gi = i0 + c, gj = j0 + jj;
       aE = (gj < 0 || gj >= NY || gi >= NX) ? 1.f : gld(ap1, (unsigned)((kk * NY + gj) * NX + gi));
     }
-    if (et < 2 * TY) {
-      if (et & 1) {
-        const int gi = i0 + TX, gj = j0 + (et >> 1);
+    if (etv < 2 * TY) {
+      if (etv & 1) {
+        const int gi = i0 + TX, gj = j0 + (etv >> 1);
         aS = (gi < 0 || gi >= NX || gj >= NY) ? 1.f : gld(ap0, (unsigned)((kk * NY + gj) * NX + gi));
       }
-    } else if (et < 2 * TY + 2 * TX) {
-      const int u = et - 2 * TY;
+    } else if (etv < 2 * TY + 2 * TX) {
+      const int u = etv - 2 * TY;
       if (u & 1) {
         const int gi = i0 + (u >> 1), gj = j0 + TY;
         aS = (gj < 0 || gj >= NY || gi >= NX) ? 1.f : gld(ap1, (unsigned)((kk * NY + gj) * NX + gi));
@@ -289,6 +312,9 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
 
   for (int k = k0; k < k1; k++) {
     if constexpr (kPrio) progress_prio(k - k0, k1 - k0);
+    keep_vgpr(tidv); keep_vgpr(etv);
+#pragma unroll
+    for (int p = 0; p < NHP; p++) { keep_vgpr(hoffv[p]); keep_vgpr(hinv[p]); keep_vgpr(hmirv[p]); }
     // ---------------- load phase
     Tz[3] = load_prim(cv, kc, qin, ns, own ? i : -1, j, k + 2, Pz[3]);
     const float A2z = apz(k + 1);                   // face k+3/2
@@ -300,23 +326,23 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
     // load_prim), then the prefetch of plane k + 1
 #pragma unroll
     for (int p = 0; p < NHP; p++) {
-      if (hoff[p] < 0) continue;
+      if (hoffv[p] < 0) continue;
       float w[5];
       prim_fast(ha[p][0], ha[p][1], ha[p][2], ha[p][3], ha[p][4], w);
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        const float x = !hin[p] ? kc.Fw[v] : w[v];
-        (&sm.P[v][0][0])[hoff[p]] = (v == 2 && hin[p] && hmir[p]) ? -x : x;
+        const float x = !hinv[p] ? kc.Fw[v] : w[v];
+        (&sm.P[v][0][0])[hoffv[p]] = (v == 2 && hinv[p] && hmirv[p]) ? -x : x;
       }
-      (&sm.T[0][0])[hoff[p]] = (int8_t)(!hin[p] ? -1 : (hmir[p] ? -2 : ht[p]));
+      (&sm.T[0][0])[hoffv[p]] = (int8_t)(!hinv[p] ? -1 : (hmirv[p] ? -2 : ht[p]));
     }
     if (k + 1 < k1) halo_load(k + 1);
     // apertures: own x / y faces, plus the two faces left of / below the tile
     const unsigned qk = (unsigned)(k * NXY) + qc;
     sm.A0[ty][tx + 2] = aX;
     sm.A1[ty + 2][tx] = aY;
-    if (tid < 2 * TY) sm.A0[tid >> 1][(tid & 1)] = aE;
-    else if (tid >= 64 && tid < 64 + 2 * TX) sm.A1[(tid - 64) & 1][(tid - 64) >> 1] = aE;
+    if (tidv < 2 * TY) sm.A0[tidv >> 1][(tidv & 1)] = aE;
+    else if (tidv >= 64 && tidv < 64 + 2 * TX) sm.A1[(tidv - 64) & 1][(tidv - 64) >> 1] = aE;
     const float aSk = aS;
     if (k + 1 < k1) ap_load(k + 1);
     __syncthreads();
@@ -339,7 +365,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       for (int v = 0; v < 5; v++) sm.Sy[v][ty + 1][tx] = s[v];
     }
     // the tile-edge slopes go to waves 2.. (waves 0 and 1 take the tile-edge fluxes)
-    for (int e = et; e < 2 * TY + 2 * TX; e += TX * TY) {
+    for (int e = etv; e < 2 * TY + 2 * TX; e += TX * TY) {
       float P0[5], Pm[5], Pp[5], s[5];
       if (e < 2 * TY) {                                 // x slope of cell -1 or TX in row r
         const int r = e >> 1, cx = (e & 1) ? TX : -1, xi = cx + 2, yj = r + 2;
@@ -388,16 +414,16 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Fy[v][ty + 1][tx] = F[v];
     }
-    if (tid < TX) {                                    // y face below row 0, column tid
-      const int c = tid, xi = c + 2;
+    if (tidv < TX) {                                    // y face below row 0, column tid
+      const int c = tidv, xi = c + 2;
       float PL[5], PR[5], sL[5], sR[5], F[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) { PL[v] = sm.P[v][1][xi]; PR[v] = sm.P[v][2][xi]; sL[v] = sm.Sy[v][0][c]; sR[v] = sm.Sy[v][1][c]; }
       face_flux<1>(PL, sL, sm.T[1][xi], PR, sR, sm.T[2][xi], sm.A1[1][c], F);
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Fy[v][0][c] = F[v];
-    } else if (tid >= TX && tid < TX + TY) {           // x face left of column 0, row r
-      const int r = tid - TX, yj = r + 2;
+    } else if (tidv >= TX && tidv < TX + TY) {           // x face left of column 0, row r
+      const int r = tidv - TX, yj = r + 2;
       float PL[5], PR[5], sL[5], sR[5], F[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) { PL[v] = sm.P[v][yj][1]; PR[v] = sm.P[v][yj][2]; sL[v] = sm.Sx[v][r][0]; sR[v] = sm.Sx[v][r][1]; }
