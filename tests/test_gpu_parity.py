@@ -150,6 +150,29 @@ def test_device_reductions_match_oracle_on_same_state():
     del Q0
 
 
+def test_force_list_replaced():
+    """A force list can be replaced on a live solver by a shorter and then a
+    longer one (the device buffers are freed and allocated again); the device
+    sums then cover exactly the new list, to summation-order rounding."""
+    g = Golden("transport60k")
+    F, fix = g.s["F"], g.s["fix"]
+    s = make_hip(g, F, fix, 0.5)
+    s.step(5)
+    Q = s.read()
+    fl = g.force_list()
+    r = g.s["ref"]
+    args = (g.s["alpha"], g.s["M"], r["S"], r["xcg"], 0, r["c"])
+    half = {k: np.ascontiguousarray(v[: len(v) // 2]) for k, v in fl.items()}
+    for lst in (fl, half, fl):
+        s.set_force_list(lst)
+        f = s.forces(*args)
+        fo = O.forces(Q, lst, *args)
+        say("force list", len(lst["q"]), f, fo)
+        for k in ("CL", "CD", "Cm"):
+            assert rel(f[k], fo[k]) < 1e-10, (len(lst["q"]), k, f[k], fo[k])
+    s.dispose()
+
+
 def test_write_read_roundtrip_and_step_zero():
     g = Golden("sphere")
     F = g.s["F"]
