@@ -22,12 +22,61 @@ i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 EXPORTS = ["aero3d_create", "aero3d_destroy", "aero3d_last_error", "aero3d_set_free", "aero3d_set_fix",
            "aero3d_set_opt", "aero3d_reset", "aero3d_step", "aero3d_iter", "aero3d_read", "aero3d_write",
            "aero3d_sync", "aero3d_set_force_list", "aero3d_forces", "aero3d_trefftz", "aero3d_residual",
-           "aero3d_time_steps", "aero3d_set_tile", "aero3d_get_tile", "aero3d_autotune", "aero3d_footprint"]
+           "aero3d_time_steps", "aero3d_set_tile", "aero3d_get_tile", "aero3d_autotune", "aero3d_footprint",
+           "aero3d_readback"]
 
 
 class Grid(C.Structure):
     _fields_ = [("NX", C.c_int), ("NY", C.c_int), ("NZ", C.c_int),
                 ("fx", C.POINTER(C.c_double)), ("fy", C.POINTER(C.c_double)), ("fz", C.POINTER(C.c_double))]
+
+
+class ReadbackIn(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("alpha", "M", "S", "xcg", "zcg", "cref", "xT")]
+
+
+class ReadbackOut(C.Structure):
+    _fields_ = [("it", C.c_int)] + [(k, C.c_double) for k in ("res", "CL", "CD", "Cm", "CDi", "Mmax")]
+
+
+RULE_RES_DROP, RULE_REFERENCE = 0, 1
+
+
+class ConvRule:
+    """Convergence rules of a monitored solve, one update per readback: the
+    host mirror of the device rules of the served form.
+
+    RULE_RES_DROP: converged at res / res0 <= res_drop (res0: the first
+    positive residual), done then or at `cap` iterations. RULE_REFERENCE: the
+    reference controller's rule (CFD3.readback, src/air_cfd.js:573-576): more
+    than six readbacks, res / res0 < 2e-3 and the lift coefficients of the last
+    four readbacks within 2e-3; done then or past `cap` iterations. Both end a
+    solve whose residual is not finite after its first readback (diverged)."""
+
+    def __init__(self, rule=RULE_REFERENCE, res_drop=1e-4, cap=30000):
+        self.rule, self.res_drop, self.cap = rule, res_drop, cap
+        self.res0 = float("nan")
+        self.cl = []
+        self.nreads = 0
+        self.conv = self.diverged = self.done = False
+
+    def update(self, res, CL, it):
+        import math
+        self.nreads += 1
+        if not (self.res0 > 0) and res > 0:
+            self.res0 = res
+        self.cl = (self.cl + [CL])[-4:]
+        self.diverged = self.nreads > 1 and not math.isfinite(res)
+        if self.rule == RULE_REFERENCE:
+            ok = self.nreads > 6 and self.res0 > 0 and res / self.res0 < 2e-3
+            if ok:
+                ok = not any(c != c for c in self.cl) and max(self.cl) - min(self.cl) < 2e-3
+            self.conv = ok
+            self.done = self.conv or self.diverged or it > self.cap
+        else:
+            self.conv = self.res0 > 0 and res / self.res0 <= self.res_drop
+            self.done = self.conv or self.diverged or it >= self.cap
+        return self.done
 
 
 _lib = None
@@ -63,6 +112,7 @@ def lib():
         L.aero3d_set_tile.argtypes = [vp, C.c_int, C.c_int]
         L.aero3d_get_tile.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.aero3d_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
+        L.aero3d_readback.argtypes = [vp, C.POINTER(ReadbackIn), C.POINTER(ReadbackOut)]
         _lib = L
     return _lib
 
@@ -169,6 +219,30 @@ class CfdHIP:
         self._chk(lib().aero3d_residual(self.h, C.byref(v)))
         return v.value
 
+    def readback(self, alpha, M, S, xcg, zcg, cref, xT=float("nan")):
+        """CFD3.readback on the device: residual since the previous readback,
+        coefficients, Trefftz CDi at xT (NaN: none), Mach maximum."""
+        o = ReadbackOut()
+        self._chk(lib().aero3d_readback(self.h, C.byref(ReadbackIn(alpha, M, S, xcg, zcg, cref, xT)), C.byref(o)))
+        return {k: getattr(o, k) for k, _ in ReadbackOut._fields_}
+
+    def solve(self, ref, rule, every=50, xT=float("nan")):
+        """Step to convergence under `rule` (a ConvRule), one readback at
+        iteration 0 and every `every` iterations (and at the cap of the
+        residual-drop rule); returns the readback history. ref: (alpha, M, S,
+        xcg, zcg, cref)."""
+        hist = [self.readback(*ref, xT)]
+        rule.update(hist[-1]["res"], hist[-1]["CL"], hist[-1]["it"])
+        while not rule.done:
+            it = self.it
+            nxt = (it // every + 1) * every
+            if rule.rule == RULE_RES_DROP:
+                nxt = min(nxt, rule.cap)
+            self.step(nxt - it)
+            hist.append(self.readback(*ref, xT))
+            rule.update(hist[-1]["res"], hist[-1]["CL"], hist[-1]["it"])
+        return hist
+
     def time_steps(self, n):
         self._sync_opt()
         ms = C.c_float()
@@ -202,12 +276,13 @@ class CfdHIP:
 class Request(C.Structure):
     _fields_ = [("geometry", C.c_int), ("F", C.c_double * 5), ("fix", C.c_double * 15), ("cfl", C.c_double),
                 ("order", C.c_int), ("alpha", C.c_double), ("M", C.c_double), ("S", C.c_double), ("xcg", C.c_double),
-                ("zcg", C.c_double), ("cref", C.c_double), ("res_drop", C.c_double), ("max_iters", C.c_int),
-                ("arrival", C.c_double), ("tag", C.c_int64)]
+                ("zcg", C.c_double), ("cref", C.c_double), ("rule", C.c_int), ("res_drop", C.c_double), ("max_iters", C.c_int),
+                ("arrival", C.c_double), ("tag", C.c_int64), ("state_out", C.c_void_p)]
 
 
 class Result(C.Structure):
-    _fields_ = [("tag", C.c_int64), ("iters", C.c_int), ("converged", C.c_int), ("res0", C.c_double), ("res", C.c_double),
+    _fields_ = [("tag", C.c_int64), ("iters", C.c_int), ("converged", C.c_int), ("diverged", C.c_int), ("failed", C.c_int),
+                ("nreads", C.c_int), ("res0", C.c_double), ("res", C.c_double),
                 ("CL", C.c_double), ("CD", C.c_double), ("Cm", C.c_double), ("arrival", C.c_double),
                 ("t_admit", C.c_double), ("t_done", C.c_double)]
 
@@ -257,7 +332,7 @@ def blib():
         L.aero3d_server_submit.argtypes = [vp, C.POINTER(Request)]
         L.aero3d_server_tick.argtypes = [vp, pi, pi]
         L.aero3d_server_poll.argtypes = [vp, C.POINTER(Result), ci, pi]
-        L.aero3d_server_stats.argtypes = [vp, pd, p64]
+        L.aero3d_server_stats.argtypes = [vp, pd, pd, p64]
         _blib = L
     return _blib
 
@@ -306,6 +381,9 @@ class Batch:
                                                    *[a(fl[k], np.float64) for k in ("cx", "cz", "x", "z")], len(fl["q"]), C.byref(gid)))
         self.N[gid.value] = g.NX * g.NY * g.NZ
         return gid.value
+
+    def remove_geometry(self, gid):
+        self._chk(blib().aero3d_batch_remove_geometry(self.h, gid))
 
     def case_bytes(self, gid):
         v = C.c_int64()
@@ -380,6 +458,7 @@ class Server:
             raise Aero3DError(f"aero3d_server_create failed ({rc})")
         self.h = h
         self.batch = Batch(handle=C.c_void_p(blib().aero3d_server_batch(h)))
+        self._keep_out = {}
 
     def dispose(self):
         if getattr(self, "h", None):
@@ -397,14 +476,19 @@ class Server:
         self._chk(blib().aero3d_server_clock(self.h, C.byref(v)))
         return v.value
 
-    def submit(self, gid, F, fix, cfl, order, force_ref, res_drop, max_iters, arrival, tag):
+    def submit(self, gid, F, fix, cfl, order, force_ref, res_drop, max_iters, arrival, tag, rule=RULE_RES_DROP, state_out=None):
+        """state_out: optional float32 array of 5 x N that receives the final state."""
         r = Request()
         r.geometry = gid
         r.F[:] = list(F)
         r.fix[:] = list(np.asarray(fix, np.float64).reshape(15))
         r.cfl, r.order = cfl, order
         r.alpha, r.M, r.S, r.xcg, r.zcg, r.cref = force_ref
-        r.res_drop, r.max_iters, r.arrival, r.tag = res_drop, max_iters, arrival, tag
+        r.rule, r.res_drop, r.max_iters, r.arrival, r.tag = rule, res_drop, max_iters, arrival, tag
+        if state_out is not None:
+            assert state_out.dtype == np.float32 and state_out.flags["C_CONTIGUOUS"] and state_out.size == 5 * self.batch.N[gid]
+            self._keep_out[tag] = state_out
+            r.state_out = state_out.ctypes.data
         self._chk(blib().aero3d_server_submit(self.h, C.byref(r)))
 
     def tick(self):
@@ -416,12 +500,16 @@ class Server:
         buf = (Result * max_n)()
         n = C.c_int()
         self._chk(blib().aero3d_server_poll(self.h, buf, max_n, C.byref(n)))
-        return [{f: getattr(buf[i], f) for f, _ in Result._fields_} for i in range(n.value)]
+        out = [{f: getattr(buf[i], f) for f, _ in Result._fields_} for i in range(n.value)]
+        for r in out:
+            self._keep_out.pop(r["tag"], None)
+        return out
 
     def stats(self):
-        b, c = C.c_double(), C.c_int64()
-        self._chk(blib().aero3d_server_stats(self.h, C.byref(b), C.byref(c)))
-        return b.value, c.value
+        """(wall seconds with resident work, device seconds of the ticks, cell-updates of retired cases)"""
+        b, g, c = C.c_double(), C.c_double(), C.c_int64()
+        self._chk(blib().aero3d_server_stats(self.h, C.byref(b), C.byref(g), C.byref(c)))
+        return b.value, g.value, c.value
 
 
 # ---------------------------------------------------------------------------
@@ -444,7 +532,7 @@ AXI_EXPORTS = ["axi_create", "axi_destroy", "axi_last_error", "axi_set_opt", "ax
 class AxiMonitorOut(C.Structure):
     _fields_ = [("F", C.c_double), ("Fb", C.c_double), ("Fw", C.c_double), ("mfuel", C.c_double), ("pw", C.c_double * 8),
                 ("res_sum2", C.c_double), ("res_n", C.c_double), ("res_valid", C.c_int), ("nsec", C.c_int),
-                ("sec_mdot", C.c_double * 4), ("sec_w", C.c_double * 4), ("sec_wTt", C.c_double * 4)]
+                ("sec_mdot", C.c_double * 4), ("sec_w", C.c_double * 4), ("sec_wTt", C.c_double * 4), ("bad", C.c_int)]
 _alib = None
 
 
@@ -574,7 +662,7 @@ class AxiHIP:
         self._chk(alib().axi_monitor(self.h, np.ascontiguousarray(comps, np.float64), P0, C.byref(o)))
         n = o.nsec
         return {"F": o.F, "Fb": o.Fb, "Fw": o.Fw, "mfuel": o.mfuel, "pw": list(o.pw), "res_sum2": o.res_sum2,
-                "res_n": o.res_n, "res_valid": bool(o.res_valid),
+                "res_n": o.res_n, "res_valid": bool(o.res_valid), "bad": o.bad,
                 "sec": [{"mdot": o.sec_mdot[q], "w": o.sec_w[q], "wTt": o.sec_wTt[q]} for q in range(n)]}
 
 EXPORTS = EXPORTS + AXI_EXPORTS

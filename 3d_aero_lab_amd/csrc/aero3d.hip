@@ -61,6 +61,7 @@ struct aero3d_solver {
   int32_t *fq = nullptr;
   double *fcx = nullptr, *fcz = nullptr, *fx = nullptr, *fz = nullptr;
   double *partials = nullptr, *sums = nullptr;
+  RbSeg *dseg = nullptr;      // device copy of this solve's readback segment
   std::string err;
   int64_t bytes = 0;
 };
@@ -149,10 +150,6 @@ __global__ void fixcells_kernel(float *__restrict__ dst, size_t ns, int N, const
   for (int c = 0; c < 5; c++) dst[c * ns + q] = f.q[g][c];
 }
 
-__global__ void copy_kernel(float *__restrict__ dst, const float *__restrict__ src, int N) {
-  const int q = blockIdx.x * kBlock + threadIdx.x;
-  if (q < N) dst[q] = src[q];
-}
 
 // ---------------------------------------------------------------------------
 // host helpers
@@ -247,6 +244,56 @@ int enqueue_steps(aero3d_solver *s, int n) {
 }
 
 
+// Readback sums of the current state through the shared segment kernels:
+// the residual since the snapshot (then the snapshot is retaken) when `res`,
+// the force-list sums and coefficients when `frc`. o[] gets kRbOut values
+// (plus the Trefftz sum and squared Mach maximum when `extra`); waits.
+int readback_sums(aero3d_solver *s, bool res, bool frc, double alphaDeg, double M, double S, double xcg, double zcg, double cref,
+                  double *o, bool extra = false) {
+  RbSeg g{};
+  g.Q = buf(s, s->cur); g.ns = s->ns; g.t = s->t; g.N = s->N; g.rsnap = s->rsnap;
+  g.nit = (s->snap_it >= 0 && s->it > s->snap_it) ? s->it - s->snap_it : 0;
+  g.fq = s->fq; g.cx = s->fcx; g.cz = s->fcz; g.x = s->fx; g.z = s->fz; g.nf = frc ? s->nf : 0;
+  g.xcg = xcg; g.zcg = zcg;
+  const double a = alphaDeg * (M_PI / 180);
+  g.ca = std::cos(a); g.sa = std::sin(a); g.kq = 2 / (0.5 * M * M * S); g.cref = cref;
+  g.nb_res = res ? reduce_blocks((s->N + 2) / 3) : 0;
+  g.nb_f = frc ? reduce_blocks(s->nf) : 0;
+  g.blk0 = 0; g.slot = 0; g.it = s->it;
+  g.part = s->partials; g.out = s->sums;
+  HIPCHK(s, hipMemcpyAsync(s->dseg, &g, sizeof g, hipMemcpyHostToDevice, s->stream));
+  if (g.nb_res + g.nb_f > 0) rb_partials_kernel<<<g.nb_res + g.nb_f, kRedBlock, 0, s->stream>>>(s->dseg, 1);
+  rb_finish_kernel<<<1, kRedBlock, 0, s->stream>>>(s->dseg);
+  HIPCHK(s, hipGetLastError());
+  HIPCHK(s, hipMemcpyAsync(o, s->sums, sizeof(double) * (kRbOut + (extra ? 2 : 0)), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  if (res) s->snap_it = s->it;
+  if (!res) o[5] = NAN;
+  return 0;
+}
+
+// Trefftz plane sum at xT into sums[kRbOut] (cfdLocate on the FP32 centres,
+// src/air_cfd.js:453, 560-569); enqueued.
+int enqueue_trefftz(aero3d_solver *s, double xT, double alphaDeg) {
+  const std::vector<float> &c = s->hxc;
+  const int n = s->NX;
+  int i0; double tx;
+  if (xT <= c[0]) { i0 = 0; tx = 0; }
+  else if (xT >= c[n - 1]) { i0 = n - 2; tx = 1; }
+  else {
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) { const int m = (lo + hi) >> 1; if (c[m] <= xT) lo = m; else hi = m; }
+    i0 = lo; tx = (xT - c[lo]) / ((double)c[hi] - c[lo]);
+  }
+  const double a = alphaDeg * (M_PI / 180);
+  const int nb = reduce_blocks(s->NY * s->NZ);
+  trefftz_kernel<<<nb, kRedBlock, 0, s->stream>>>(buf(s, s->cur), s->ns, s->t, s->dy, s->dz, s->NX, s->NY, s->NZ, i0, tx,
+                                                  std::cos(a), std::sin(a), s->partials + 4 * kRedMaxBlocks);
+  finish_kernel<1><<<1, kRedBlock, 0, s->stream>>>(s->partials + 4 * kRedMaxBlocks, nb, s->sums + kRbOut);
+  HIPCHK(s, hipGetLastError());
+  return 0;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -294,7 +341,7 @@ int aero3d_create(const aero3d_grid *g, const uint8_t *t, const float *phi, cons
   if ((rc = dalloc(s, &s->dx, s->NX)) || (rc = dalloc(s, &s->dy, s->NY)) || (rc = dalloc(s, &s->dz, s->NZ)) ||
       (rc = dalloc(s, &s->t, N)) || (rc = dalloc(s, &s->phi, N)) || (rc = dalloc(s, &s->ap, 4 * N)) ||
       (rc = dalloc(s, &s->state, 15 * s->ns)) || (rc = dalloc(s, &s->dt, N)) || (rc = dalloc(s, &s->rsnap, N)) ||
-      (rc = dalloc(s, &s->partials, 3 * kRedMaxBlocks)) || (rc = dalloc(s, &s->sums, 8)))
+      (rc = dalloc(s, &s->partials, 5 * kRedMaxBlocks)) || (rc = dalloc(s, &s->sums, 16)) || (rc = dalloc(s, &s->dseg, 1)))
     return bail(rc);
   auto up = [&](void *d, const void *h, size_t b) { return hipMemcpy(d, h, b, hipMemcpyHostToDevice); };
   if (up(s->dx, hdx.data(), 4 * s->NX) || up(s->dy, hdy.data(), 4 * s->NY) || up(s->dz, hdz.data(), 4 * s->NZ) ||
@@ -310,7 +357,8 @@ int aero3d_destroy(aero3d_solver *s) {
   if (!s) return 0;
   // teardown: an error here leaves nothing to recover, so return codes are not checked
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  void *ptrs[] = {s->dx, s->dy, s->dz, s->t, s->phi, s->ap, s->state, s->dt, s->rsnap, s->fq, s->fcx, s->fcz, s->fx, s->fz, s->partials, s->sums};
+  void *ptrs[] = {s->dx, s->dy, s->dz, s->t, s->phi, s->ap, s->state, s->dt, s->rsnap, s->fq, s->fcx, s->fcz, s->fx, s->fz, s->partials, s->sums,
+                  s->dseg};
   for (void *p : ptrs) if (p) (void)hipFree(p);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -410,43 +458,18 @@ int aero3d_set_force_list(aero3d_solver *s, const int32_t *q, const double *cx, 
 int aero3d_forces(aero3d_solver *s, double alphaDeg, double M, double S, double xcg, double zcg, double cref, double out[3]) {
   if (!s || !out) return AERO3D_E_ARG;
   if (!s->fq) return fail(s, AERO3D_E_STATE, "forces: no force list set");
-  const int nb = reduce_blocks(s->nf);
-  forces_kernel<<<nb, kRedBlock, 0, s->stream>>>(buf(s, s->cur), s->ns, s->fq, s->fcx, s->fcz, s->fx, s->fz, s->nf, xcg, zcg, s->partials);
-  finish_kernel<3><<<1, kRedBlock, 0, s->stream>>>(s->partials, nb, s->sums);
-  HIPCHK(s, hipGetLastError());
-  double F3[3];
-  HIPCHK(s, hipMemcpyAsync(F3, s->sums, sizeof F3, hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(s, hipStreamSynchronize(s->stream));
-  // wind-axis rotation and half-model doubling (src/air_cfd.js:450-451)
-  const double a = alphaDeg * (M_PI / 180), q = 0.5 * M * M, k = 2 / (q * S);
-  out[0] = k * (F3[1] * std::cos(a) - F3[0] * std::sin(a));
-  out[1] = k * (F3[0] * std::cos(a) + F3[1] * std::sin(a));
-  out[2] = k * F3[2] / cref;
+  double o[kRbOut];
+  if (int rc = readback_sums(s, false, true, alphaDeg, M, S, xcg, zcg, cref, o)) return rc;
+  out[0] = o[6]; out[1] = o[7]; out[2] = o[8];
   return 0;
 }
 
 int aero3d_trefftz(aero3d_solver *s, double xT, double alphaDeg, double M, double S, double *cdi) {
   if (!s || !cdi) return AERO3D_E_ARG;
   if (s->NX < 2) return fail(s, AERO3D_E_ARG, "trefftz: needs NX >= 2");
-  // cfdLocate on the FP32 centres (src/air_cfd.js:453)
-  const std::vector<float> &c = s->hxc;
-  const int n = s->NX;
-  int i0; double tx;
-  if (xT <= c[0]) { i0 = 0; tx = 0; }
-  else if (xT >= c[n - 1]) { i0 = n - 2; tx = 1; }
-  else {
-    int lo = 0, hi = n - 1;
-    while (hi - lo > 1) { const int m = (lo + hi) >> 1; if (c[m] <= xT) lo = m; else hi = m; }
-    i0 = lo; tx = (xT - c[lo]) / ((double)c[hi] - c[lo]);
-  }
-  const double a = alphaDeg * (M_PI / 180);
-  const int nb = reduce_blocks(s->NY * s->NZ);
-  trefftz_kernel<<<nb, kRedBlock, 0, s->stream>>>(buf(s, s->cur), s->ns, s->t, s->dy, s->dz, s->NX, s->NY, s->NZ, i0, tx,
-                                                  std::cos(a), std::sin(a), s->partials);
-  finish_kernel<1><<<1, kRedBlock, 0, s->stream>>>(s->partials, nb, s->sums);
-  HIPCHK(s, hipGetLastError());
+  if (int rc = enqueue_trefftz(s, xT, alphaDeg)) return rc;
   double sum;
-  HIPCHK(s, hipMemcpyAsync(&sum, s->sums, sizeof sum, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipMemcpyAsync(&sum, s->sums + kRbOut, sizeof sum, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(s, hipStreamSynchronize(s->stream));
   const double Mn = std::max(M, 0.05);
   *cdi = 2 * sum / (Mn * Mn * S);
@@ -455,22 +478,31 @@ int aero3d_trefftz(aero3d_solver *s, double xT, double alphaDeg, double M, doubl
 
 int aero3d_residual(aero3d_solver *s, double *res) {
   if (!s || !res) return AERO3D_E_ARG;
-  const float *r = buf(s, s->cur);
-  *res = NAN;
-  const int nIt = s->it - s->snap_it;
-  if (s->snap_it >= 0 && nIt > 0) {
-    const int nb = reduce_blocks((s->N + 2) / 3);
-    residual_kernel<<<nb, kRedBlock, 0, s->stream>>>(r, s->rsnap, s->t, s->N, s->partials);
-    finish_kernel<2><<<1, kRedBlock, 0, s->stream>>>(s->partials, nb, s->sums);
-    HIPCHK(s, hipGetLastError());
-    double v[2];
-    HIPCHK(s, hipMemcpyAsync(v, s->sums, sizeof v, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(s, hipStreamSynchronize(s->stream));
-    *res = std::sqrt(v[0] / std::max(1.0, v[1])) / nIt;
-  }
-  copy_kernel<<<(s->N + kBlock - 1) / kBlock, kBlock, 0, s->stream>>>(s->rsnap, r, s->N);
+  double o[kRbOut];
+  if (int rc = readback_sums(s, true, false, 0, 1, 1, 0, 0, 1, o)) return rc;
+  *res = o[5];
+  return 0;
+}
+
+int aero3d_readback(aero3d_solver *s, const aero3d_readback_in *in, aero3d_readback_out *out) {
+  if (!s || !in || !out) return fail(s, AERO3D_E_ARG, "readback: null argument");
+  const bool tz = !std::isnan(in->xT);
+  if (tz && s->NX < 2) return fail(s, AERO3D_E_ARG, "readback: the Trefftz plane needs NX >= 2");
+  const double Mn = std::max(in->M, 0.05);     // the readback's Mach floor (src/air_cfd.js:558, 561)
+  const int it = s->it;
+  // enqueued: Trefftz plane, Mach maximum, then the residual and forces (the residual retakes the snapshot)
+  if (tz) { if (int rc = enqueue_trefftz(s, in->xT, in->alpha)) return rc; }
+  const int nbm = reduce_blocks((s->N + 1) / 2);
+  machmax_kernel<<<nbm, kRedBlock, 0, s->stream>>>(buf(s, s->cur), s->ns, s->t, s->N, s->partials + 3 * kRedMaxBlocks);
+  machmax_finish<<<1, 64, 0, s->stream>>>(s->partials + 3 * kRedMaxBlocks, nbm, s->sums + kRbOut + 1);
   HIPCHK(s, hipGetLastError());
-  s->snap_it = s->it;
+  double o[kRbOut + 2];
+  if (int rc = readback_sums(s, true, s->fq != nullptr, in->alpha, Mn, in->S, in->xcg, in->zcg, in->cref, o, true)) return rc;
+  out->it = it;
+  out->res = o[5];
+  out->CL = s->fq ? o[6] : NAN; out->CD = s->fq ? o[7] : NAN; out->Cm = s->fq ? o[8] : NAN;
+  out->CDi = tz ? 2 * o[kRbOut] / (Mn * Mn * in->S) : NAN;
+  out->Mmax = std::sqrt(o[kRbOut + 1]);
   return 0;
 }
 
@@ -507,11 +539,13 @@ int aero3d_autotune(aero3d_solver *s, int iters, int *ty, int *zchunk, float *ms
   // candidates: both tile heights, z chunks from short (many blocks) to long (little preamble)
   const int tys[] = {4, 8};
   const int zcs[] = {0, -2, -3, 2, 3, 4, 6, 8};
-  // time on a scratch copy of the state: save the current state, restore after
-  const size_t b = sizeof(float) * 5 * s->ns;
+  // time on a scratch copy of the state: save the current state and the
+  // local dt of the last iteration, restore both after
+  const size_t b = sizeof(float) * 5 * s->ns, bdt = sizeof(float) * (size_t)s->N;
   float *save = nullptr;
-  HIPCHK(s, hipMalloc((void **)&save, b));
+  HIPCHK(s, hipMalloc((void **)&save, b + bdt));
   HIPCHK(s, hipMemcpyAsync(save, buf(s, s->cur), b, hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(s, hipMemcpyAsync(save + 5 * s->ns, s->dt, bdt, hipMemcpyDeviceToDevice, s->stream));
   const int cur0 = s->cur, it0 = s->it;
   int bty = s->tile_ty, bzc = s->tile_zc;
   float best = 1e30f;
@@ -528,6 +562,7 @@ int aero3d_autotune(aero3d_solver *s, int iters, int *ty, int *zchunk, float *ms
   s->tile_ty = bty; s->tile_zc = bzc;
   s->cur = cur0; s->it = it0;
   HIPCHK(s, hipMemcpyAsync(buf(s, s->cur), save, b, hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(s, hipMemcpyAsync(s->dt, save + 5 * s->ns, bdt, hipMemcpyDeviceToDevice, s->stream));
   HIPCHK(s, hipStreamSynchronize(s->stream));
   HIPCHK(s, hipFree(save));
   if (ty) *ty = bty;

@@ -8,7 +8,8 @@
 //     admitting and retiring cases never calls hipMalloc / hipFree;
 //   * a device case table lists, per active case, its CaseView, stage
 //     constants, state buffers and the range of work items (64 x TY x zc
-//     tile chunks) it owns;
+//     tile chunks) it owns; it is staged through pinned host buffers and
+//     copied asynchronously (no wait on the stream);
 //   * one launch per RK stage covers all work items of all cases of a group:
 //     by default one block per item (grid = the group's item count, XCD-
 //     grouped so neighbouring tiles share an L2), or, with
@@ -20,9 +21,14 @@
 //     when its launch grid (item count) changed, and never in the persistent
 //     form.
 // Server: a FIFO of solve requests admitted by HBM footprint into the batch
-// as memory frees up (continuous batching); each tick steps every resident
-// case by the check interval, measures each case's density residual on the
-// device, retires converged / capped cases with their force coefficients.
+// as memory frees up (continuous batching). Each tick is enqueued without
+// waiting: the admission readback of new cases, the steps to the next
+// readback of any case, then the readback of every case due (density
+// residual and forces through the shared segment reductions) and the
+// convergence rule, decided on the device: a finished case is frozen there
+// (its work items exit at once) until the host, which lags one tick behind,
+// retires it and frees its slot. The host keeps two ticks in flight and
+// waits only for the older one's results.
 // ============================================================================
 #include <hip/hip_runtime.h>
 
@@ -36,6 +42,7 @@
 #include <vector>
 
 #include "aero3d.h"
+#include "converge.hpp"
 #include "euler3d.hpp"
 #include "euler3d_tile.hpp"
 #include "reduce.hpp"
@@ -54,8 +61,17 @@ struct SlotDev {                    // one case in the device case table
   unsigned long long ns;
   int cur;                          // buffer holding Q at phase 0
   int ntx, nty, zc, item0, nitems;
+  int slot;                         // the case's status record
 };
-struct TableHdr { int nslots, total; };
+struct TableHdr { int nslots, total, base; };   // base: first entry of the group in the table
+
+// Device status of a slot: the served form's convergence state and last
+// readback; `frozen` makes the stage kernels skip the case.
+struct SlotStat {
+  ConvState cs;
+  int frozen, it;
+  double res, CL, CD, Cm;
+};
 
 // Stage kernel over the case table; a work item is (case, tile column, z
 // chunk). persist = 0: one item per block, the launch grid is the group's item
@@ -64,18 +80,21 @@ struct TableHdr { int nslots, total; };
 // the loop state stays live across the tile stage and spills to scratch, so on
 // 16 cases of the batched workload it runs 11.2 against 13.1 G cell-updates/s.
 // phase: iterations since the table's buffer indices were written (mod 3);
-// stage: 0 or 1 of SSP-RK2. At least four waves per SIMD (<= 128 VGPRs).
+// stage: 0 or 1 of SSP-RK2. A frozen case's items return at once. At least
+// four waves per SIMD (<= 128 VGPRs).
 template <int stage, int persist>
-__global__ void __launch_bounds__(64 * kTY) __attribute__((amdgpu_waves_per_eu(4))) batch_kernel(const TableHdr *__restrict__ hdr, const SlotDev *__restrict__ slots,
-                                                          int phase) {
+__global__ void __launch_bounds__(64 * kTY) __attribute__((amdgpu_waves_per_eu(4)))
+batch_kernel(const TableHdr *__restrict__ hdr, const SlotDev *__restrict__ table, const SlotStat *__restrict__ stat, int phase) {
   __shared__ TileSmem<64, kTY> sm;
   const int total = hdr->total, nsl = hdr->nslots;
+  const SlotDev *__restrict__ slots = table + hdr->base;
   const int G = gridDim.x, r = xcd_remap(blockIdx.x, G);
   auto run = [&](int item) {
     int lo = 0, hi = nsl - 1;
     while (lo < hi) { const int m = (lo + hi + 1) >> 1; if (slots[m].item0 <= item) lo = m; else hi = m - 1; }
     lo = __builtin_amdgcn_readfirstlane(lo);      // block-uniform: keep the case record in scalar registers
     const SlotDev &S = slots[lo];
+    if (__builtin_amdgcn_readfirstlane(stat[S.slot].frozen)) return;
     const int li = item - S.item0;
     const int bx = li % S.ntx, by = (li / S.ntx) % S.nty, bz = li / (S.ntx * S.nty);
     const int k0 = bz * S.zc, k1 = min(S.cv.NZ, k0 + S.zc);
@@ -111,9 +130,31 @@ __global__ void bfixcells_kernel(float *__restrict__ dst, size_t ns, int N, cons
   if (g < 0) return;
   for (int c = 0; c < 5; c++) dst[c * ns + q] = f.q[g][c];
 }
-__global__ void bcopy_kernel(float *__restrict__ dst, const float *__restrict__ src, int N) {
-  const int q = blockIdx.x * 256 + threadIdx.x;
-  if (q < N) dst[q] = src[q];
+// A new case's status record: no readback yet, not frozen.
+__global__ void stat_reset_kernel(SlotStat *__restrict__ stat, int slot) {
+  if (threadIdx.x != 0) return;
+  SlotStat &x = stat[slot];
+  conv_init(x.cs);
+  x.frozen = 0; x.it = 0;
+  x.res = x.CL = x.CD = x.Cm = NAN;
+}
+// Served form: final sums of each due case, then its convergence rule; a case
+// that is done is frozen. A case frozen earlier keeps its record.
+__global__ void __launch_bounds__(kRedBlock) srv_finish_kernel(const RbSeg *__restrict__ seg, SlotStat *__restrict__ stat) {
+  __shared__ double l2[kRedBlock / 64][2];
+  __shared__ double l3[kRedBlock / 64][3];
+  const RbSeg g = seg[blockIdx.x];
+  SlotStat &x = stat[g.slot];
+  if (__builtin_amdgcn_readfirstlane(x.frozen)) return;
+  finish_sum<2>(g.part, g.nb_res, g.out, l2);
+  finish_sum<3>(g.part + 2 * g.nb_res, g.nb_f, g.out + 2, l3);
+  if (threadIdx.x == 0) {
+    rb_results(g);
+    const double *o = g.out;
+    conv_update(x.cs, g.rule, g.res_drop, g.cap, o[5], o[6], g.it);
+    x.it = g.it; x.res = o[5]; x.CL = o[6]; x.CD = o[7]; x.Cm = o[8];
+    if (x.cs.done) x.frozen = 1;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -170,11 +211,22 @@ struct Slot {
   int geo = -1;
   void *mem = nullptr; size_t bytes = 0;
   float *state = nullptr, *rsnap = nullptr;
-  double *partials = nullptr;       // FP64 block partials and the slot's sums
+  double *partials = nullptr;       // FP64 readback partials (5 kRedMaxBlocks) and results (kRbOut)
   size_t ns = 0;
   int cur = 0, it = 0, snap_it = -1;
   StageConsts kc;
   int zc = 3;
+};
+
+// Pinned host staging buffers, reused in turn; each records an event after
+// the last copy that reads or writes it, and is waited for before reuse.
+struct PinnedRing {
+  static constexpr int kN = 6;
+  char *buf[kN] = {};
+  hipEvent_t ev[kN] = {};
+  bool held[kN] = {};
+  size_t cap = 0;
+  int next = 0;
 };
 
 }  // namespace
@@ -185,7 +237,9 @@ struct aero3d_batch {
   std::vector<Geometry> geos;
   std::vector<Slot> slots;
   TableHdr *d_hdr = nullptr;         // one header per case group
-  SlotDev *d_table = nullptr;        // max_slots entries per case group
+  SlotDev *d_table = nullptr;        // max_slots entries, groups one after another
+  SlotStat *d_stat = nullptr;        // max_slots status records
+  RbSeg *d_seg = nullptr;            // 2 x max_slots readback segments (admission, check)
   int max_slots = 0;
   int grid = 0;                      // resident blocks of the persistent form
   std::vector<hipGraph_t> graphs;    // per case group: three iterations
@@ -198,6 +252,7 @@ struct aero3d_batch {
   int ngroups = 0;
   int64_t group_cells = int64_t(1) << 40;  // cases are stepped in groups of about this many cells (default: one group)
   int zc = 3;
+  PinnedRing ring;
   std::string err;
 };
 
@@ -210,11 +265,33 @@ int bfail(aero3d_batch *b, int code, const std::string &m) { if (b) b->err = m; 
     if (e_ != hipSuccess) return bfail((b), AERO3D_E_HIP_BASE - (int)e_, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Take the next staging buffer (waiting for its previous copies if they are
+// still pending); *k = its index. The caller records ring.ev[*k] after its
+// last copy and releases it.
+int ring_acquire(aero3d_batch *b, int *k) {
+  PinnedRing &R = b->ring;
+  for (int n = 0; n < PinnedRing::kN; n++) {
+    const int i = (R.next + n) % PinnedRing::kN;
+    if (R.held[i]) continue;
+    BCHK(b, hipEventSynchronize(R.ev[i]));
+    R.held[i] = true;
+    R.next = (i + 1) % PinnedRing::kN;
+    *k = i;
+    return 0;
+  }
+  return bfail(b, AERO3D_E_STATE, "staging buffers exhausted");
+}
+int ring_release(aero3d_batch *b, int k) {
+  BCHK(b, hipEventRecord(b->ring.ev[k], b->stream));
+  b->ring.held[k] = false;
+  return 0;
+}
+
 size_t slot_bytes(const Geometry &g, size_t *ns_out) {
   const size_t ns = ((size_t)g.N + 63) / 64 * 64;
   if (ns_out) *ns_out = ns;
   auto al = [](size_t b) { return (b + kAlign - 1) / kAlign * kAlign; };
-  return al(4 * 15 * ns) + al(4 * (size_t)g.N) + al(8 * (3 * kRedMaxBlocks + 8));
+  return al(4 * 15 * ns) + al(4 * (size_t)g.N) + al(8 * (5 * kRedMaxBlocks + 16));
 }
 
 void prim5h(const double *a, double *o) {
@@ -223,19 +300,35 @@ void prim5h(const double *a, double *o) {
   o[4] = std::max((1.4 - 1) * (a[4] - 0.5 * r * (u * u + v * v + w * w)), 1e-7);
 }
 
-// Rewrite the device case tables from the active slots. Cases are split into
-// groups of about group_cells cells (a group's state and masks stay in the
-// Infinity Cache across its stages and iterations); each group has its own
-// header and table region, stepped by its own launches.
+// Byte offsets inside one staging buffer: table headers, table rows, two
+// segment areas, the slot status copy.
+struct StageLayout {
+  size_t hdr, rows, segA, segC, stat, total;
+  explicit StageLayout(int m) {
+    auto al = [](size_t x) { return (x + kAlign - 1) / kAlign * kAlign; };
+    hdr = 0; rows = al(sizeof(TableHdr) * m); segA = rows + al(sizeof(SlotDev) * m); segC = segA + al(sizeof(RbSeg) * m);
+    stat = segC + al(sizeof(RbSeg) * m); total = stat + al(sizeof(SlotStat) * m);
+  }
+};
+
+// Rewrite the device case table from the active slots, through a pinned
+// staging buffer (asynchronous). Cases are split into groups of about
+// group_cells cells (a group's state and masks stay in the Infinity Cache
+// across its stages and iterations); each group has its own header and a
+// contiguous range of the table, stepped by its own launches.
 int upload_table(aero3d_batch *b) {
   b->order.clear();
-  std::vector<std::vector<SlotDev>> groups(1);
+  std::vector<SlotDev> rows;
+  std::vector<TableHdr> hdrs;
   int64_t cells = 0;
   for (int s = 0; s < (int)b->slots.size(); s++) {
     Slot &S = b->slots[s];
     if (!S.used) continue;
     const Geometry &g = b->geos[S.geo];
-    if (!groups.back().empty() && cells + g.N > b->group_cells) { groups.emplace_back(); cells = 0; }
+    if (hdrs.empty() || (hdrs.back().nslots > 0 && cells + g.N > b->group_cells)) {
+      hdrs.push_back(TableHdr{0, 0, (int)rows.size()});
+      cells = 0;
+    }
     cells += g.N;
     SlotDev d;
     d.cv = CaseView{g.NX, g.NY, g.NZ, g.N, g.dx, g.dy, g.dz, g.t, g.phi, g.ap};
@@ -244,25 +337,27 @@ int upload_table(aero3d_batch *b) {
     d.zc = S.zc;
     d.ntx = (g.NX + 63) / 64; d.nty = (g.NY + kTY - 1) / kTY;
     d.nitems = d.ntx * d.nty * ((g.NZ + S.zc - 1) / S.zc);
-    d.item0 = 0;
-    groups.back().push_back(d);
+    d.item0 = hdrs.back().total;
+    d.slot = s;
+    hdrs.back().total += d.nitems;
+    hdrs.back().nslots++;
+    rows.push_back(d);
     b->order.push_back(s);
   }
-  if (groups.back().empty()) groups.pop_back();
-  b->ngroups = (int)groups.size();
-  std::vector<TableHdr> hdrs(groups.size());
-  for (size_t gi = 0; gi < groups.size(); gi++) {
-    int item = 0;
-    for (auto &d : groups[gi]) { d.item0 = item; item += d.nitems; }
-    hdrs[gi] = TableHdr{(int)groups[gi].size(), item};
-    BCHK(b, hipMemcpyAsync(b->d_table + gi * (size_t)b->max_slots, groups[gi].data(), sizeof(SlotDev) * groups[gi].size(),
-                           hipMemcpyHostToDevice, b->stream));
-  }
-  if (!hdrs.empty()) BCHK(b, hipMemcpyAsync(b->d_hdr, hdrs.data(), sizeof(TableHdr) * hdrs.size(), hipMemcpyHostToDevice, b->stream));
+  b->ngroups = (int)hdrs.size();
   b->totals.clear();
   for (auto &h : hdrs) b->totals.push_back(h.total);
-  // the host copies above must stay alive until the copies run: wait here (table changes are rare)
-  BCHK(b, hipStreamSynchronize(b->stream));
+  if (!rows.empty()) {
+    int k, rc;
+    if ((rc = ring_acquire(b, &k))) return rc;
+    char *p = b->ring.buf[k];
+    std::memcpy(p, hdrs.data(), sizeof(TableHdr) * hdrs.size());
+    const StageLayout L(b->max_slots);
+    std::memcpy(p + L.rows, rows.data(), sizeof(SlotDev) * rows.size());
+    BCHK(b, hipMemcpyAsync(b->d_hdr, p, sizeof(TableHdr) * hdrs.size(), hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipMemcpyAsync(b->d_table, p + L.rows, sizeof(SlotDev) * rows.size(), hipMemcpyHostToDevice, b->stream));
+    if ((rc = ring_release(b, k))) return rc;
+  }
   b->table_dirty = false;
   return 0;
 }
@@ -270,14 +365,13 @@ int upload_table(aero3d_batch *b) {
 // Both stages of one SSP-RK2 iteration of case group gi at rotation phase ph.
 void launch_iteration(aero3d_batch *b, int gi, int ph) {
   const TableHdr *h = b->d_hdr + gi;
-  const SlotDev *t = b->d_table + gi * (size_t)b->max_slots;
   if (b->persistent) {
-    batch_kernel<0, 1><<<b->grid, dim3(64, kTY), 0, b->stream>>>(h, t, ph);
-    batch_kernel<1, 1><<<b->grid, dim3(64, kTY), 0, b->stream>>>(h, t, ph);
+    batch_kernel<0, 1><<<b->grid, dim3(64, kTY), 0, b->stream>>>(h, b->d_table, b->d_stat, ph);
+    batch_kernel<1, 1><<<b->grid, dim3(64, kTY), 0, b->stream>>>(h, b->d_table, b->d_stat, ph);
   } else {
     const int G = std::max(1, b->totals[gi]);
-    batch_kernel<0, 0><<<G, dim3(64, kTY), 0, b->stream>>>(h, t, ph);
-    batch_kernel<1, 0><<<G, dim3(64, kTY), 0, b->stream>>>(h, t, ph);
+    batch_kernel<0, 0><<<G, dim3(64, kTY), 0, b->stream>>>(h, b->d_table, b->d_stat, ph);
+    batch_kernel<1, 0><<<G, dim3(64, kTY), 0, b->stream>>>(h, b->d_table, b->d_stat, ph);
   }
 }
 
@@ -321,32 +415,44 @@ int batch_steps(aero3d_batch *b, int n) {
 
 inline float *slot_Q(const Slot &S) { return S.state + (size_t)S.cur * 5 * S.ns; }
 
-// Enqueue the residual of slot S into its sums area (sums[0..1]); snapshot after.
-void enqueue_residual(aero3d_batch *b, Slot &S) {
+// The readback segment of slot s (residual when res, forces when frc). The
+// coefficient constants come from (alpha, M, S, cref) as in cfdForces.
+RbSeg make_seg(const aero3d_batch *b, int s, bool res, bool frc, double alphaDeg, double M, double Sref, double xcg, double zcg,
+               double cref) {
+  const Slot &S = b->slots[s];
   const Geometry &g = b->geos[S.geo];
-  double *partials = S.partials, *sums = S.partials + 3 * kRedMaxBlocks;
-  const float *r = slot_Q(S);
-  if (S.snap_it >= 0 && S.it > S.snap_it) {
-    const int nb = reduce_blocks((g.N + 2) / 3);
-    residual_kernel<<<nb, kRedBlock, 0, b->stream>>>(r, S.rsnap, g.t, g.N, partials);
-    finish_kernel<2><<<1, kRedBlock, 0, b->stream>>>(partials, nb, sums);
-  }
-  bcopy_kernel<<<(g.N + 255) / 256, 256, 0, b->stream>>>(S.rsnap, r, g.N);
+  RbSeg x{};
+  x.Q = slot_Q(S); x.ns = S.ns; x.t = g.t; x.N = g.N; x.rsnap = S.rsnap;
+  x.nit = (S.snap_it >= 0 && S.it > S.snap_it) ? S.it - S.snap_it : 0;
+  x.fq = g.fq; x.cx = g.fcx; x.cz = g.fcz; x.x = g.fx; x.z = g.fz; x.nf = frc ? g.nf : 0;
+  x.xcg = xcg; x.zcg = zcg;
+  const double a = alphaDeg * (M_PI / 180);
+  x.ca = std::cos(a); x.sa = std::sin(a); x.kq = 2 / (0.5 * M * M * Sref); x.cref = cref;
+  x.nb_res = res ? reduce_blocks((g.N + 2) / 3) : 0;
+  x.nb_f = frc ? reduce_blocks(g.nf) : 0;
+  x.slot = s; x.it = S.it;
+  x.part = S.partials; x.out = S.partials + 5 * kRedMaxBlocks;
+  return x;
 }
 
-void enqueue_forces(aero3d_batch *b, Slot &S, double xcg, double zcg) {
-  const Geometry &g = b->geos[S.geo];
-  double *partials = S.partials, *sums = S.partials + 3 * kRedMaxBlocks + 2;
-  const int nb = reduce_blocks(g.nf);
-  forces_kernel<<<nb, kRedBlock, 0, b->stream>>>(slot_Q(S), S.ns, g.fq, g.fcx, g.fcz, g.fx, g.fz, g.nf, xcg, zcg, partials);
-  finish_kernel<3><<<1, kRedBlock, 0, b->stream>>>(partials, nb, sums);
+// Stage segments in staging buffer area `p`, copy them to device segments
+// dseg and enqueue their partial sums (the caller enqueues the finish).
+int enqueue_segments(aero3d_batch *b, std::vector<RbSeg> &segs, char *p, RbSeg *dseg) {
+  int blk = 0;
+  for (auto &x : segs) { x.blk0 = blk; blk += x.nb_res + x.nb_f; }
+  std::memcpy(p, segs.data(), sizeof(RbSeg) * segs.size());
+  BCHK(b, hipMemcpyAsync(dseg, p, sizeof(RbSeg) * segs.size(), hipMemcpyHostToDevice, b->stream));
+  if (blk > 0) rb_partials_kernel<<<blk, kRedBlock, 0, b->stream>>>(dseg, (int)segs.size());
+  BCHK(b, hipGetLastError());
+  return 0;
 }
 
-void coefficients(const double *F3, double alphaDeg, double M, double S, double cref, double *out) {
-  const double a = alphaDeg * (M_PI / 180), q = 0.5 * M * M, k = 2 / (q * S);
-  out[0] = k * (F3[1] * std::cos(a) - F3[0] * std::sin(a));
-  out[1] = k * (F3[0] * std::cos(a) + F3[1] * std::sin(a));
-  out[2] = k * F3[2] / cref;
+void free_slot(aero3d_batch *b, int slot) {
+  Slot &S = b->slots[slot];
+  b->arena.put(S.mem, S.bytes);
+  b->geos[S.geo].refs--;
+  S = Slot();
+  b->table_dirty = true;
 }
 
 }  // namespace
@@ -357,16 +463,24 @@ void coefficients(const double *F3, double alphaDeg, double M, double S, double 
 extern "C" {
 
 int aero3d_batch_create(int64_t arena_bytes, int max_slots, aero3d_batch **out) {
-  if (!out || arena_bytes <= 0 || max_slots < 1) return AERO3D_E_ARG;
+  if (!out || arena_bytes <= 0 || max_slots < 1 || max_slots > 4096) return AERO3D_E_ARG;
   *out = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return AERO3D_E_NODEV;
   aero3d_batch *b = new aero3d_batch();
   b->max_slots = max_slots;
   b->slots.resize(max_slots);
-  if (!b->arena.init((size_t)arena_bytes) || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc((void **)&b->d_hdr, sizeof(TableHdr) * max_slots) != hipSuccess ||
-      hipMalloc((void **)&b->d_table, sizeof(SlotDev) * (size_t)max_slots * max_slots) != hipSuccess) {
+  const StageLayout L(max_slots);
+  bool ok = b->arena.init((size_t)arena_bytes) && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipMalloc((void **)&b->d_hdr, sizeof(TableHdr) * max_slots) == hipSuccess &&
+            hipMalloc((void **)&b->d_table, sizeof(SlotDev) * (size_t)max_slots) == hipSuccess &&
+            hipMalloc((void **)&b->d_stat, sizeof(SlotStat) * (size_t)max_slots) == hipSuccess &&
+            hipMalloc((void **)&b->d_seg, sizeof(RbSeg) * 2 * (size_t)max_slots) == hipSuccess;
+  b->ring.cap = L.total;
+  for (int k = 0; ok && k < PinnedRing::kN; k++)
+    ok = hipHostMalloc((void **)&b->ring.buf[k], L.total, hipHostMallocDefault) == hipSuccess &&
+         hipEventCreate(&b->ring.ev[k]) == hipSuccess;
+  if (!ok) {
     aero3d_batch_destroy(b);
     return AERO3D_E_HIP_BASE;
   }
@@ -388,8 +502,12 @@ int aero3d_batch_destroy(aero3d_batch *b) {
   for (auto e : b->execs) if (e) (void)hipGraphExecDestroy(e);
   for (auto g : b->graphs) if (g) (void)hipGraphDestroy(g);
   if (b->arena.base) (void)hipFree(b->arena.base);
-  if (b->d_hdr) (void)hipFree(b->d_hdr);
-  if (b->d_table) (void)hipFree(b->d_table);
+  void *dp[] = {b->d_hdr, b->d_table, b->d_stat, b->d_seg};
+  for (void *p : dp) if (p) (void)hipFree(p);
+  for (int k = 0; k < PinnedRing::kN; k++) {
+    if (b->ring.buf[k]) (void)hipHostFree(b->ring.buf[k]);
+    if (b->ring.ev[k]) (void)hipEventDestroy(b->ring.ev[k]);
+  }
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return 0;
@@ -425,6 +543,7 @@ int aero3d_batch_add_geometry(aero3d_batch *b, const aero3d_grid *g, const uint8
   for (int i = 0; i < G.NX; i++) { hdx[i] = (float)(g->fx[i + 1] - g->fx[i]); G.hxc[i] = (float)(0.5 * (g->fx[i] + g->fx[i + 1])); }
   for (int j = 0; j < G.NY; j++) G.hdy[j] = (float)(g->fy[j + 1] - g->fy[j]);
   for (int k = 0; k < G.NZ; k++) G.hdz[k] = (float)(g->fz[k + 1] - g->fz[k]);
+  // synchronous copies: the caller's arrays may change after return
   auto up = [&](void *d, const void *h, size_t n) { return hipMemcpy(d, h, n, hipMemcpyHostToDevice); };
   if (up(G.dx, hdx.data(), 4 * G.NX) || up(G.dy, G.hdy.data(), 4 * G.NY) || up(G.dz, G.hdz.data(), 4 * G.NZ) ||
       up(G.t, t, N) || up(G.phi, phi, 4 * N) || up(G.ap, ap0, 4 * N) || up(G.ap + N, ap1, 4 * N) || up(G.ap + 2 * N, ap2, 4 * N) ||
@@ -483,13 +602,14 @@ int aero3d_batch_add(aero3d_batch *b, int gid, const double F[5], const double f
   S.kc.cfl = (float)cfl; S.kc.order = order;
   S.zc = b->zc;
   G.refs++;
-  // reset(): state := F
+  // reset(): state := F; fixed cells := their states; status record cleared.
+  // Stream-ordered after every earlier launch, so memory of a retired case can be reused at once.
   bfill_kernel<<<(G.N + 255) / 256, 256, 0, b->stream>>>(S.state, ns, G.N, (float)F[0], (float)F[1], (float)F[2], (float)F[3], (float)F[4]);
-  BCHK(b, hipGetLastError());
   BFixStates fs;
   for (int gg = 0; gg < 3; gg++)
     for (int v = 0; v < 5; v++) fs.q[gg][v] = S.kc.fixq[gg][v];
   bfixcells_kernel<<<(G.N + 255) / 256, 256, 0, b->stream>>>(S.state, ns, G.N, G.t, fs);
+  stat_reset_kernel<<<1, 64, 0, b->stream>>>(b->d_stat, s);
   BCHK(b, hipGetLastError());
   b->table_dirty = true;
   *slot = s;
@@ -498,12 +618,7 @@ int aero3d_batch_add(aero3d_batch *b, int gid, const double F[5], const double f
 
 int aero3d_batch_remove(aero3d_batch *b, int slot) {
   if (!b || slot < 0 || slot >= b->max_slots || !b->slots[slot].used) return bfail(b, AERO3D_E_ARG, "remove: bad slot");
-  BCHK(b, hipStreamSynchronize(b->stream));
-  Slot &S = b->slots[slot];
-  b->arena.put(S.mem, S.bytes);
-  b->geos[S.geo].refs--;
-  S = Slot();
-  b->table_dirty = true;
+  free_slot(b, slot);
   return 0;
 }
 
@@ -559,37 +674,41 @@ int aero3d_batch_read(aero3d_batch *b, int slot, float *r, float *ru, float *rv,
 // Residuals of `n` slots (density RMS change per iteration since each slot's
 // last snapshot, src/air_cfd.js:553-554); NaN for a slot's first call.
 int aero3d_batch_residuals(aero3d_batch *b, const int *slots, int n, double *res) {
-  if (!b || n < 0 || (n > 0 && (!slots || !res))) return AERO3D_E_ARG;
-  std::vector<int> nit(n);
+  if (!b || n < 0 || n > b->max_slots || (n > 0 && (!slots || !res))) return AERO3D_E_ARG;
+  std::vector<RbSeg> segs;
   for (int k = 0; k < n; k++) {
     if (slots[k] < 0 || slots[k] >= b->max_slots || !b->slots[slots[k]].used) return bfail(b, AERO3D_E_ARG, "residuals: bad slot");
-    Slot &S = b->slots[slots[k]];
-    nit[k] = (S.snap_it >= 0 && S.it > S.snap_it) ? S.it - S.snap_it : 0;
-    enqueue_residual(b, S);
+    for (int j = 0; j < k; j++) if (slots[j] == slots[k]) return bfail(b, AERO3D_E_ARG, "residuals: slot listed twice");
+    segs.push_back(make_seg(b, slots[k], true, false, 0, 1, 1, 0, 0, 1));
   }
+  if (n == 0) return 0;
+  int k, rc;
+  if ((rc = ring_acquire(b, &k))) return rc;
+  const StageLayout L(b->max_slots);
+  char *p = b->ring.buf[k];
+  if ((rc = enqueue_segments(b, segs, p + L.segC, b->d_seg + b->max_slots))) return rc;
+  rb_finish_kernel<<<n, kRedBlock, 0, b->stream>>>(b->d_seg + b->max_slots);
   BCHK(b, hipGetLastError());
-  std::vector<double> v(2 * (size_t)n, 0.0);
-  for (int k = 0; k < n; k++)
-    if (nit[k]) BCHK(b, hipMemcpyAsync(&v[2 * k], b->slots[slots[k]].partials + 3 * kRedMaxBlocks, 16, hipMemcpyDeviceToHost, b->stream));
+  std::vector<double> v(n);
+  for (int j = 0; j < n; j++) BCHK(b, hipMemcpyAsync(&v[j], segs[j].out + 5, 8, hipMemcpyDeviceToHost, b->stream));
   BCHK(b, hipStreamSynchronize(b->stream));
-  for (int k = 0; k < n; k++) {
-    Slot &S = b->slots[slots[k]];
-    res[k] = nit[k] ? std::sqrt(v[2 * k] / std::max(1.0, v[2 * k + 1])) / nit[k] : NAN;
-    S.snap_it = S.it;
-  }
+  if ((rc = ring_release(b, k))) return rc;
+  for (int j = 0; j < n; j++) { res[j] = v[j]; b->slots[slots[j]].snap_it = b->slots[slots[j]].it; }
   return 0;
 }
 
 int aero3d_batch_forces(aero3d_batch *b, int slot, double alphaDeg, double M, double S_, double xcg, double zcg, double cref, double out[3]) {
   if (!b || !out || slot < 0 || slot >= b->max_slots || !b->slots[slot].used) return bfail(b, AERO3D_E_ARG, "forces: bad slot");
-  Slot &S = b->slots[slot];
-  enqueue_forces(b, S, xcg, zcg);
+  std::vector<RbSeg> segs{make_seg(b, slot, false, true, alphaDeg, M, S_, xcg, zcg, cref)};
+  int k, rc;
+  if ((rc = ring_acquire(b, &k))) return rc;
+  const StageLayout L(b->max_slots);
+  if ((rc = enqueue_segments(b, segs, b->ring.buf[k] + L.segC, b->d_seg + b->max_slots))) return rc;
+  rb_finish_kernel<<<1, kRedBlock, 0, b->stream>>>(b->d_seg + b->max_slots);
   BCHK(b, hipGetLastError());
-  double F3[3];
-  BCHK(b, hipMemcpyAsync(F3, S.partials + 3 * kRedMaxBlocks + 2, 24, hipMemcpyDeviceToHost, b->stream));
+  BCHK(b, hipMemcpyAsync(out, segs[0].out + 6, 24, hipMemcpyDeviceToHost, b->stream));
   BCHK(b, hipStreamSynchronize(b->stream));
-  coefficients(F3, alphaDeg, M, S_, cref, out);
-  return 0;
+  return ring_release(b, k);
 }
 
 int aero3d_batch_footprint(const aero3d_batch *b, int64_t *used, int64_t *capacity) {
@@ -620,31 +739,47 @@ int aero3d_batch_time_steps(aero3d_batch *b, int n, float *ms) {
 // ---------------------------------------------------------------------------
 namespace {
 
-struct Pending {
+struct SrvCase {                    // a resident request (host view)
+  bool live = false;
+  unsigned gen = 0;                 // admissions into this slot so far
   aero3d_request r;
-  double arrival;
+  double arrival = 0, t_admit = 0;
+  int last_read_it = 0;             // iterations at its last readback
 };
-struct Active {
-  aero3d_request r;
-  int slot;
-  double arrival, t_admit, res0 = NAN, res = NAN;
+struct Due { int slot; unsigned gen; int cur, it; };
+struct TickRec {                    // one enqueued tick
+  int buf;                          // staging buffer (segments, status copy)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<Due> due;
 };
+
+// A request's next readback: every check_every iterations from its admission,
+// and, under the residual-drop rule, at its iteration cap.
+int next_check(const aero3d_request &r, int it, int every) {
+  const int n = (it / every + 1) * every;
+  return r.rule == kRuleResDrop ? std::min(n, r.max_iters) : n;
+}
 
 }  // namespace
 
 struct aero3d_server {
   aero3d_batch *b = nullptr;
   int check_every = 50;
-  std::deque<Pending> pending;
-  std::vector<Active> active;
+  std::deque<aero3d_request> pending;
+  std::vector<SrvCase> cases;       // by slot
+  int live = 0;
   std::vector<aero3d_result> done;
+  std::deque<TickRec> inflight;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> evpool;
   std::chrono::steady_clock::time_point epoch;
-  double busy_s = 0;
+  double busy_s = 0, gpu_s = 0;
   int64_t cell_updates = 0;
-  std::string err;
+  double t_busy0 = -1;              // start of the current stretch with resident work
 };
 
 namespace {
+
+constexpr int kInFlight = 2;        // ticks enqueued ahead of the host's processing
 
 double clock_s(const aero3d_server *s) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->epoch).count();
@@ -658,6 +793,58 @@ int64_t free_when_idle(const aero3d_batch *b) {
   return (int64_t)b->arena.cap - (int64_t)geo;
 }
 
+aero3d_result failed_result(const aero3d_server *s, const aero3d_request &r, double t) {
+  aero3d_result o;
+  std::memset(&o, 0, sizeof o);
+  o.tag = r.tag; o.failed = 1;
+  o.res0 = o.res = o.CL = o.CD = o.Cm = NAN;
+  o.arrival = r.arrival; o.t_admit = o.t_done = t;
+  (void)s;
+  return o;
+}
+
+// Results of the oldest enqueued tick: retire every due case the device froze.
+int process_oldest(aero3d_server *s) {
+  aero3d_batch *b = s->b;
+  TickRec T = s->inflight.front();
+  s->inflight.pop_front();
+  BCHK(b, hipEventSynchronize(T.ev1));
+  float ms = 0;
+  BCHK(b, hipEventElapsedTime(&ms, T.ev0, T.ev1));
+  s->gpu_s += ms * 1e-3;
+  s->evpool.push_back({T.ev0, T.ev1});
+  const StageLayout L(b->max_slots);
+  const SlotStat *st = (const SlotStat *)(b->ring.buf[T.buf] + L.stat);
+  const double now = clock_s(s);
+  for (const Due &d : T.due) {
+    SrvCase &c = s->cases[d.slot];
+    if (!c.live || c.gen != d.gen) continue;     // retired at an earlier tick
+    const SlotStat &x = st[d.slot];
+    if (!x.frozen) continue;
+    aero3d_result o;
+    std::memset(&o, 0, sizeof o);
+    o.tag = c.r.tag; o.iters = x.it; o.converged = x.cs.conv; o.diverged = x.cs.diverged; o.nreads = x.cs.nreads;
+    o.res0 = x.cs.res0; o.res = x.res; o.CL = x.CL; o.CD = x.CD; o.Cm = x.Cm;
+    o.arrival = c.arrival; o.t_admit = c.t_admit; o.t_done = now;
+    if (c.r.state_out) {            // the frozen state, still in buffer d.cur (stream-ordered copy, then a wait)
+      const Slot &S = b->slots[d.slot];
+      const size_t N = b->geos[S.geo].N;
+      const float *Q = S.state + (size_t)d.cur * 5 * S.ns;
+      for (int v = 0; v < 5; v++)
+        BCHK(b, hipMemcpyAsync(c.r.state_out + v * N, Q + v * S.ns, 4 * N, hipMemcpyDeviceToHost, b->stream));
+      BCHK(b, hipStreamSynchronize(b->stream));
+    }
+    s->cell_updates += (int64_t)x.it * b->geos[b->slots[d.slot].geo].N;
+    free_slot(b, d.slot);
+    c.live = false;
+    s->live--;
+    s->done.push_back(o);
+  }
+  b->ring.held[T.buf] = false;       // its event (T.ev1's copy) is already complete
+  if (s->live == 0 && s->t_busy0 >= 0) { s->busy_s += now - s->t_busy0; s->t_busy0 = -1; }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -669,6 +856,7 @@ int aero3d_server_create(int64_t arena_bytes, int max_slots, int check_every, ae
   int rc = aero3d_batch_create(arena_bytes, max_slots, &s->b);
   if (rc) { delete s; return rc; }
   s->check_every = check_every;
+  s->cases.resize(max_slots);
   s->epoch = std::chrono::steady_clock::now();
   *out = s;
   return 0;
@@ -676,6 +864,9 @@ int aero3d_server_create(int64_t arena_bytes, int max_slots, int check_every, ae
 
 int aero3d_server_destroy(aero3d_server *s) {
   if (!s) return 0;
+  if (s->b && s->b->stream) (void)hipStreamSynchronize(s->b->stream);
+  for (auto &T : s->inflight) s->evpool.push_back({T.ev0, T.ev1});
+  for (auto &e : s->evpool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
   aero3d_batch_destroy(s->b);
   delete s;
   return 0;
@@ -691,91 +882,133 @@ int aero3d_server_clock(const aero3d_server *s, double *now) {
 
 int aero3d_server_submit(aero3d_server *s, const aero3d_request *r) {
   if (!s || !r) return AERO3D_E_ARG;
-  if (r->geometry < 0 || r->geometry >= (int)s->b->geos.size() || s->b->geos[r->geometry].dead) return AERO3D_E_ARG;
-  if (r->max_iters < 1 || !(r->res_drop > 0)) return AERO3D_E_ARG;
+  aero3d_batch *b = s->b;
+  if (r->geometry < 0 || r->geometry >= (int)b->geos.size() || b->geos[r->geometry].dead)
+    return bfail(b, AERO3D_E_ARG, "submit: unknown geometry");
+  if (!(r->cfl > 0 && r->cfl < 10) || (r->order != 1 && r->order != 2)) return bfail(b, AERO3D_E_ARG, "submit: cfl in (0,10), order 1 or 2");
+  if (r->rule != AERO3D_RULE_RES_DROP && r->rule != AERO3D_RULE_REFERENCE) return bfail(b, AERO3D_E_ARG, "submit: unknown rule");
+  if (r->max_iters < 1) return bfail(b, AERO3D_E_ARG, "submit: max_iters >= 1");
+  if (r->rule == AERO3D_RULE_RES_DROP && !(r->res_drop > 0)) return bfail(b, AERO3D_E_ARG, "submit: res_drop > 0");
+  if (!(r->M > 0) || !(r->S > 0) || !(r->cref != 0) || !std::isfinite(r->alpha))
+    return bfail(b, AERO3D_E_ARG, "submit: coefficient reference needs M > 0, S > 0, cref != 0, finite alpha");
   int64_t need;
-  int rc = aero3d_batch_case_bytes(s->b, r->geometry, &need);
+  int rc = aero3d_batch_case_bytes(b, r->geometry, &need);
   if (rc) return rc;
-  if (free_when_idle(s->b) < need + (int64_t)kAlign) {
-    return bfail(s->b, AERO3D_E_STATE, "submit: request " + std::to_string(r->tag) + " needs " + std::to_string(need) +
-                                           " bytes, more than the arena holds beside the registered geometries");
+  if (free_when_idle(b) < need + (int64_t)kAlign) {
+    return bfail(b, AERO3D_E_STATE, "submit: request " + std::to_string(r->tag) + " needs " + std::to_string(need) +
+                                        " bytes, more than the arena holds beside the registered geometries");
   }
-  s->pending.push_back(Pending{*r, r->arrival});
+  s->pending.push_back(*r);
   return 0;
 }
 
-// Admit pending requests (FIFO) while the arena has room, step every resident
-// case by check_every iterations, measure residuals, retire finished cases.
+// Process finished ticks (waiting only when kInFlight ticks are queued, or
+// for the last ones when nothing is resident), admit pending requests by
+// footprint, and enqueue the next tick.
 int aero3d_server_tick(aero3d_server *s, int *n_active, int *n_pending) {
   if (!s) return AERO3D_E_ARG;
   aero3d_batch *b = s->b;
   int rc;
+  while (!s->inflight.empty()) {
+    const bool full = (int)s->inflight.size() >= kInFlight;
+    const hipError_t q = hipEventQuery(s->inflight.front().ev1);
+    if (q != hipSuccess && q != hipErrorNotReady) BCHK(b, q);
+    if (!(full || q == hipSuccess || s->live == 0)) break;
+    if ((rc = process_oldest(s))) return rc;
+  }
+  // admissions (FIFO, by HBM footprint)
+  std::vector<int> admitted;
   while (!s->pending.empty()) {
-    Pending &p = s->pending.front();
-    int64_t need;
-    if ((rc = aero3d_batch_case_bytes(b, p.r.geometry, &need))) return rc;
+    const aero3d_request r = s->pending.front();
+    int64_t need = 0;
     int slot = -1;
-    if ((int64_t)(b->arena.cap - b->arena.used) >= need + (int64_t)kAlign) {      // admission by HBM footprint
-      rc = aero3d_batch_add(b, p.r.geometry, p.r.F, p.r.fix, p.r.cfl, p.r.order, &slot);
-      if (rc && rc != AERO3D_E_STATE) return rc;
-      if (rc) slot = -1;                                                          // no slot / fragmented
+    rc = aero3d_batch_case_bytes(b, r.geometry, &need);
+    if (!rc && (int64_t)(b->arena.cap - b->arena.used) >= need + (int64_t)kAlign) {
+      rc = aero3d_batch_add(b, r.geometry, r.F, r.fix, r.cfl, r.order, &slot);
+      if (rc == AERO3D_E_STATE) { rc = 0; slot = -1; }                    // no slot / fragmented: wait
     }
-    if (slot < 0) {
-      if (!s->active.empty()) break;                 // wait: a resident case will free room
-      // nothing resident will free room (geometries registered after the
-      // submit, or slots taken on the batch directly): return the request as
-      // failed so the queue does not stall
-      aero3d_result o;
-      std::memset(&o, 0, sizeof o);
-      o.tag = p.r.tag; o.res0 = o.res = o.CL = o.CD = o.Cm = NAN;
-      o.arrival = p.arrival; o.t_admit = o.t_done = clock_s(s);
-      s->done.push_back(o);
-      b->err = "tick: request " + std::to_string(p.r.tag) + " cannot be admitted with no case resident";
+    if (rc == AERO3D_E_ARG) {        // cannot run (geometry removed since the submit): a failed result
+      s->done.push_back(failed_result(s, r, clock_s(s)));
+      b->err = "tick: request " + std::to_string(r.tag) + " rejected at admission";
       s->pending.pop_front();
       continue;
     }
-    Active a;
-    a.r = p.r; a.slot = slot; a.arrival = p.arrival; a.t_admit = clock_s(s);
-    s->active.push_back(a);
-    s->pending.pop_front();
-    // first snapshot (reset state) so the first check measures iterations 0..check_every
-    double dummy;
-    if ((rc = aero3d_batch_residuals(b, &slot, 1, &dummy))) return rc;
-  }
-  if (!s->active.empty()) {
-    const double t0 = clock_s(s);
-    if ((rc = aero3d_batch_step(b, s->check_every))) return rc;
-    std::vector<int> sl;
-    for (auto &a : s->active) sl.push_back(a.slot);
-    std::vector<double> res(sl.size());
-    if ((rc = aero3d_batch_residuals(b, sl.data(), (int)sl.size(), res.data()))) return rc;
-    for (auto &a : s->active) s->cell_updates += (int64_t)b->geos[a.r.geometry].N * s->check_every;
-    std::vector<Active> keep;
-    for (size_t k = 0; k < s->active.size(); k++) {
-      Active &a = s->active[k];
-      a.res = res[k];
-      if (!(a.res0 > 0) && a.res > 0) a.res0 = a.res;
-      int it;
-      aero3d_batch_iter(b, a.slot, &it);
-      const bool conv = a.res0 > 0 && a.res / a.res0 <= a.r.res_drop;
-      if (conv || it >= a.r.max_iters || !std::isfinite(a.res)) {
-        aero3d_result o;
-        std::memset(&o, 0, sizeof o);
-        o.tag = a.r.tag; o.iters = it; o.res0 = a.res0; o.res = a.res; o.converged = conv ? 1 : 0;
-        double c[3] = {NAN, NAN, NAN};
-        if (b->geos[a.r.geometry].nf > 0 && (rc = aero3d_batch_forces(b, a.slot, a.r.alpha, a.r.M, a.r.S, a.r.xcg, a.r.zcg, a.r.cref, c))) return rc;
-        o.CL = c[0]; o.CD = c[1]; o.Cm = c[2];
-        o.arrival = a.arrival; o.t_admit = a.t_admit; o.t_done = clock_s(s);
-        if ((rc = aero3d_batch_remove(b, a.slot))) return rc;
-        s->done.push_back(o);
-      } else {
-        keep.push_back(a);
-      }
+    if (rc) return rc;
+    if (slot < 0) {
+      if (s->live > 0 || !s->inflight.empty()) break;   // a resident case will free room
+      // nothing resident will free room: return the request as failed so the queue does not stall
+      s->done.push_back(failed_result(s, r, clock_s(s)));
+      b->err = "tick: request " + std::to_string(r.tag) + " cannot be admitted with no case resident";
+      s->pending.pop_front();
+      continue;
     }
-    s->active.swap(keep);
-    s->busy_s += clock_s(s) - t0;
+    SrvCase &c = s->cases[slot];
+    c.live = true; c.gen++; c.r = r; c.arrival = r.arrival; c.t_admit = clock_s(s); c.last_read_it = 0;
+    s->live++;
+    admitted.push_back(slot);
+    s->pending.pop_front();
   }
-  if (n_active) *n_active = (int)s->active.size();
+  if (s->live > 0) {
+    if (s->t_busy0 < 0) s->t_busy0 = clock_s(s);
+    TickRec T;
+    if ((rc = ring_acquire(b, &T.buf))) return rc;
+    if (s->evpool.empty()) {
+      hipEvent_t e0, e1;
+      BCHK(b, hipEventCreate(&e0));
+      BCHK(b, hipEventCreate(&e1));
+      s->evpool.push_back({e0, e1});
+    }
+    T.ev0 = s->evpool.back().first; T.ev1 = s->evpool.back().second;
+    s->evpool.pop_back();
+    const StageLayout L(b->max_slots);
+    char *p = b->ring.buf[T.buf];
+    BCHK(b, hipEventRecord(T.ev0, b->stream));
+    auto seg_of = [&](int slot, bool first) {
+      const aero3d_request &r = s->cases[slot].r;
+      RbSeg x = make_seg(b, slot, true, true, r.alpha, r.M, r.S, r.xcg, r.zcg, r.cref);
+      if (first) x.nit = 0;
+      x.rule = r.rule; x.cap = r.max_iters; x.res_drop = r.res_drop;
+      x.skip = &b->d_stat[slot].frozen;
+      return x;
+    };
+    // admission readback (iteration 0): snapshot, forces at the reset state, first rule update
+    if (!admitted.empty()) {
+      std::vector<RbSeg> segs;
+      for (int slot : admitted) { segs.push_back(seg_of(slot, true)); b->slots[slot].snap_it = 0; }
+      if ((rc = enqueue_segments(b, segs, p + L.segA, b->d_seg))) return rc;
+      srv_finish_kernel<<<(int)segs.size(), kRedBlock, 0, b->stream>>>(b->d_seg, b->d_stat);
+    }
+    // steps to the next readback of any resident case
+    int n = 1 << 30;
+    for (int slot = 0; slot < b->max_slots; slot++) {
+      const SrvCase &c = s->cases[slot];
+      if (c.live) n = std::min(n, next_check(c.r, b->slots[slot].it, s->check_every) - b->slots[slot].it);
+    }
+    n = std::max(n, 1);
+    if ((rc = batch_steps(b, n))) return rc;
+    // readback and rule of every case due
+    std::vector<RbSeg> segs;
+    for (int slot = 0; slot < b->max_slots; slot++) {
+      SrvCase &c = s->cases[slot];
+      if (!c.live) continue;
+      Slot &S = b->slots[slot];
+      if (S.it != next_check(c.r, S.it - n, s->check_every)) continue;
+      segs.push_back(seg_of(slot, false));
+      S.snap_it = S.it;
+      c.last_read_it = S.it;
+      T.due.push_back(Due{slot, c.gen, S.cur, S.it});
+    }
+    if (!segs.empty()) {
+      if ((rc = enqueue_segments(b, segs, p + L.segC, b->d_seg + b->max_slots))) return rc;
+      srv_finish_kernel<<<(int)segs.size(), kRedBlock, 0, b->stream>>>(b->d_seg + b->max_slots, b->d_stat);
+    }
+    BCHK(b, hipGetLastError());
+    BCHK(b, hipMemcpyAsync(p + L.stat, b->d_stat, sizeof(SlotStat) * b->max_slots, hipMemcpyDeviceToHost, b->stream));
+    BCHK(b, hipEventRecord(T.ev1, b->stream));
+    BCHK(b, hipEventRecord(b->ring.ev[T.buf], b->stream));
+    s->inflight.push_back(T);          // the buffer stays held until the tick is processed
+  }
+  if (n_active) *n_active = s->live;
   if (n_pending) *n_pending = (int)s->pending.size();
   return 0;
 }
@@ -789,9 +1022,10 @@ int aero3d_server_poll(aero3d_server *s, aero3d_result *out, int max, int *n) {
   return 0;
 }
 
-int aero3d_server_stats(const aero3d_server *s, double *busy_s, int64_t *cell_updates) {
-  if (!s || !busy_s || !cell_updates) return AERO3D_E_ARG;
-  *busy_s = s->busy_s; *cell_updates = s->cell_updates;
+int aero3d_server_stats(const aero3d_server *s, double *busy_s, double *gpu_s, int64_t *cell_updates) {
+  if (!s || !busy_s || !gpu_s || !cell_updates) return AERO3D_E_ARG;
+  *busy_s = s->busy_s + (s->t_busy0 >= 0 ? clock_s(s) - s->t_busy0 : 0);
+  *gpu_s = s->gpu_s; *cell_updates = s->cell_updates;
   return 0;
 }
 

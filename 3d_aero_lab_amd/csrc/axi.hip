@@ -113,7 +113,7 @@ void enqueue_iters(axi_solver *s, int n) {
     if (capped) {
       cell_kernel<<<gc, kB, 0, s->stream>>>(v, c, s->Q, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 2, s->cap);
       dtmin_kernel<<<nbm, kB, 0, s->stream>>>(s->dt, s->solid, s->N, s->partial);
-      dtcap_kernel<<<1, 64, 0, s->stream>>>(s->partial, nbm, s->global ? 1.f : (float)s->dtRatio, s->cap);
+      dtcap_kernel<<<1, 64, 0, s->stream>>>(s->partial, nbm, c, s->cap);
       cell_kernel<<<gc, kB, 0, s->stream>>>(v, c, s->Q, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 3, s->cap);
     } else {
       cell_kernel<<<gc, kB, 0, s->stream>>>(v, c, s->Q, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 0, s->cap);
@@ -226,6 +226,7 @@ int axi_set_opt(axi_solver *s, double cfl, int order, int visc, double hf, int g
   if (!(cfl > 0 && cfl < 10) || (order != 1 && order != 2) || dtRatio < 0) return afail(s, -1, "set_opt: cfl in (0,10), order 1|2, dtRatio >= 0");
   s->c.cfl = (float)cfl; s->c.order = order; s->c.visc = visc ? 1 : 0; s->c.hf = hf > 0 ? (float)hf : 0.99f * 43e6f;
   s->global = global ? 1 : 0; s->dtRatio = dtRatio;
+  s->c.capf = s->global ? 1.f : (float)dtRatio;
   s->dc_dirty = true;
   return 0;
 }
@@ -309,6 +310,7 @@ int axi_monitor(axi_solver *s, const double comps[32], double P0, axi_monitor_ou
   out->Fb = o[0]; out->Fw = o[1]; out->F = o[0] + o[1]; out->mfuel = o[2];
   for (int q = 0; q < 8; q++) out->pw[q] = o[3 + q];
   out->res_sum2 = o[11]; out->res_n = o[12]; out->res_valid = in.prev_valid;
+  out->bad = (int)o[13];
   out->nsec = s->nsec;
   for (int q = 0; q < s->nsec; q++) {
     out->sec_mdot[q] = o[kMonK + 3 * q]; out->sec_w[q] = o[kMonK + 3 * q + 1]; out->sec_wTt[q] = o[kMonK + 3 * q + 2];

@@ -25,6 +25,7 @@ struct AxiConsts {
   float fs_r, fs_u, fs_p;
   float cfl, hf;
   int order, visc;
+  float capf;        // dt cap factor of the global (1) / dtRatio caps, read at every replay
 };
 
 struct AxiView {
@@ -308,11 +309,13 @@ __global__ void __launch_bounds__(kB) dtmin_kernel(const float *__restrict__ dt,
     partial[blockIdx.x] = a;
   }
 }
-__global__ void dtcap_kernel(const float *__restrict__ partial, int nb, float factor, float *__restrict__ cap) {
+// The factor comes from the device constants, so a captured graph replays the
+// current global / dtRatio choice.
+__global__ void dtcap_kernel(const float *__restrict__ partial, int nb, const AxiConsts *__restrict__ cp, float *__restrict__ cap) {
   if (threadIdx.x != 0) return;
   float a = INFINITY;
   for (int b = 0; b < nb; b++) a = fminf(a, partial[b]);
-  *cap = a * factor;
+  *cap = a * cp->capf;
 }
 
 
@@ -320,8 +323,9 @@ __global__ void dtcap_kernel(const float *__restrict__ partial, int nb, float fa
 // Readback monitors in FP64: what the application's convergence loop reads
 // back after each chunk of iterations (src/app_cfd.js:361-404).
 //   monitor_kernel  thrust parts of cfdForces with the solver's own sources
-//                   (src/cfd_post.js:44-75) and the density residual of the
-//                   readback (src/app_cfd.js:368-371): block partial sums.
+//                   (src/cfd_post.js:44-75), the density residual of the
+//                   readback (src/app_cfd.js:368-371) and the count of
+//                   non-physical cells (src/app_cfd.js:377): block partial sums.
 //   section_kernel  mass flow and mass-weighted Tt sums of cfdSection
 //                   (src/cfd_post.js:83-102) at the control stations, one
 //                   thread per section, rows in the host's order.
@@ -329,7 +333,7 @@ __global__ void dtcap_kernel(const float *__restrict__ partial, int nb, float fa
 // (contraction off, NaN-propagating max / min / clamp as in JavaScript); only
 // the order of the grid-wide sums differs from the host's sequential loop.
 // ---------------------------------------------------------------------------
-constexpr int kMonK = 13;         // Fb, Fw, mfuel, pw[0..7], sum of d^2, count
+constexpr int kMonK = 14;         // Fb, Fw, mfuel, pw[0..7], sum of d^2, count, non-physical cells
 constexpr int kMonMaxSec = 4;
 constexpr int kMonBlocks = 512;
 constexpr double kPi = 3.141592653589793;
@@ -373,6 +377,9 @@ __global__ void __launch_bounds__(kB) monitor_kernel(AxiView v, const double *__
   const bool pv = in->prev_valid != 0;
   const double P0 = in->P0;
   for (int k = blockIdx.x * kB + threadIdx.x; k < N; k += gridDim.x * kB) {
+    // divergence check of the readback: every seventh cell, solid or not, with
+    // energy or density not positive (src/app_cfd.js:377)
+    if (k % 7 == 0 && (!(Q[3 * N + k] > 0.f) || !(Q[k] > 0.f))) acc[13] += 1.0;
     if (v.solid[k]) continue;
     const int i = k % W, j = k / W;
     const double A = 2.0 * kPi * rcD[j] * drD[j], V = A * dxD[i] * (double)v.phi[k];

@@ -16,9 +16,15 @@
  * and initial state). Prints one JSON line: per-grid iterations, residual and
  * thrust histories, final thrust, wall time.
  * By default each readback is the device monitor (AxiHIP.monitor: thrust,
- * blade-row powers, residual sums and the control sections in FP64, 28
- * numbers back); `host` reads the whole state back and computes the same
- * quantities with cfdForces / cfdSection on the host, as the application does.
+ * blade-row powers, residual sums, the control sections and the divergence
+ * count in FP64, 29 numbers back); `host` reads the whole state back and
+ * computes the same quantities with cfdForces / cfdSection on the host, as
+ * the application does.
+ * Divergence (src/app_cfd.js:377-378, 405-409): a readback that finds a cell
+ * k = 0, 7, 14, ... with density or energy not positive abandons the run; it
+ * starts again from the initial state and the setup's constants (the whole
+ * sequence when sequenced) with the CFL lowered by 0.2 (not below 0.3) and
+ * first-order reconstruction, at most `maxRestarts` times.
  * ========================================================================== */
 const P = require('./cfd_post.js');
 const { interpState } = require('./cfd_grid.js');
@@ -37,10 +43,23 @@ function residual(zones, Q0, prev, dIt) {
   return Math.sqrt(s2 / Math.max(1, n)) / dIt;
 }
 
-/* Runs one grid until the reference's criteria; returns the final state. */
-function runGrid(C, comps, Q0, seq, every, cap, log, host) {
+/* cells k = 0, 7, 14, ... with energy or density not positive (the divergence
+ * check of src/app_cfd.js:377) */
+function badCells(Q) {
+  let bad = 0;
+  for (let k = 0; k < Q[0].length; k += 7) if (!(Q[3][k] > 0) || !(Q[0][k] > 0)) bad++;
+  return bad;
+}
+
+function makeAxiHIP(C, comps, Q0, opt) {
+  return new AxiHIP(C.grid, C.zones, comps, C.s.fs, Q0, Object.assign({}, C.s.opt, { hf: C.s.hf }, opt));
+}
+
+/* Runs one grid until the reference's criteria; returns the final state, or
+ * null when a readback found non-physical cells. */
+function runGrid(C, comps, Q0, seq, every, cap, log, host, opt, make) {
   const s = C.s;
-  const S = new AxiHIP(C.grid, C.zones, comps, s.fs, Q0, Object.assign({}, s.opt, { hf: s.hf }));
+  const S = make(C, comps, Q0, opt);
   const minIt = seq ? 2500 : 3000;
   let it = 0, prev = null, res = NaN, Q = null, f = null;
   const lastF = [];
@@ -53,6 +72,7 @@ function runGrid(C, comps, Q0, seq, every, cap, log, host) {
       Q = S.read(Q);
       if (prev) res = residual(C.zones, Q[0], prev, every);
       prev = Float32Array.from(Q[0]);
+      if (badCells(Q)) { S.dispose(); log.itTotal += it; return null; }
       f = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, s.hf, { S: S.src, F: S.mfx });
       if (gate) {
         const f0 = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, 0, { S: S.src, F: S.mfx });
@@ -63,6 +83,7 @@ function runGrid(C, comps, Q0, seq, every, cap, log, host) {
       // with the solver's sources the forces do not depend on the heating value
       const m = S.monitor(comps, s.ctl.P0);
       if (m.resValid) res = Math.sqrt(m.resSum2 / Math.max(1, m.resN)) / every;
+      if (m.bad) { S.dispose(); log.itTotal += it; return null; }
       f = m;
       if (gate) {
         P.controlFromSections(s.type || 'tj', s.ctl, st => P.sectionFromSums(m.sec[st.id === 4 ? 0 : 1]), comps, m.pw, s.ctl.etam, every);
@@ -88,27 +109,46 @@ function runGrid(C, comps, Q0, seq, every, cap, log, host) {
   return Q;
 }
 
-function run(coarseDir, targetDir, every, cap, host) {
-  const log = { itTotal: 0, hist: [], grids: [] };
-  const T = loadCase(targetDir);
+/* One attempt of the whole run (coarse grid first when sequenced); null on divergence. */
+function attempt(coarseDir, T, every, cap, host, opt, make, load, log) {
   const comps = Float64Array.from(T.comps);
   let Q0 = T.Q0;
-  const t0 = process.hrtime.bigint();
   if (coarseDir && coarseDir !== '-') {
-    const Cc = loadCase(coarseDir), cc = Float64Array.from(Cc.comps);
-    const Qc = runGrid(Cc, cc, Cc.Q0, true, every, cap, log, host);
+    const Cc = load(coarseDir), cc = Float64Array.from(Cc.comps);
+    const Qc = runGrid(Cc, cc, Cc.Q0, true, every, cap, log, host, opt, make);
+    if (!Qc) return null;
     const Q64 = [0, 1, 2, 3, 4].map(q => Float64Array.from(Qc[q]));
     Q0 = interpState(Cc.grid, Cc.zones, Q64, T.grid, T.zones, T.s.fl);
     comps[17] = cc[17]; comps[29] = cc[29];                 // control gains carried (src/app_cfd.js:281)
   }
-  runGrid(T, comps, Q0, false, every, cap, log, host);
+  return runGrid(T, comps, Q0, false, every, cap, log, host, opt, make);
+}
+
+/* o: {makeSolver(C, comps, Q0, opt), loadCase(dir), maxRestarts} (defaults: AxiHIP, the fixture loader, 3). */
+function run(coarseDir, targetDir, every, cap, host, o = {}) {
+  const make = o.makeSolver || makeAxiHIP, load = o.loadCase || loadCase, maxRestarts = o.maxRestarts === undefined ? 3 : o.maxRestarts;
+  const T = load(targetDir);
+  const opt = { cfl: T.s.opt.cfl, order: T.s.opt.order };
+  const restarts = [];
+  const t0 = process.hrtime.bigint();
+  let log;
+  for (;;) {
+    log = { itTotal: 0, hist: [], grids: [] };
+    if (attempt(coarseDir, T, every, cap, host, opt, make, load, log)) break;
+    restarts.push({ at: log.itTotal, cfl: opt.cfl, order: opt.order });
+    if (restarts.length > maxRestarts) {
+      return { mode: host ? 'host' : 'device', diverged: true, restarts, wall_s: Number(process.hrtime.bigint() - t0) / 1e9 };
+    }
+    opt.cfl = Math.max(0.3, opt.cfl - 0.2); opt.order = 1;   // src/app_cfd.js:407
+  }
   const wall = Number(process.hrtime.bigint() - t0) / 1e9;
   const last = log.grids[log.grids.length - 1];
-  return { mode: host ? 'host' : 'device', grids: log.grids, iters: log.itTotal, F: last.F, res: last.res, wall_s: wall, hist: log.hist };
+  return { mode: host ? 'host' : 'device', grids: log.grids, iters: log.itTotal, F: last.F, res: last.res, wall_s: wall, hist: log.hist,
+    restarts, opt };
 }
 
 if (require.main === module) {
   const [coarse, target, every, cap, mode] = process.argv.slice(2);
   console.log(JSON.stringify(run(coarse, target, +(every || 250), +(cap || 60000), mode === 'host')));
 }
-module.exports = { run, residual };
+module.exports = { run, residual, badCells };

@@ -69,13 +69,15 @@ class AxiHIP {
   /* The readback of the convergence loop on the device (FP64): thrust parts
    * and blade-row powers of cfdForces with the solver's own sources, the
    * density residual sums against the previous monitor call, and the
-   * section sums {mdot, w, wTt}. Waits for the GPU; reads back 28 numbers. */
+   * section sums {mdot, w, wTt}, and the count of non-physical cells of the
+   * divergence check (bad). Waits for the GPU; reads back 29 numbers. */
   monitor(comps, P0) {
-    const o = this._mon || (this._mon = new Float64Array(28));
+    const o = this._mon || (this._mon = new Float64Array(29));
     loadAddon().axiMonitor(this.h, comps instanceof Float64Array ? comps : Float64Array.from(comps), P0, o);
     const sec = [];
     for (let q = 0; q < o[15]; q++) sec.push({ mdot: o[16 + 3 * q], w: o[17 + 3 * q], wTt: o[18 + 3 * q] });
-    return { F: o[0], Fb: o[1], Fw: o[2], mfuel: o[3], pw: o.slice(4, 12), resSum2: o[12], resN: o[13], resValid: o[14] !== 0, sec };
+    return { F: o[0], Fb: o[1], Fw: o[2], mfuel: o[3], pw: o.slice(4, 12), resSum2: o[12], resN: o[13], resValid: o[14] !== 0, sec,
+      bad: o[28] };
   }
 }
 

@@ -144,7 +144,7 @@ def run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes, m
         if a == 0 and p == 0 and nxt < len(pending):
             time.sleep(max(0.0, reqs[pending[nxt]][3] - (srv.clock() - t_start)))
     wall = srv.clock() - t_start
-    busy, cu = srv.stats()
+    busy, gpu, cu = srv.stats()
     srv.dispose()
     return done, wall, busy, cu
 

@@ -93,6 +93,23 @@ int aero3d_trefftz(aero3d_solver *s, double xT, double alphaDeg, double M, doubl
  * after create/reset only takes the snapshot and returns *res = NaN. */
 int aero3d_residual(aero3d_solver *s, double *res);
 
+/* The readback of the reference's controller, CFD3.readback
+ * (src/air_cfd.js:550-571), on the device in one call: the density residual
+ * since the previous readback / residual call (NaN at the first; the snapshot
+ * is retaken), cfdForces with the force list (NaN without one), the Trefftz
+ * induced drag at plane xT (NaN when xT is NaN) and the largest Mach number
+ * over every second fluid cell. M is floored at 0.05 as the reference does.
+ * FP64 sums with the reference's per-term arithmetic; waits. */
+typedef struct {
+  double alpha, M, S, xcg, zcg, cref;   /* cfdForces arguments (ac.ref), alpha in degrees */
+  double xT;                            /* Trefftz plane x, NaN: none                      */
+} aero3d_readback_in;
+typedef struct {
+  int it;                               /* iteration count at the readback                 */
+  double res, CL, CD, Cm, CDi, Mmax;
+} aero3d_readback_out;
+int aero3d_readback(aero3d_solver *s, const aero3d_readback_in *in, aero3d_readback_out *out);
+
 /* Benchmark helper: run n iterations on the handle's stream bracketed by HIP
  * events recorded on that stream; *ms = elapsed device time. */
 int aero3d_time_steps(aero3d_solver *s, int n, float *ms);
@@ -108,7 +125,8 @@ int aero3d_set_tile(aero3d_solver *s, int ty, int zchunk);
 int aero3d_get_tile(const aero3d_solver *s, int *ty, int *zchunk);
 /* Autotuner of the stage-kernel shape: times every candidate (ty in {4, 8},
  * zchunk in {0, -2, -3, 2, 3, 4, 6, 8}) over `iters` iterations on this handle's grid, keeps
- * the fastest, and restores the state and iteration count. Outputs may be
+ * the fastest, and restores the state, the local dt that read() returns and
+ * the iteration count. Outputs may be
  * null; *ms_best is the best time per iteration. */
 int aero3d_autotune(aero3d_solver *s, int iters, int *ty, int *zchunk, float *ms_best);
 /* Device-memory footprint of the handle in bytes. */
@@ -171,11 +189,23 @@ int aero3d_batch_time_steps(aero3d_batch *b, int n, float *ms);
  * Served form (no reference counterpart: the reference runs one solve at a
  * time from its UI loop, src/air_cfd.js:539-578). Requests queue FIFO and are
  * admitted into a batch as slots and arena memory free up (admission by HBM
- * footprint, continuous batching); each tick steps every resident case by the
- * check interval, measures its density residual on the device and retires it
- * once res / res0 <= res_drop (the readback residual, src/air_cfd.js:553-554)
- * or at max_iters, with its force coefficients.
+ * footprint, continuous batching). A tick enqueues, without waiting on the
+ * stream: the admission readback of new cases, the iterations to the next
+ * readback of any case, then the readback of every case due (density residual
+ * since its previous readback and force coefficients, src/air_cfd.js:552-558)
+ * and its convergence rule, decided on the device; a finished case is frozen
+ * on the device at once. The host lags one tick: each tick processes the
+ * results of the tick before the last, retires the frozen cases with their
+ * coefficients and frees their memory. Readbacks come every check_every
+ * iterations from a case's admission (and at max_iters under the residual-
+ * drop rule), so a case's iterations, residuals and coefficients do not
+ * depend on the other cases: they equal a plain solve read back at the same
+ * iterations (aero3d_readback) bit for bit.
  * ------------------------------------------------------------------------- */
+#define AERO3D_RULE_RES_DROP 0   /* done at res / res0 <= res_drop, or at max_iters iterations               */
+#define AERO3D_RULE_REFERENCE 1  /* the reference controller (src/air_cfd.js:573-576): more than 6 readbacks,
+                                    res / res0 < 2e-3 and CL of the last 4 readbacks within 2e-3; or more than
+                                    max_iters iterations (the reference uses 30000)                         */
 typedef struct aero3d_server aero3d_server;
 typedef struct {
   int geometry;                  /* gid registered on aero3d_server_batch(s)        */
@@ -183,14 +213,17 @@ typedef struct {
   double cfl;
   int order;
   double alpha, M, S, xcg, zcg, cref;   /* force-coefficient reference (cfdForces)  */
-  double res_drop;               /* convergence: res / res0 <= res_drop              */
+  int rule;                      /* AERO3D_RULE_*                                    */
+  double res_drop;               /* AERO3D_RULE_RES_DROP: res / res0 <= res_drop     */
   int max_iters;                 /* iteration cap                                    */
   double arrival;                /* arrival time on the server clock (s)             */
   int64_t tag;                   /* caller's id, returned in the result              */
+  float *state_out;              /* optional: 5 planes of N floats that receive the
+                                    final state at retirement (costs a wait then)    */
 } aero3d_request;
 typedef struct {
   int64_t tag;
-  int iters, converged;
+  int iters, converged, diverged, failed, nreads;  /* failed: never ran (not admissible) */
   double res0, res, CL, CD, Cm;
   double arrival, t_admit, t_done;  /* server clock (s); latency = t_done - arrival */
 } aero3d_result;
@@ -200,17 +233,22 @@ int aero3d_server_destroy(aero3d_server *s);
 aero3d_batch *aero3d_server_batch(aero3d_server *s);
 /* Seconds since the server was created (steady clock). */
 int aero3d_server_clock(const aero3d_server *s, double *now);
-/* Queue a request (FIFO). AERO3D_E_STATE when its case cannot fit the arena
- * even with no case resident (message on aero3d_server_batch(s)). */
+/* Queue a request (FIFO). AERO3D_E_ARG for a bad request (geometry, cfl,
+ * order, rule, res_drop, max_iters, coefficient reference); AERO3D_E_STATE
+ * when its case cannot fit the arena even with no case resident (message on
+ * aero3d_server_batch(s)). */
 int aero3d_server_submit(aero3d_server *s, const aero3d_request *r);
-/* Admit, step check_every iterations, retire; returns resident / queued counts.
- * A queued request that still cannot be admitted when no case is resident is
- * returned at once as a result with iters 0 and NaN residuals / coefficients. */
+/* Process the results of finished ticks (waiting only when two ticks are
+ * queued), admit, enqueue the next tick; returns resident (not yet retired)
+ * and queued counts. A queued request that cannot be admitted with no case
+ * resident, or whose geometry was removed, is returned as a result with
+ * failed = 1. */
 int aero3d_server_tick(aero3d_server *s, int *n_active, int *n_pending);
 /* Move up to max finished results to out; *n = number moved. */
 int aero3d_server_poll(aero3d_server *s, aero3d_result *out, int max, int *n);
-/* Wall time spent in ticks with resident work, and cell-updates performed. */
-int aero3d_server_stats(const aero3d_server *s, double *busy_s, int64_t *cell_updates);
+/* Wall time with resident work, device time of the ticks (HIP events around
+ * each tick's work), and cell-updates of retired cases. */
+int aero3d_server_stats(const aero3d_server *s, double *busy_s, double *gpu_s, int64_t *cell_updates);
 
 #ifdef __cplusplus
 }

@@ -63,6 +63,7 @@ int axi_time_steps(axi_solver *s, int n, float *ms);
  * from a full read() after each chunk of iterations.
  *   F, Fb, Fw, mfuel, pw  cfdForces with the solver's own sources
  *                         (src/cfd_post.js:44-75; F = Fb + Fw)
+ *   bad                   non-physical cells of the divergence check
  *   res_sum2, res_n       sum of ((rho - rho_prev) / rho_prev)^2 and the count
  *                         over every second non-solid cell, against the state
  *                         of the previous axi_monitor call (src/app_cfd.js:368-371);
@@ -77,6 +78,8 @@ typedef struct {
   double res_sum2, res_n;
   int res_valid, nsec;
   double sec_mdot[4], sec_w[4], sec_wTt[4];
+  int bad;                        /* cells k = 0, 7, 14, ... with rho or E not positive (the
+                                     divergence check of src/app_cfd.js:377); > 0: diverged */
 } axi_monitor_out;
 int axi_monitor(axi_solver *s, const double comps[32], double P0, axi_monitor_out *out);
 /* Sections for axi_monitor (at most 4): column col[q] and row mask rows[q*H + j]

@@ -10,7 +10,8 @@
  * so that on identical inputs it reproduces the reference bit for bit (the
  * golden vectors under tests/golden pin this). Post-processing restates
  * cfdForces (src/air_cfd.js:443-452), the readback's Trefftz sum
- * (src/air_cfd.js:560-570) and density residual (src/air_cfd.js:553-554).
+ * (src/air_cfd.js:560-570), density residual (src/air_cfd.js:553-554) and
+ * Mach maximum (src/air_cfd.js:571).
  *
  * The only extension is OpenMP over cells inside a stage (cells are
  * independent within a stage), used to time the baseline on all host cores;
@@ -297,4 +298,20 @@ double o3_residual(int N, const uint8_t *t, const float *r, const float *prev, i
   double sum = 0; int n = 0;
   for (int q = 0; q < N; q += 3) { if (t[q] != 0) continue; const double d = (double)r[q] - prev[q]; sum += d * d; n++; }
   return sqrt(sum / (n > 1 ? n : 1)) / nIt;
+}
+
+/* Mach maximum of the readback: src/air_cfd.js:571. Every second cell, fluid
+ * cells only: m2 = (u^2 + v^2 + w^2) / (CG p / r) in doubles from the Float32
+ * state; the square root of the largest m2 (0 with no fluid cell; a NaN m2
+ * never compares larger). */
+double o3_mach_max(int N, const uint8_t *t, const float *r_, const float *ru_, const float *rv_, const float *rw_, const float *E_) {
+  double mMax = 0;
+  for (int q = 0; q < N; q += 2) {
+    if (t[q] != 0) continue;
+    const double r = r_[q], u = ru_[q] / r, v = rv_[q] / r, w = rw_[q] / r;
+    const double p = (CG - 1) * (E_[q] - 0.5 * r * (u * u + v * v + w * w));
+    const double m2 = (u * u + v * v + w * w) / (CG * p / r);
+    if (m2 > mMax) mMax = m2;
+  }
+  return sqrt(mMax);
 }

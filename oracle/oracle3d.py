@@ -47,6 +47,8 @@ def lib():
         L.o3_trefftz.argtypes = [C.c_int] * 3 + [f32p] * 3 + [u8p] + [f32p] * 4 + [C.c_double] * 4
         L.o3_residual.restype = C.c_double
         L.o3_residual.argtypes = [C.c_int, u8p, f32p, f32p, C.c_int]
+        L.o3_mach_max.restype = C.c_double
+        L.o3_mach_max.argtypes = [C.c_int, u8p] + [f32p] * 5
         _lib = L
     return _lib
 
@@ -124,3 +126,8 @@ def trefftz(grid, t, Q, xT, alpha, M, S):
 def residual(t, r, prev, nit):
     return lib().o3_residual(len(r), np.ascontiguousarray(t, dtype=np.uint8), np.ascontiguousarray(r, dtype=np.float32),
                              np.ascontiguousarray(prev, dtype=np.float32), nit)
+
+
+def mach_max(t, Q):
+    a = lambda x: np.ascontiguousarray(x, dtype=np.float32)
+    return lib().o3_mach_max(len(Q[0]), np.ascontiguousarray(t, dtype=np.uint8), *[a(Q[c]) for c in range(5)])

@@ -44,8 +44,11 @@ class OracleAxi:
                      [f(zones[k], np.float32) for k in ("wgt", "lmix", "tgt", "rate", "yrate", "mfd")] + \
                      [f(comps, np.float64), np.array([fs["r"], fs["u"], fs["p"]], np.float64), f(np.concatenate(Q0), np.float64)]
         self.h = L.ax_create(W, H, *self._keep)
-        L.ax_set_opt(self.h, opt.get("cfl", 0.6), opt.get("order", 2), int(opt.get("visc", True)), opt.get("hf", 0.0),
-                     int(bool(opt.get("global", False))), float(opt.get("dtRatio", 0) or 0))
+        self.set_opt(opt)
+
+    def set_opt(self, opt):
+        lib().ax_set_opt(self.h, opt.get("cfl", 0.6), opt.get("order", 2), int(opt.get("visc", True)), opt.get("hf", 0.0),
+                         int(bool(opt.get("global", False))), float(opt.get("dtRatio", 0) or 0))
 
     def __del__(self):
         if getattr(self, "h", None):

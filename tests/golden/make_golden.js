@@ -7,6 +7,7 @@
  *   node tests/golden/make_golden.js [case ...]
  *
  * Cases: specs sphere transport1m transport60k fighter30k interceptor sampling
+ *        readback_sphere readback_fighter readback_transport
  * Each case writes tests/golden/<case>/manifest.json plus gzip'd raw arrays.
  * The reference runs only here; its outputs are data (grid faces, masks,
  * states after N iterations, force coefficients), never its source.
@@ -15,7 +16,8 @@
  * buildAircraft / aircraftPanels (src/aircraft.js:108, 166), AirGeom
  * (src/air_field.js:104), cfdGrid / cfdGeom / cfdForceList / CfdCPU /
  * cfdForces (src/air_cfd.js:21-259, 443-452), the Trefftz sum of
- * CFD3.readback (src/air_cfd.js:560-570).
+ * CFD3.readback (src/air_cfd.js:560-570), and CFD3.readback itself
+ * (src/air_cfd.js:550-578: residual, forces, CDi, Mmax, convergence flag).
  * ========================================================================== */
 const fs = require('fs');
 const path = require('path');
@@ -239,8 +241,76 @@ function caseSampling() {
   w.close();
 }
 
+/* The reference's own readback controller, CFD3.readback (src/air_cfd.js:550-578),
+ * over a CfdCPU run: one readback every `every` iterations (density residual
+ * against the previous readback, cfdForces, Trefftz CDi, Mmax, the history and
+ * the convergence flag), until the controller sets conv or the run reaches
+ * maxIt. The controller's page hooks (CUR, paintAll, drawCfdCard,
+ * updateSolverBar) are stood in for by the case's aircraft record and no-ops. */
+function runReadback(w, g, geo, list, F, fix, opt, ac, alpha, M, every, maxIt, dumpFinal) {
+  globalThis.CUR = { R: { alpha }, ac };
+  globalThis.paintAll = () => {}; globalThis.updateSolverBar = () => {}; globalThis.drawCfdCard = () => {};
+  const s = new CfdCPU(g, geo);
+  s.opt.cfl = opt.cfl; s.opt.order = opt.order;
+  s.setFree(F); s.setFix(fix); s.reset();
+  Object.assign(CFD3, { solver: s, g, mask: geo, faces: list, H: null, hist: [], conv: false, prevR: null, prevIt: 0,
+    res: NaN, res0: NaN, alpha, M });
+  const reads = [], t0 = Date.now();
+  while (!CFD3.conv && s.it < maxIt) {
+    s.step(every);
+    CFD3.readback();
+    reads.push({ it: s.it, res: CFD3.res, res0: CFD3.res0, CL: CFD3.F.CL, CD: CFD3.F.CD, Cm: CFD3.F.Cm, CDi: CFD3.CDi, xT: CFD3.xT,
+      Mmax: CFD3.Mmax, conv: CFD3.conv });
+    if (reads.length % 10 === 0) console.log(`  it ${s.it} res/res0 ${(CFD3.res / CFD3.res0).toExponential(3)} CL ${CFD3.F.CL.toFixed(5)} CD ${CFD3.F.CD.toFixed(5)}`);
+  }
+  w.set('every', every); w.set('maxIt', maxIt); w.set('reads', reads);
+  w.set('ms_per_iter', (Date.now() - t0) / s.it);
+  if (dumpFinal) dumpState(w, 'final', s);
+  w.set('final_it', s.it);
+  return s;
+}
+
+function caseReadbackSphere() {
+  const w = new Writer('readback_sphere');
+  const C = Mine.cases.sphereCase();
+  const g = C.g, { ac, G } = sphereAsAircraft(C.body);
+  const geo = cfdGeom(ac, G, g, 0), list = cfdForceList(g, geo);
+  const M = 0.4, F = [1, M, 0, 0, 1 / (CG * (CG - 1)) + 0.5 * M * M], R = C.body.R;
+  // the controller's aircraft record: reference quantities of the sphere (S = pi R^2),
+  // body length 2R for the Trefftz plane x = 2R + 0.12 (2R)
+  const rac = { fus: { L: 2 * R }, htail: { semi: 0, ct: 0, xLE: () => 0 }, ref: { S: Math.PI * R * R, xcg: 0, zcg: 0, c: 1 } };
+  w.set('F', F); w.set('M', M); w.set('alpha', 0); w.set('cfl', 0.5); w.set('ref', rac.ref); w.set('fusL', 2 * R);
+  runReadback(w, g, geo, list, F, [F, F, F], { cfl: 0.5, order: 2 }, rac, 0, M, 50, +(process.env.READBACK_MAX_IT || 8000), true);
+  w.close();
+}
+
+function caseReadbackFighter() {
+  const w = new Writer('readback_fighter30k');
+  const { ac, G } = refAircraft('fighter');
+  const save = CFD_GRIDS.coarse.cells; CFD_GRIDS.coarse.cells = 30000;
+  const g = cfdGrid(ac, 'coarse'); CFD_GRIDS.coarse.cells = save;
+  const geo = cfdGeom(ac, G, g, 3), list = cfdForceList(g, geo);
+  const M = 0.6, alpha = 3, a = alpha * DEG, F = [1, M * Math.cos(a), 0, M * Math.sin(a), 1 / (CG * (CG - 1)) + 0.5 * M * M];
+  w.set('F', F); w.set('M', M); w.set('alpha', alpha); w.set('delta', 3); w.set('cfl', 0.7); w.set('ref', ac.ref);
+  runReadback(w, g, geo, list, F, [F, F, F], { cfl: 0.7, order: 2 }, ac, alpha, M, 50, +(process.env.READBACK_MAX_IT || 8000), true);
+  w.close();
+}
+
+function caseReadbackTransport() {
+  const w = new Writer('readback_transport60k');
+  const { ac, G } = refAircraft('airliner', Mine.bodies.TRANSPORT_OVERRIDES);
+  const save = CFD_GRIDS.coarse.cells; CFD_GRIDS.coarse.cells = 60e3;
+  const g = cfdGrid(ac, 'coarse'); CFD_GRIDS.coarse.cells = save;
+  const geo = cfdGeom(ac, G, g, 0), list = cfdForceList(g, geo);
+  const M = 0.78, alpha = 2, a = alpha * DEG, F = [1, M * Math.cos(a), 0, M * Math.sin(a), 1 / (CG * (CG - 1)) + 0.5 * M * M];
+  w.set('F', F); w.set('fix', Mine.cases.TRANSPORT_FIX); w.set('M', M); w.set('alpha', alpha); w.set('cfl', 0.5); w.set('ref', ac.ref);
+  runReadback(w, g, geo, list, F, Mine.cases.TRANSPORT_FIX, { cfl: 0.5, order: 2 }, ac, alpha, M, 25, 100, false);
+  w.close();
+}
+
 const want = process.argv.slice(2);
 const all = { specs: caseSpecs, sphere: caseSphere, transport1m: () => caseTransport('transport1m', 1.0e6, 0, [], false),
   transport60k: () => caseTransport('transport60k', 60e3, 100, [2, 100], true), fighter30k: caseFighter, interceptor: caseInterceptor,
-  sampling: caseSampling };
+  sampling: caseSampling, readback_sphere: caseReadbackSphere, readback_fighter: caseReadbackFighter,
+  readback_transport: caseReadbackTransport };
 for (const k of (want.length ? want : Object.keys(all))) { const t0 = Date.now(); all[k](); console.log(`${k}: ${((Date.now() - t0) / 1000).toFixed(1)} s`); }

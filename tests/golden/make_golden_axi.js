@@ -9,7 +9,7 @@
  *   node tests/golden/make_golden_axi.js [case ...]
  * cases: xcoarse medium longrun (TJ), hbtf_xcoarse hbtf_medium lbtf_xcoarse,
  * longrun_hbtf longrun_lbtf (the README.md:71-75 validation rows), setups,
- * interp, tj_coarse, checkpoint
+ * interp, tj_coarse, checkpoint, options
  *
  * The reference's own harness (tools/cfdlib.js:6-20) uses `??`, which this
  * Node (v12) cannot parse; the same setup sequence is written out here:
@@ -281,6 +281,29 @@ function caseCheckpoint() {
   console.log('wrote axi_ckpt');
 }
 
+/* CPUSolver's option paths (src/cfd_cpu.js:53, 193-203) on TJ xcoarse: the
+ * global and dtRatio dt caps and first order without the viscous r fluxes,
+ * 10 iterations each; and dtRatio 3 for 30 iterations, then 5 for 30 more.
+ * The conserved planes go in full (Float64). */
+function caseOptions() {
+  const w = new Writer('axi_tj_options');
+  const dumpQ = (tag, C) => { for (let q = 0; q < 5; q++) w.arr(`${tag}_Q${q}`, C.S.Q[q]); };
+  for (const [tag, o] of [['global', { global: true }], ['dtr3', { dtRatio: 3 }], ['o1e', { order: 1, visc: false }]]) {
+    const C = setup('tj', 'xcoarse');
+    Object.assign(C.S.opt, o);
+    C.S.step(10);
+    dumpQ(`${tag}_it10`, C);
+    if (tag === 'global') w.arr('global_it10_dt', C.S.dt);
+  }
+  const C = setup('tj', 'xcoarse');
+  C.S.opt.dtRatio = 3; C.S.step(30);
+  C.S.opt.dtRatio = 5; C.S.step(30);
+  dumpQ('dtr35_it60', C);
+  w.set('runs', { global: { global: true, iters: 10 }, dtr3: { dtRatio: 3, iters: 10 }, o1e: { order: 1, visc: false, iters: 10 },
+    dtr35: { dtRatio: [3, 5], iters: [30, 30] } });
+  w.close();
+}
+
 const want = process.argv.slice(2);
 const all = {
   xcoarse: caseXcoarse, medium: caseMedium,
@@ -290,5 +313,6 @@ const all = {
   tj_coarse: () => caseSetupOnly('tj', 'coarse'), checkpoint: caseCheckpoint,
   longrun_hbtf: () => caseLongrun('hbtf', 'medium', 14000, 'axi_hbtf_medium_longrun'),
   longrun_lbtf: () => caseLongrun('lbtf', 'xcoarse', 10000, 'axi_lbtf_xcoarse_longrun'),
+  options: caseOptions,
 };
 for (const k of (want.length ? want : ['xcoarse', 'medium'])) { const t0 = Date.now(); all[k](); console.log(`${k}: ${((Date.now() - t0) / 1000).toFixed(1)} s`); }

@@ -53,6 +53,56 @@ def test_axi_matches_oracle(case, iters, opt):
     h.dispose()
 
 
+@pytest.mark.parametrize("tag,opt,sched", [("global", {"global": True}, [(10, None)]), ("dtr3", {"dtRatio": 3.0}, [(10, None)]),
+                                          ("o1e", {"order": 1, "visc": False}, [(10, None)]),
+                                          ("dtr35", {"dtRatio": 3.0}, [(30, None), (30, {"dtRatio": 5.0})])])
+def test_axi_option_paths_match_reference(tag, opt, sched):
+    """CPUSolver's option paths run by the reference itself (TJ xcoarse): the
+    global and dtRatio dt caps, first order without viscous terms, and dtRatio
+    changed from 3 to 5 after 30 iterations. The 30-iteration chunks replay the
+    captured 25-iteration graph, so a changed cap factor must reach the replay."""
+    import aero3d
+    from golden_io import Golden
+    g0, z, Q0, comps, xf, rf = load("axi_tj_xcoarse")
+    g = Golden("axi_tj_options")
+    s = g0.s
+    base = dict(s["opt"], hf=s["hf"])
+    h = aero3d.AxiHIP(xf, rf, z, comps, s["fs"], Q0, {**base, **opt})
+    done = 0
+    for n, change in sched:
+        if change:
+            h.opt.update(change)
+        h.step(n)
+        done += n
+    Q = h.read()[0]
+    ref = np.stack([g.arr(f"{tag}_it{done}_Q{q}") for q in range(5)])
+    d = rel_diff(Q, ref, z["solid"])
+    say("options", tag, done, d)
+    assert max(d.values()) < 1e-5, d
+    h.dispose()
+
+
+def test_axi_monitor_counts_nonphysical_cells():
+    """The divergence check of the readback (src/app_cfd.js:377) on the
+    device: cells k = 0, 7, 14, ... with density or energy not positive, solid
+    or not, counted as the host loop counts them."""
+    import aero3d
+    g, z, Q0, comps, xf, rf = load("axi_tj_xcoarse")
+    s = g.s
+    h = aero3d.AxiHIP(xf, rf, z, comps, s["fs"], Q0, dict(s["opt"], hf=s["hf"]))
+    h.step(2)
+    m = h.monitor(comps, s["ctl"]["P0"])
+    assert m["bad"] == 0
+    Q = h.read()[0].astype(np.float64)
+    Q[0][0] = -1.0; Q[3][7 * 11] = np.nan; Q[0][7 * 11] = np.nan; Q[3][7 * 40] = 0.0; Q[0][5] = -1.0
+    h.set_state(Q)
+    m = h.monitor(comps, s["ctl"]["P0"])
+    host = sum(1 for k in range(0, Q.shape[1], 7) if not (Q[3][k] > 0) or not (Q[0][k] > 0))
+    say("non-physical cells", m["bad"], "host", host)
+    assert m["bad"] == host == 3
+    h.dispose()
+
+
 @pytest.mark.parametrize("case,its", [("axi_tj_xcoarse", (100, 500)), ("axi_hbtf_xcoarse", (100,)), ("axi_lbtf_xcoarse", (100,))])
 def test_axi_thrust_matches_reference(case, its):
     g, *_ = load(case)

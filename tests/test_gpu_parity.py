@@ -219,11 +219,101 @@ def test_autotune_keeps_state():
     F = g.s["F"]
     s = make_hip(g, F, [F, F, F], 0.5)
     s.step(10)
-    Q = s.read()
+    Q = s.read(with_dt=True)
     ty, zc, ms = s.autotune(3)
     say("autotune sphere", ty, zc, ms)
     assert ty in (4, 8) and zc >= -8 and ms > 0
-    assert s.it == 10 and np.array_equal(s.read(), Q)
+    assert s.it == 10 and np.array_equal(s.read(with_dt=True), Q)      # state and the local dt restored
     s.step(190)
     d = rel_diff(s.read(), g.state("it200"), g.geo()["t"])
     assert max(d.values()) < STATE_TOL, d
+
+
+def test_readback_matches_oracle_functions():
+    """aero3d_readback (residual since the previous readback, coefficients,
+    Trefftz CDi, Mach maximum in one call) against the oracle's functions on
+    the GPU's own states: the sums to summation-order rounding, the Mach
+    maximum (order-independent) bit for bit."""
+    import aero3d
+    g = Golden("readback_transport60k")
+    b = Golden("transport60k")
+    F, fix = g.s["F"], g.s["fix"]
+    s = make_hip(b, F, fix, 0.5)
+    fl = b.force_list()
+    s.set_force_list(fl)
+    grid, geo = b.grid(), b.geo()
+    r = g.s["ref"]
+    ref = (g.s["alpha"], g.s["M"], r["S"], r["xcg"], 0.0, r["c"])
+    xT = g.s["reads"][0]["xT"]
+    o0 = s.readback(*ref, xT)
+    assert o0["it"] == 0 and np.isnan(o0["res"])
+    s.step(30)
+    Qa = s.read()
+    o1 = s.readback(*ref, xT)
+    s.step(20)
+    Qb = s.read()
+    o2 = s.readback(*ref, xT)
+    assert o2["it"] == 50
+    assert rel(o1["res"], O.residual(geo["t"], Qa[0], np.full(len(Qa[0]), F[0], np.float32), 30)) < 1e-12
+    assert rel(o2["res"], O.residual(geo["t"], Qb[0], Qa[0], 20)) < 1e-12
+    fo = O.forces(Qb, fl, *ref)
+    for k in ("CL", "CD", "Cm"):
+        assert rel(o2[k], fo[k]) < 1e-12, (k, o2[k], fo[k])
+    assert rel(o2["CDi"], O.trefftz(grid, geo["t"], Qb, xT, g.s["alpha"], g.s["M"], r["S"])) < 1e-12
+    assert o2["Mmax"] == O.mach_max(geo["t"], Qb)
+    say("readback at 50", o2)
+    # no force list, no Trefftz plane: NaN coefficients and CDi, the rest as before
+    s2 = make_hip(b, F, fix, 0.5)
+    s2.step(5)
+    o = s2.readback(*ref)
+    assert np.isnan(o["CL"]) and np.isnan(o["CDi"]) and o["Mmax"] == O.mach_max(geo["t"], s2.read())
+    s.dispose(); s2.dispose()
+
+
+@pytest.mark.parametrize("case,base", [("readback_sphere", "sphere"), ("readback_fighter30k", "fighter30k")])
+def test_converges_like_the_reference_controller(case, base):
+    """The reference's own readback controller (CFD3.readback, run to its
+    convergence flag on CfdCPU) against the HIP solver read back every 50
+    iterations under the same rule (aero3d.ConvRule, reference rule).
+    At every common readback the coefficients, CDi and the Mach maximum agree
+    within 1e-3 relative and the residual within 25 % (it is the difference of
+    two states 50 iterations apart, so FP32 arithmetic shows in it first); the
+    rule, whose residual ratio falls about 10 % per readback near 2e-3, fires
+    within three readbacks of the reference's; the state at the reference's
+    convergence iteration is within 1e-5 per cell. For the sphere this gives
+    C_d of the reference's scheme at radius 4.8 cells on this grid, 0.4390
+    (the README's 0.56, README.md:99, is quoted for a grid it does not state)."""
+    import aero3d
+    g, b = Golden(case), Golden(base)
+    F = g.s["F"]
+    s = make_hip(b, F, [F, F, F], g.s["cfl"])
+    s.set_force_list(b.force_list())
+    r = g.s["ref"]
+    ref = (g.s["alpha"], g.s["M"], r["S"], r["xcg"], r.get("zcg") or 0.0, r["c"])
+    rule = aero3d.ConvRule(aero3d.RULE_REFERENCE, cap=30000)
+    hist = s.solve(ref, rule, 50, g.s["reads"][0]["xT"])
+    last, want = hist[-1], g.s["reads"][-1]
+    say(case, "HIP converged at", last["it"], "CL", last["CL"], "CD", last["CD"], "CDi", last["CDi"], "Mmax", last["Mmax"],
+        "| reference at", want["it"], "CL", want["CL"], "CD", want["CD"], "CDi", want["CDi"], "Mmax", want["Mmax"])
+    assert rule.conv and abs(last["it"] - want["it"]) <= 150
+    mine = {h["it"]: h for h in hist}
+    worst = {}
+    for rd in g.s["reads"]:
+        h = mine.get(rd["it"])
+        if h is None or rd["res"] is None:
+            continue
+        keys = ("CD", "CDi", "Mmax") + (("CL", "Cm") if case != "readback_sphere" else ())
+        for k in keys + ("res",):
+            worst[k] = max(worst.get(k, 0.0), rel(h[k], rd[k]))
+        say(case, rd["it"], "res", h["res"], rd["res"], "ratio", h["res"] / rule.res0, rd["res"] / rd["res0"], "CD", h["CD"], rd["CD"])
+    say(case, "largest relative differences over the common readbacks", worst)
+    for k, v in worst.items():
+        assert v < (0.25 if k == "res" else 1e-3), (k, v)
+    if case == "readback_sphere":
+        assert max(abs(h["CL"]) for h in hist) < 1e-5
+    s.reset()
+    s.step(want["it"])
+    d = rel_diff(s.read(), g.state("final"), b.geo()["t"])
+    say(case, "state at the reference's convergence iteration", want["it"], d)
+    assert max(d.values()) < STATE_TOL, d
+    s.dispose()
