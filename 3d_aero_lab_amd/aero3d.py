@@ -227,12 +227,12 @@ class CfdHIP:
         return {k: getattr(o, k) for k, _ in ReadbackOut._fields_}
 
     def solve(self, ref, rule, every=50, xT=float("nan")):
-        """Step to convergence under `rule` (a ConvRule), one readback at
-        iteration 0 and every `every` iterations (and at the cap of the
-        residual-drop rule); returns the readback history. ref: (alpha, M, S,
+        """Step to convergence under `rule` (a ConvRule), one readback every
+        `every` iterations (and at the cap of the residual-drop rule); the
+        first takes the density snapshot only, as the reference controller's
+        first readback does. Returns the readback history. ref: (alpha, M, S,
         xcg, zcg, cref)."""
-        hist = [self.readback(*ref, xT)]
-        rule.update(hist[-1]["res"], hist[-1]["CL"], hist[-1]["it"])
+        hist = []
         while not rule.done:
             it = self.it
             nxt = (it // every + 1) * every

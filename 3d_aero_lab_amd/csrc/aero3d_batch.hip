@@ -22,10 +22,9 @@
 //     form.
 // Server: a FIFO of solve requests admitted by HBM footprint into the batch
 // as memory frees up (continuous batching). Each tick is enqueued without
-// waiting: the admission readback of new cases, the steps to the next
-// readback of any case, then the readback of every case due (density
-// residual and forces through the shared segment reductions) and the
-// convergence rule, decided on the device: a finished case is frozen there
+// waiting: the steps to the next readback of any case, then the readback of
+// every case due (density residual and forces through the shared segment
+// reductions) and the convergence rule, decided on the device: a finished case is frozen there
 // (its work items exit at once) until the host, which lags one tick behind,
 // retires it and frees its slot. The host keeps two ticks in flight and
 // waits only for the older one's results.
@@ -917,7 +916,6 @@ int aero3d_server_tick(aero3d_server *s, int *n_active, int *n_pending) {
     if ((rc = process_oldest(s))) return rc;
   }
   // admissions (FIFO, by HBM footprint)
-  std::vector<int> admitted;
   while (!s->pending.empty()) {
     const aero3d_request r = s->pending.front();
     int64_t need = 0;
@@ -945,7 +943,6 @@ int aero3d_server_tick(aero3d_server *s, int *n_active, int *n_pending) {
     SrvCase &c = s->cases[slot];
     c.live = true; c.gen++; c.r = r; c.arrival = r.arrival; c.t_admit = clock_s(s); c.last_read_it = 0;
     s->live++;
-    admitted.push_back(slot);
     s->pending.pop_front();
   }
   if (s->live > 0) {
@@ -963,21 +960,15 @@ int aero3d_server_tick(aero3d_server *s, int *n_active, int *n_pending) {
     const StageLayout L(b->max_slots);
     char *p = b->ring.buf[T.buf];
     BCHK(b, hipEventRecord(T.ev0, b->stream));
-    auto seg_of = [&](int slot, bool first) {
+    // a case's first readback (no snapshot yet) only takes the snapshot: its
+    // residual is NaN and res0 comes from the second, as in the reference's controller
+    auto seg_of = [&](int slot) {
       const aero3d_request &r = s->cases[slot].r;
-      RbSeg x = make_seg(b, slot, true, true, r.alpha, r.M, r.S, r.xcg, r.zcg, r.cref);
-      if (first) x.nit = 0;
+      RbSeg x = make_seg(b, slot, true, true, r.alpha, std::max(r.M, 0.05), r.S, r.xcg, r.zcg, r.cref);
       x.rule = r.rule; x.cap = r.max_iters; x.res_drop = r.res_drop;
       x.skip = &b->d_stat[slot].frozen;
       return x;
     };
-    // admission readback (iteration 0): snapshot, forces at the reset state, first rule update
-    if (!admitted.empty()) {
-      std::vector<RbSeg> segs;
-      for (int slot : admitted) { segs.push_back(seg_of(slot, true)); b->slots[slot].snap_it = 0; }
-      if ((rc = enqueue_segments(b, segs, p + L.segA, b->d_seg))) return rc;
-      srv_finish_kernel<<<(int)segs.size(), kRedBlock, 0, b->stream>>>(b->d_seg, b->d_stat);
-    }
     // steps to the next readback of any resident case
     int n = 1 << 30;
     for (int slot = 0; slot < b->max_slots; slot++) {
@@ -993,7 +984,7 @@ int aero3d_server_tick(aero3d_server *s, int *n_active, int *n_pending) {
       if (!c.live) continue;
       Slot &S = b->slots[slot];
       if (S.it != next_check(c.r, S.it - n, s->check_every)) continue;
-      segs.push_back(seg_of(slot, false));
+      segs.push_back(seg_of(slot));
       S.snap_it = S.it;
       c.last_read_it = S.it;
       T.due.push_back(Due{slot, c.gen, S.cur, S.it});

@@ -133,10 +133,11 @@ class CfdMonitor {
     if (this.hist.length > 400) this.hist.splice(0, this.hist.length - 400);
     return o;
   }
-  /* Step to the rule's end: a read now, then one every `every` iterations
-   * (and at the cap of the residual-drop rule); returns the last readback. */
+  /* Step to the rule's end: a read every `every` iterations (and at the cap of
+   * the residual-drop rule), the first taking the density snapshot only, as
+   * the reference controller's first readback does; returns the last readback. */
   solve(every = 50) {
-    let o = this.read();
+    let o = null;
     while (!this.rule.done) {
       let nxt = (Math.floor(this.s.it / every) + 1) * every;
       if (this.rule.rule === RULE_RES_DROP) nxt = Math.min(nxt, this.rule.cap);

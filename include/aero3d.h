@@ -190,11 +190,12 @@ int aero3d_batch_time_steps(aero3d_batch *b, int n, float *ms);
  * time from its UI loop, src/air_cfd.js:539-578). Requests queue FIFO and are
  * admitted into a batch as slots and arena memory free up (admission by HBM
  * footprint, continuous batching). A tick enqueues, without waiting on the
- * stream: the admission readback of new cases, the iterations to the next
- * readback of any case, then the readback of every case due (density residual
- * since its previous readback and force coefficients, src/air_cfd.js:552-558)
- * and its convergence rule, decided on the device; a finished case is frozen
- * on the device at once. The host lags one tick: each tick processes the
+ * stream: the iterations to the next readback of any case, then the readback
+ * of every case due (density residual since its previous readback and force
+ * coefficients, src/air_cfd.js:552-558) and its convergence rule, decided on
+ * the device; a finished case is frozen on the device at once. A case's first
+ * readback only takes the density snapshot (residual NaN), as the reference
+ * controller's first readback does. The host lags one tick: each tick processes the
  * results of the tick before the last, retires the frozen cases with their
  * coefficients and frees their memory. Readbacks come every check_every
  * iterations from a case's admission (and at max_iters under the residual-
