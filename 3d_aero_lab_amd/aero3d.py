@@ -23,7 +23,7 @@ EXPORTS = ["aero3d_create", "aero3d_destroy", "aero3d_last_error", "aero3d_set_f
            "aero3d_set_opt", "aero3d_reset", "aero3d_step", "aero3d_iter", "aero3d_read", "aero3d_write",
            "aero3d_sync", "aero3d_set_force_list", "aero3d_forces", "aero3d_trefftz", "aero3d_residual",
            "aero3d_time_steps", "aero3d_set_tile", "aero3d_get_tile", "aero3d_autotune", "aero3d_footprint",
-           "aero3d_readback"]
+           "aero3d_readback", "aero3d_readback_static"]
 
 
 class Grid(C.Structure):
@@ -37,6 +37,23 @@ class ReadbackIn(C.Structure):
 
 class ReadbackOut(C.Structure):
     _fields_ = [("it", C.c_int)] + [(k, C.c_double) for k in ("res", "CL", "CD", "Cm", "CDi", "Mmax")]
+
+
+class Field(C.Structure):
+    """aero3d_field: a state on the device in arrays the caller owns (device
+    addresses as integers, xc on the host)."""
+    _fields_ = [("NX", C.c_int), ("NY", C.c_int), ("NZ", C.c_int), ("ns", C.c_uint64), ("Q", C.c_void_p), ("rsnap", C.c_void_p),
+                ("nit", C.c_int), ("t", C.c_void_p), ("dy", C.c_void_p), ("dz", C.c_void_p), ("xc", C.c_void_p),
+                ("fq", C.c_void_p), ("cx", C.c_void_p), ("cz", C.c_void_p), ("x", C.c_void_p), ("z", C.c_void_p), ("nf", C.c_int)]
+
+
+def readback_static(field, alpha, M, S, xcg, zcg, cref, xT=float("nan"), stream=None):
+    """aero3d_readback_static: the readback over caller-owned device arrays, with no allocation."""
+    o = ReadbackOut()
+    rc = lib().aero3d_readback_static(C.byref(field), C.byref(ReadbackIn(alpha, M, S, xcg, zcg, cref, xT)), C.byref(o), stream)
+    if rc:
+        raise Aero3DError(f"aero3d_readback_static failed ({rc}): {lib().aero3d_last_error(None).decode()}")
+    return {k: getattr(o, k) for k, _ in ReadbackOut._fields_}
 
 
 RULE_RES_DROP, RULE_REFERENCE = 0, 1
@@ -113,6 +130,7 @@ def lib():
         L.aero3d_get_tile.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.aero3d_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
         L.aero3d_readback.argtypes = [vp, C.POINTER(ReadbackIn), C.POINTER(ReadbackOut)]
+        L.aero3d_readback_static.argtypes = [C.POINTER(Field), C.POINTER(ReadbackIn), C.POINTER(ReadbackOut), vp]
         _lib = L
     return _lib
 

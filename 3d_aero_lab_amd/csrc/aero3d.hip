@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -294,6 +295,41 @@ int enqueue_trefftz(aero3d_solver *s, double xT, double alphaDeg) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of aero3d_readback_static: block partials of the residual and force
+// sums (5 kRedMaxBlocks), of the Mach maximum and of the Trefftz sum (one
+// kRedMaxBlocks each), the sums, and the segment record. Every sum is grid-
+// strided over at most kRedMaxBlocks blocks, so the size does not depend on
+// the field's size. Each device holds its own copy until the process ends;
+// host code uses the address hipGetSymbolAddress returns for the current
+// device, never the array's name.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchPart = 7 * (size_t)kRedMaxBlocks;        // doubles
+constexpr size_t kScratchSums = 16;                                // doubles
+constexpr size_t kScratchBytes = (kScratchPart + kScratchSums) * sizeof(double) + ((sizeof(RbSeg) + 255) / 256) * 256;
+__device__ __align__(256) char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+constexpr int64_t kStaticMaxCells = int64_t(1) << 30;   // 32-bit element offsets in the kernels
+constexpr int kStaticMaxForces = 1 << 30;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Address of this device's scratch, looked up once per device (the current one).
+int scratch_of_device(void **out, std::string &err) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) { err = "readback_static: no usable HIP device"; return AERO3D_E_NODEV; }
+  std::lock_guard<std::mutex> l(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) { err = std::string("readback_static: hipGetSymbolAddress: ") + hipGetErrorString(e); return AERO3D_E_HIP_BASE - (int)e; }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -501,6 +537,74 @@ int aero3d_readback(aero3d_solver *s, const aero3d_readback_in *in, aero3d_readb
   out->it = it;
   out->res = o[5];
   out->CL = s->fq ? o[6] : NAN; out->CD = s->fq ? o[7] : NAN; out->Cm = s->fq ? o[8] : NAN;
+  out->CDi = tz ? 2 * o[kRbOut] / (Mn * Mn * in->S) : NAN;
+  out->Mmax = std::sqrt(o[kRbOut + 1]);
+  return 0;
+}
+
+int aero3d_readback_static(const aero3d_field *f, const aero3d_readback_in *in, aero3d_readback_out *out, void *stream) {
+  static thread_local std::string err;
+  if (!f || !in || !out) { err = "readback_static: null argument"; g_create_error = err; return AERO3D_E_ARG; }
+  const int64_t N = (int64_t)f->NX * f->NY * f->NZ;
+  if (f->NX < 1 || f->NY < 1 || f->NZ < 1 || N > kStaticMaxCells || f->ns < (uint64_t)N || f->nf < 0 || f->nf > kStaticMaxForces ||
+      f->nit < 0 || !f->Q || !f->t || !f->rsnap || (f->nf > 0 && (!f->fq || !f->cx || !f->cz || !f->x || !f->z))) {
+    g_create_error = "readback_static: bad field (NX, NY, NZ >= 1, NX*NY*NZ <= 2^30, ns >= N, 0 <= nf <= 2^30, nit >= 0, non-null arrays)";
+    return AERO3D_E_ARG;
+  }
+  const bool tz = !std::isnan(in->xT);
+  if (tz && (f->NX < 2 || !f->xc || !f->dy || !f->dz)) { g_create_error = "readback_static: the Trefftz plane needs NX >= 2, xc, dy and dz"; return AERO3D_E_ARG; }
+  void *base = nullptr;
+  if (int rc = scratch_of_device(&base, err)) { g_create_error = err; return rc; }
+  double *part = (double *)base, *sums = part + kScratchPart;
+  RbSeg *dseg = (RbSeg *)(sums + kScratchSums);
+  hipStream_t st = (hipStream_t)stream;
+  const double Mn = std::max(in->M, 0.05);
+  auto chk = [&](hipError_t e, const char *what) {
+    if (e == hipSuccess) return 0;
+    g_create_error = std::string("readback_static: ") + what + ": " + hipGetErrorString(e);
+    return AERO3D_E_HIP_BASE - (int)e;
+  };
+  int rc;
+  const size_t ns = (size_t)f->ns;
+  if (tz) {   // cfdLocate on the caller's FP32 centres (src/air_cfd.js:453), then the plane sum
+    const float *c = f->xc;
+    const int n = f->NX;
+    int i0; double tx;
+    if (in->xT <= c[0]) { i0 = 0; tx = 0; }
+    else if (in->xT >= c[n - 1]) { i0 = n - 2; tx = 1; }
+    else {
+      int lo = 0, hi = n - 1;
+      while (hi - lo > 1) { const int m = (lo + hi) >> 1; if (c[m] <= in->xT) lo = m; else hi = m; }
+      i0 = lo; tx = (in->xT - c[lo]) / ((double)c[hi] - c[lo]);
+    }
+    const double a = in->alpha * (M_PI / 180);
+    const int nb = reduce_blocks(f->NY * f->NZ);
+    trefftz_kernel<<<nb, kRedBlock, 0, st>>>(f->Q, ns, f->t, f->dy, f->dz, f->NX, f->NY, f->NZ, i0, tx, std::cos(a), std::sin(a),
+                                            part + 6 * kRedMaxBlocks);
+    finish_kernel<1><<<1, kRedBlock, 0, st>>>(part + 6 * kRedMaxBlocks, nb, sums + kRbOut);
+  }
+  const int nbm = reduce_blocks((int)((N + 1) / 2));
+  machmax_kernel<<<nbm, kRedBlock, 0, st>>>(f->Q, ns, f->t, (int)N, part + 5 * kRedMaxBlocks);
+  machmax_finish<<<1, 64, 0, st>>>(part + 5 * kRedMaxBlocks, nbm, sums + kRbOut + 1);
+  RbSeg g{};
+  g.Q = f->Q; g.ns = ns; g.t = f->t; g.N = (int)N; g.rsnap = f->rsnap; g.nit = f->nit;
+  g.fq = f->fq; g.cx = f->cx; g.cz = f->cz; g.x = f->x; g.z = f->z; g.nf = f->nf;
+  g.xcg = in->xcg; g.zcg = in->zcg;
+  const double a = in->alpha * (M_PI / 180);
+  g.ca = std::cos(a); g.sa = std::sin(a); g.kq = 2 / (0.5 * Mn * Mn * in->S); g.cref = in->cref;
+  g.nb_res = reduce_blocks((int)((N + 2) / 3));
+  g.nb_f = f->nf > 0 ? reduce_blocks(f->nf) : 0;
+  g.part = part; g.out = sums;
+  if ((rc = chk(hipMemcpyAsync(dseg, &g, sizeof g, hipMemcpyHostToDevice, st), "segment copy"))) return rc;
+  rb_partials_kernel<<<g.nb_res + g.nb_f, kRedBlock, 0, st>>>(dseg, 1);
+  rb_finish_kernel<<<1, kRedBlock, 0, st>>>(dseg);
+  if ((rc = chk(hipGetLastError(), "launch"))) return rc;
+  double o[kRbOut + 2];
+  if ((rc = chk(hipMemcpyAsync(o, sums, sizeof o, hipMemcpyDeviceToHost, st), "result copy"))) return rc;
+  if ((rc = chk(hipStreamSynchronize(st), "wait"))) return rc;
+  out->it = -1;
+  out->res = o[5];
+  out->CL = f->nf > 0 ? o[6] : NAN; out->CD = f->nf > 0 ? o[7] : NAN; out->Cm = f->nf > 0 ? o[8] : NAN;
   out->CDi = tz ? 2 * o[kRbOut] / (Mn * Mn * in->S) : NAN;
   out->Mmax = std::sqrt(o[kRbOut + 1]);
   return 0;

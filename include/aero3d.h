@@ -110,6 +110,38 @@ typedef struct {
 } aero3d_readback_out;
 int aero3d_readback(aero3d_solver *s, const aero3d_readback_in *in, aero3d_readback_out *out);
 
+/* No reference counterpart: a variant of aero3d_readback for callers that
+ * cannot allocate device memory (code under stream capture, serving loops
+ * that own a fixed arena). It works on device arrays the caller owns (no
+ * handle) and calls no hipMalloc / hipFree: its block partials and sums live
+ * in a static device buffer of about 58 KB per GPU, held until the process
+ * ends, whose size does not depend on the field's (every sum is strided over
+ * at most 1024 blocks); fields of up to 2^30 cells and force lists of up to
+ * 2^30 entries, larger ones are rejected with AERO3D_E_ARG. The buffer is
+ * shared by every call on a GPU: calls that overlap on one GPU race. Results
+ * equal aero3d_readback's on the same state bit for bit (same kernels).
+ * Q: 5 planes of ns floats; rsnap: the density snapshot (N floats, every third
+ * cell read and rewritten with the current density); nit: iterations since
+ * the snapshot (0: snapshot only, res = NaN); t, dy, dz, fq..z: device arrays;
+ * xc: the FP32 x cell centres on the HOST (Trefftz plane lookup), needed with
+ * dy, dz only when in->xT is not NaN. `stream` is a hipStream_t (null: the
+ * default stream); the call waits for it. out->it is -1. Errors: message in
+ * aero3d_last_error(NULL). */
+typedef struct {
+  int NX, NY, NZ;
+  uint64_t ns;
+  const float *Q;
+  float *rsnap;
+  int nit;
+  const uint8_t *t;
+  const float *dy, *dz;
+  const float *xc;
+  const int32_t *fq;
+  const double *cx, *cz, *x, *z;
+  int nf;
+} aero3d_field;
+int aero3d_readback_static(const aero3d_field *f, const aero3d_readback_in *in, aero3d_readback_out *out, void *stream);
+
 /* Benchmark helper: run n iterations on the handle's stream bracketed by HIP
  * events recorded on that stream; *ms = elapsed device time. */
 int aero3d_time_steps(aero3d_solver *s, int n, float *ms);
