@@ -544,7 +544,7 @@ class AxiZones(C.Structure):
 
 AXI_EXPORTS = ["axi_create", "axi_destroy", "axi_last_error", "axi_set_opt", "axi_set_comps", "axi_set_zones", "axi_set_fs",
                "axi_set_state", "axi_step", "axi_iter", "axi_read", "axi_sync", "axi_time_steps", "axi_monitor",
-               "axi_monitor_sections"]
+               "axi_monitor_sections", "axi_set_fused", "axi_autotune"]
 
 
 class AxiMonitorOut(C.Structure):
@@ -575,6 +575,8 @@ def alib():
         L.axi_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_float)]
         L.axi_monitor.argtypes = [vp, f64p, C.c_double, C.POINTER(AxiMonitorOut)]
         L.axi_monitor_sections.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p]
+        L.axi_set_fused.argtypes = [vp, C.c_int]
+        L.axi_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)]
         _alib = L
     return _alib
 
@@ -633,6 +635,17 @@ class AxiHIP:
 
     def set_comps(self, c):
         self._chk(alib().axi_set_comps(self.h, np.ascontiguousarray(c, np.float64)))
+
+    def set_fused(self, form):
+        """0: face + cell kernels; 1 .. 4: one fused tile kernel per stage (tile shapes)."""
+        self._chk(alib().axi_set_fused(self.h, int(form)))
+
+    def autotune(self, iters=50):
+        """Keep the fastest kernel form for this grid; returns (form, ms per iteration). The state is kept."""
+        self._sync_opt()
+        f, ms = C.c_int(), C.c_float()
+        self._chk(alib().axi_autotune(self.h, iters, C.byref(f), C.byref(ms)))
+        return f.value, ms.value
 
     def set_zones(self, zones):
         z = self._zones(zones)

@@ -1052,6 +1052,29 @@ napi_value AxiMonitor(napi_env env, napi_callback_info info) {
   o[28] = m.bad;
   return undef(env);
 }
+// axiSetFused(h, form): kernel form of the uncapped steps (0 face + cell, 1 .. 4 fused tiles)
+napi_value AxiSetFused(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!args(env, info, 2, a)) return nullptr;
+  axi_solver *s = axi_handle(env, a[0]);
+  if (!s) return nullptr;
+  int rc = axi_set_fused(s, (int)num(env, a[1]));
+  return rc ? axi_throw(env, s, rc) : undef(env);
+}
+// axiAutotune(h, iters) -> {form, msPerIter}
+napi_value AxiAutotune(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!args(env, info, 2, a)) return nullptr;
+  axi_solver *s = axi_handle(env, a[0]);
+  if (!s) return nullptr;
+  int form = 0; float ms = 0;
+  int rc = axi_autotune(s, (int)num(env, a[1]), &form, &ms);
+  if (rc) return axi_throw(env, s, rc);
+  napi_value o;
+  NAPI_OK(napi_create_object(env, &o));
+  set_num(env, o, "form", form); set_num(env, o, "msPerIter", ms);
+  return o;
+}
 napi_value AxiSync(napi_env env, napi_callback_info info) {
   napi_value a[1];
   if (!args(env, info, 1, a)) return nullptr;
@@ -1077,7 +1100,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"serverTickAsync", ServerTickAsync}, {"serverPoll", ServerPoll}, {"serverClock", ServerClock}, {"serverStats", ServerStats},
       {"axiCreate", AxiCreate}, {"axiDispose", AxiDispose}, {"axiSetOpt", AxiSetOpt}, {"axiSetComps", AxiSetComps},
       {"axiSetZones", AxiSetZones}, {"axiSetFs", AxiSetFs}, {"axiSetState", AxiSetState}, {"axiStep", AxiStep},
-      {"axiRead", AxiRead}, {"axiSync", AxiSync}, {"axiMonitorSections", AxiMonitorSections}, {"axiMonitor", AxiMonitor}};
+      {"axiRead", AxiRead}, {"axiSync", AxiSync}, {"axiSetFused", AxiSetFused}, {"axiAutotune", AxiAutotune}, {"axiMonitorSections", AxiMonitorSections}, {"axiMonitor", AxiMonitor}};
   for (auto &f : fns) {
     napi_value fn;
     NAPI_OK(napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));

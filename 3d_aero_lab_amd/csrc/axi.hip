@@ -12,7 +12,13 @@
 //   cell kernel  one thread per cell; flux divergence with r-weighted faces,
 //                wall / geometric pressure source, blade-row work and the
 //                relaxed heat source, local dt (stage 1), RK update, floors.
-// Per SSP-RK2 iteration: face(Q), cell(Q -> Q1, dt), face(Q1), cell(Q1, Q -> Q).
+//   fused kernel one block of 512 threads per 63 x 7 tile: the tile's
+//                primitives (with a 2-cell ring) and face fluxes in LDS, then
+//                the cell updates;
+//                the same arithmetic as face + cell, one launch per stage.
+// Per SSP-RK2 iteration: fused(Q -> Q1, dt), fused(Q1, Q -> Q); with the
+// global / dtRatio dt caps face(Q), cell(Q, dt only), dt minimum, cap,
+// cell(Q -> Q1, capped dt), face(Q1), cell(Q1, Q -> Q).
 // Device layout: Q and Q1 as 5 planes of N floats, face fluxes Fx 5 x
 // (W+1)H, Fr 5 x W(H+1), face eddy viscosity W(H+1), dt / src / mfx N floats.
 // ============================================================================
@@ -50,6 +56,8 @@ struct axi_solver {
   hipGraph_t graph = nullptr;     // captured chunk of kGraphIters iterations
   hipGraphExec_t exec = nullptr;
   bool graph_capped = false;
+  int fused = 0;                  // uncapped steps: face + cell kernels (0) or a fused tile kernel form (1 .. 4)
+  int graph_fused = -1;
   int global = 0;
   double dtRatio = 0;
   // readback monitors (axi_monitor): FP64 grid widths, the density of the last
@@ -101,6 +109,23 @@ int upload_zones(axi_solver *s, const axi_zones *z) {
 
 constexpr int kGraphIters = 25;
 
+// Both stages of one iteration with the fused kernel of tile height TH and NW waves per block.
+template <int TH, int NW>
+void fused_iteration(axi_solver *s, const AxiView &v, const AxiConsts *c, int ntx) {
+  const int nb = ntx * ((s->H + TH - 1) / TH);
+  fused_kernel<0, TH, NW><<<nb, 64 * NW, 0, s->stream>>>(v, c, s->Q, s->Q, s->Q1, s->dt, s->src, s->mfx, ntx);
+  fused_kernel<1, TH, NW><<<nb, 64 * NW, 0, s->stream>>>(v, c, s->Q1, s->Q, s->Q, s->dt, s->src, s->mfx, ntx);
+}
+// fused form f: tiles of 63 x (3, 7, 9, 5) cells, blocks of (4, 8, 16, 4) waves
+void launch_fused(axi_solver *s, const AxiView &v, const AxiConsts *c, int ntx) {
+  switch (s->fused) {
+    case 1: fused_iteration<3, 4>(s, v, c, ntx); break;
+    case 2: fused_iteration<7, 8>(s, v, c, ntx); break;
+    case 3: fused_iteration<9, 16>(s, v, c, ntx); break;
+    default: fused_iteration<5, 4>(s, v, c, ntx); break;
+  }
+}
+
 void enqueue_iters(axi_solver *s, int n) {
   const AxiView v = view(s);
   const int NF = (s->W + 1) * s->H + s->W * (s->H + 1);
@@ -108,7 +133,12 @@ void enqueue_iters(axi_solver *s, int n) {
   const bool capped = s->global || s->dtRatio != 0;
   const int nbm = std::min(1024, (s->N + kB - 1) / kB);
   const AxiConsts *c = s->dc;
+  const int ntx = (s->W + kTW - 1) / kTW;
   for (int k = 0; k < n; k++) {
+    if (!capped && s->fused) {     // one launch per stage, face fluxes in LDS
+      launch_fused(s, v, c, ntx);
+      continue;
+    }
     face_kernel<<<gf, kB, 0, s->stream>>>(v, c, s->Q, s->Fx, s->Fr, s->nutF);
     if (capped) {
       cell_kernel<<<gc, kB, 0, s->stream>>>(v, c, s->Q, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, 2, s->cap);
@@ -134,7 +164,7 @@ int steps(axi_solver *s, int n) {
     s->dc_dirty = false;
   }
   const bool capped = s->global || s->dtRatio != 0;
-  if (s->exec && s->graph_capped != capped) {
+  if (s->exec && (s->graph_capped != capped || s->graph_fused != s->fused)) {
     (void)hipGraphExecDestroy(s->exec); (void)hipGraphDestroy(s->graph);
     s->exec = nullptr; s->graph = nullptr;
   }
@@ -146,6 +176,7 @@ int steps(axi_solver *s, int n) {
       ACHK(s, hipStreamEndCapture(s->stream, &s->graph));
       ACHK(s, hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
       s->graph_capped = capped;
+      s->graph_fused = s->fused;
     }
     for (; k + kGraphIters <= n; k += kGraphIters) ACHK(s, hipGraphLaunch(s->exec, s->stream));
   }
@@ -199,6 +230,14 @@ int axi_create(const axi_grid *g, const axi_zones *z, const double comps[32], co
         hipMemcpy(s->dxD, dxD.data(), 8 * dxD.size(), hipMemcpyHostToDevice))
       return bail(afail(s, -1001, "axi_create: upload failed"));
   }
+  {  // default kernel form: tiles of 63 x 9 cells in blocks of 16 waves when they make between half and all
+     // of one block per CU (one round, each CU's block spread over its four SIMDs), else face + cell kernels
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0) {
+      const long long b3 = (long long)((s->W + kTW - 1) / kTW) * ((s->H + 8) / 9);
+      s->fused = (2 * b3 >= ncu && b3 <= ncu) ? 3 : 0;
+    }
+  }
   int rc_;
   if ((rc_ = upload_zones(s, z))) return bail(rc_);
   if ((rc_ = axi_set_state(s, Q0))) return bail(rc_);
@@ -228,6 +267,11 @@ int axi_set_opt(axi_solver *s, double cfl, int order, int visc, double hf, int g
   s->global = global ? 1 : 0; s->dtRatio = dtRatio;
   s->c.capf = s->global ? 1.f : (float)dtRatio;
   s->dc_dirty = true;
+  return 0;
+}
+int axi_set_fused(axi_solver *s, int on) {
+  if (!s || on < 0 || on > 4) return afail(s, -1, "set_fused: 0 .. 4");
+  s->fused = on;                 // the next step re-captures its graph
   return 0;
 }
 int axi_set_comps(axi_solver *s, const double comps[32]) {
@@ -323,6 +367,40 @@ int axi_sync(axi_solver *s) {
   ACHK(s, hipStreamSynchronize(s->stream));
   return 0;
 }
+int axi_autotune(axi_solver *s, int iters, int *form, float *ms_best) {
+  if (!s || iters < 1) return afail(s, -1, "autotune: iters >= 1");
+  // time on a copy: save the state and the diagnostics, restore after
+  const size_t b = 4 * (size_t)s->N;
+  float *save = nullptr;
+  ACHK(s, hipMalloc((void **)&save, 8 * b));
+  struct Free { float *p; ~Free() { if (p) (void)hipFree(p); } } fr{save};
+  ACHK(s, hipMemcpyAsync(save, s->Q, 5 * b, hipMemcpyDeviceToDevice, s->stream));
+  ACHK(s, hipMemcpyAsync(save + 5 * (size_t)s->N, s->dt, b, hipMemcpyDeviceToDevice, s->stream));
+  ACHK(s, hipMemcpyAsync(save + 6 * (size_t)s->N, s->src, b, hipMemcpyDeviceToDevice, s->stream));
+  ACHK(s, hipMemcpyAsync(save + 7 * (size_t)s->N, s->mfx, b, hipMemcpyDeviceToDevice, s->stream));
+  const int it0 = s->it, f0 = s->fused;
+  int best = f0;
+  float tbest = 1e30f;
+  for (int f = 0; f <= 4; f++) {
+    s->fused = f;
+    float ms;
+    int rc = axi_time_steps(s, kGraphIters, &ms);   // warm (and capture)
+    if (!rc) rc = axi_time_steps(s, iters, &ms);
+    if (rc) { s->fused = f0; return rc; }
+    if (ms < tbest) { tbest = ms; best = f; }
+  }
+  s->fused = best;
+  s->it = it0;
+  ACHK(s, hipMemcpyAsync(s->Q, save, 5 * b, hipMemcpyDeviceToDevice, s->stream));
+  ACHK(s, hipMemcpyAsync(s->dt, save + 5 * (size_t)s->N, b, hipMemcpyDeviceToDevice, s->stream));
+  ACHK(s, hipMemcpyAsync(s->src, save + 6 * (size_t)s->N, b, hipMemcpyDeviceToDevice, s->stream));
+  ACHK(s, hipMemcpyAsync(s->mfx, save + 7 * (size_t)s->N, b, hipMemcpyDeviceToDevice, s->stream));
+  ACHK(s, hipStreamSynchronize(s->stream));
+  if (form) *form = best;
+  if (ms_best) *ms_best = tbest / iters;
+  return 0;
+}
+
 int axi_time_steps(axi_solver *s, int n, float *ms) {
   if (!s || !ms || n < 0) return -1;
   struct Ev { hipEvent_t e = nullptr; ~Ev() { if (e) (void)hipEventDestroy(e); } } e0, e1;   // released on every path

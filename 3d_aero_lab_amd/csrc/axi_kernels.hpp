@@ -113,54 +113,41 @@ __device__ __forceinline__ void recon(const Prim &A, const Prim &Am, bool hm, co
   for (int q = 0; q < 5; q++) { L[q] = badL ? A5[q] : L[q]; R[q] = badR ? B5[q] : R[q]; }
 }
 
-// One thread per face (src/cfd_cpu.js:97-143). Every load of a face (the four
-// cells' states and volume fractions, the face's aperture, the solid flags) is
-// issued before any of them is tested, at indices clamped into the grid: one
-// memory round trip per face instead of one per decision. Closed faces compute
-// a flux too and store 0 through a select.
-__global__ void __launch_bounds__(kB) face_kernel(AxiView v, const AxiConsts *__restrict__ cp, const float *__restrict__ Q, float *__restrict__ Fx,
-                                                  float *__restrict__ Fr, float *__restrict__ nutF) {
-  const AxiConsts &c = *cp;
-  const int W = v.W, H = v.H, N = v.N;
-  const int NXF = (W + 1) * H, NRF = W * (H + 1);
-  const int t = blockIdx.x * kB + threadIdx.x;
+// x face between cells a = f - 1 and b = f of one row (src/cfd_cpu.js:98-113):
+// the aperture-weighted flux (0 where the face is closed). Pm / Pp are the
+// outer cells a - 1 / b + 1; every cell index is clamped into the row by the
+// caller, and the tests here are value selects on already-loaded data.
+__device__ __forceinline__ void x_face(const AxiConsts &c, int W, int f, const Prim &Pm, const Prim &Pa, const Prim &Pb, const Prim &Pp,
+                                       float pm, float pa, float pb, float pp, bool sa, bool sb, float aea, float *out) {
+  const int a = f - 1, b = f;
   const bool second = c.order == 2;
   float F[5], G[5], L[5], R[5];
-  if (t < NXF) {
-    const int j = t / (W + 1), f = t % (W + 1), a = f - 1, b = f;
-    // cells a-1, a, b, b+1 of row j, clamped into the row
-    const int ia = a < 0 ? 0 : a, ib = b >= W ? W - 1 : b, im = ia > 0 ? ia - 1 : 0, ip = ib + 1 < W ? ib + 1 : W - 1;
-    const int ka = j * W + ia, kb = j * W + ib, km = j * W + im, kp = j * W + ip;
-    const uint8_t sa = v.solid[ka], sb = v.solid[kb];
-    const float aea = v.ae[ka], pa = v.phi[ka], pb = v.phi[kb], pm = v.phi[km], pp = v.phi[kp];
-    const Prim Pa = prim(Q, N, ka), Pb = prim(Q, N, kb), Pm = prim(Q, N, km), Pp = prim(Q, N, kp);
-    const float al = a < 0 ? (sb ? 0.f : 1.f) : b >= W ? (sa ? 0.f : 1.f) : aea;
-    if (a < 0 || b >= W) {                 // farfield ghost at x = lo / x = hi (two faces per row)
-      const Prim &P = a < 0 ? Pb : Pa;
-      farfield(c, P, a < 0 ? -1.f : 1.f, 0.f, G);
-      if (a < 0) hllc(G[0], G[1], G[2], G[3], G[4], 1.4f - 0.07f * G[4], Pb.r, Pb.u, Pb.v, Pb.p, Pb.Y, Pb.g, F);
-      else hllc(Pa.r, Pa.u, Pa.v, Pa.p, Pa.Y, Pa.g, G[0], G[1], G[2], G[3], G[4], 1.4f - 0.07f * G[4], F);
-    } else {
-      const bool fa = pa > 0.999f && pb > 0.999f;
-      const bool hm = fa && a - 1 >= 0 && pm > 0.999f, hp = fa && b + 1 < W && pp > 0.999f;
-      recon(Pa, Pm, hm, Pb, Pp, hp, 0, second, L, R);
-      hllc(L[0], L[1], L[2], L[3], L[4], 1.4f - 0.07f * L[4], R[0], R[1], R[2], R[3], R[4], 1.4f - 0.07f * R[4], F);
-    }
-    const bool open = al > 0.f;
-#pragma unroll
-    for (int q = 0; q < 5; q++) Fx[q * NXF + t] = open ? al * F[q] : 0.f;
-    return;
+  const float al = a < 0 ? (sb ? 0.f : 1.f) : b >= W ? (sa ? 0.f : 1.f) : aea;
+  if (a < 0 || b >= W) {                 // farfield ghost at x = lo / x = hi (two faces per row)
+    const Prim &P = a < 0 ? Pb : Pa;
+    farfield(c, P, a < 0 ? -1.f : 1.f, 0.f, G);
+    if (a < 0) hllc(G[0], G[1], G[2], G[3], G[4], 1.4f - 0.07f * G[4], Pb.r, Pb.u, Pb.v, Pb.p, Pb.Y, Pb.g, F);
+    else hllc(Pa.r, Pa.u, Pa.v, Pa.p, Pa.Y, Pa.g, G[0], G[1], G[2], G[3], G[4], 1.4f - 0.07f * G[4], F);
+  } else {
+    const bool fa = pa > 0.999f && pb > 0.999f;
+    const bool hm = fa && a - 1 >= 0 && pm > 0.999f, hp = fa && b + 1 < W && pp > 0.999f;
+    recon(Pa, Pm, hm, Pb, Pp, hp, 0, second, L, R);
+    hllc(L[0], L[1], L[2], L[3], L[4], 1.4f - 0.07f * L[4], R[0], R[1], R[2], R[3], R[4], 1.4f - 0.07f * R[4], F);
   }
-  const int u = t - NXF;
-  if (u >= NRF) return;
-  const int f = u / W, i = u % W, a = f - 1, b = f;
-  // cells a-1, a, b, b+1 of column i, clamped into the column
-  const int ja = a < 0 ? 0 : a, jb = b >= H ? H - 1 : b, jm = ja > 0 ? ja - 1 : 0, jp = jb + 1 < H ? jb + 1 : H - 1;
-  const int ka = ja * W + i, kb = jb * W + i, km = jm * W + i, kp = jp * W + i;
-  const uint8_t sa = v.solid[ka];
-  const float ana = v.an[ka], pa = v.phi[ka], pb = v.phi[kb], pm = v.phi[km], pp = v.phi[kp];
-  const float la = v.lmix[ka], lb = v.lmix[kb], rca = v.rc[ja], rcb = v.rc[jb];
-  const Prim Pa = prim(Q, N, ka), Pb = prim(Q, N, kb), Pm = prim(Q, N, km), Pp = prim(Q, N, kp);
+  const bool open = al > 0.f;
+#pragma unroll
+  for (int q = 0; q < 5; q++) out[q] = open ? al * F[q] : 0.f;
+}
+
+// r face between rows a = f - 1 and b = f of one column (src/cfd_cpu.js:115-143):
+// out = the stored flux in (rho, rho u, rho v, E, rho Y) order with the
+// mixing-length shear, heat and species fluxes, *nut its eddy viscosity.
+__device__ __forceinline__ void r_face(const AxiConsts &c, int H, int f, const Prim &Pm, const Prim &Pa, const Prim &Pb, const Prim &Pp,
+                                       float pm, float pa, float pb, float pp, bool sa, float ana, float la, float lb, float rca,
+                                       float rcb, float *out, float *nutp) {
+  const int a = f - 1, b = f;
+  const bool second = c.order == 2;
+  float F[5], G[5], L[5], R[5];
   float G2[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
   float nut = 0.f;
   const float al = a < 0 ? 0.f : b >= H ? (sa ? 0.f : 1.f) : ana;
@@ -187,58 +174,98 @@ __global__ void __launch_bounds__(kB) face_kernel(AxiView v, const AxiConsts *__
     }
   }
   const bool open = al > 0.f;
-  Fr[u] = open ? al * F[0] : 0.f;
-  Fr[NRF + u] = open ? al * (F[2] - G2[1]) : 0.f;
-  Fr[2 * NRF + u] = open ? al * F[1] : 0.f;
-  Fr[3 * NRF + u] = open ? al * (F[3] - G2[3]) : 0.f;
-  Fr[4 * NRF + u] = open ? al * (F[4] - G2[4]) : 0.f;
-  nutF[u] = open ? nut : 0.f;
+  out[0] = open ? al * F[0] : 0.f;
+  out[1] = open ? al * (F[2] - G2[1]) : 0.f;
+  out[2] = open ? al * F[1] : 0.f;
+  out[3] = open ? al * (F[3] - G2[3]) : 0.f;
+  out[4] = open ? al * (F[4] - G2[4]) : 0.f;
+  *nutp = open ? nut : 0.f;
 }
 
-// One thread per cell (src/cfd_cpu.js:145-192). mode 0: stage 1 with dt,
-// 1: stage 2, 2: dt only, 3: stage 1 with dt capped at *cap. The cell's loads
-// (state, face fluxes, geometry, zone data, the stored dt) are issued before
-// the solid flag is tested; solid cells keep their state (modes 0 and 3 copy
-// it to the output) and write nothing else.
-__global__ void __launch_bounds__(kB) cell_kernel(AxiView v, const AxiConsts *__restrict__ cp, const float *__restrict__ Qin, const float *Qb,
-                                                  float *Qout, const float *__restrict__ Fx, const float *__restrict__ Fr,
-                                                  const float *__restrict__ nutF, float *__restrict__ dt, float *__restrict__ src,
-                                                  float *__restrict__ mfxo, int mode, const float *__restrict__ cap) {
+// One thread per face (src/cfd_cpu.js:97-143). Every load of a face (the four
+// cells' states and volume fractions, the face's aperture, the solid flags) is
+// issued before any of them is tested, at indices clamped into the grid: one
+// memory round trip per face instead of one per decision. Closed faces compute
+// a flux too and store 0 through a select.
+__global__ void __launch_bounds__(kB) face_kernel(AxiView v, const AxiConsts *__restrict__ cp, const float *__restrict__ Q, float *__restrict__ Fx,
+                                                  float *__restrict__ Fr, float *__restrict__ nutF) {
   const AxiConsts &c = *cp;
-  const int W = v.W, N = v.N;
-  const int k = blockIdx.x * kB + threadIdx.x;
-  if (k >= N) return;
-  const int NXF = (W + 1) * v.H, NRF = W * (v.H + 1);
-  const int i = k % W, j = k / W;
-  const int fx0 = j * (W + 1) + i, fx1 = fx0 + 1, fr0 = j * W + i, fr1 = fr0 + W;
-  // every load of the cell
-  const bool sol = v.solid[k] != 0;
-  const float *Qbb = mode == 1 ? Qb : Qin;
-  float q0[5], qb[5], fxa[5], fxb[5], fra[5], frb[5];
+  const int W = v.W, H = v.H, N = v.N;
+  const int NXF = (W + 1) * H, NRF = W * (H + 1);
+  const int t = blockIdx.x * kB + threadIdx.x;
+  float F[5];
+  if (t < NXF) {
+    const int j = t / (W + 1), f = t % (W + 1), a = f - 1, b = f;
+    // cells a-1, a, b, b+1 of row j, clamped into the row
+    const int ia = a < 0 ? 0 : a, ib = b >= W ? W - 1 : b, im = ia > 0 ? ia - 1 : 0, ip = ib + 1 < W ? ib + 1 : W - 1;
+    const int ka = j * W + ia, kb = j * W + ib, km = j * W + im, kp = j * W + ip;
+    const Prim Pa = prim(Q, N, ka), Pb = prim(Q, N, kb), Pm = prim(Q, N, km), Pp = prim(Q, N, kp);
+    x_face(c, W, f, Pm, Pa, Pb, Pp, v.phi[km], v.phi[ka], v.phi[kb], v.phi[kp], v.solid[ka] != 0, v.solid[kb] != 0, v.ae[ka], F);
 #pragma unroll
-  for (int q = 0; q < 5; q++) {
-    q0[q] = Qin[q * N + k]; qb[q] = Qbb[q * N + k];
-    fxa[q] = Fx[q * NXF + fx0]; fxb[q] = Fx[q * NXF + fx1]; fra[q] = Fr[q * NRF + fr0]; frb[q] = Fr[q * NRF + fr1];
+    for (int q = 0; q < 5; q++) Fx[q * NXF + t] = F[q];
+    return;
   }
-  const float dx = v.dx[i], dr = v.dr[j], ax0 = v.rf[j], ax1 = v.rf[j + 1], rcj = v.rc[j], ph = v.phi[k];
-  const float nu0 = nutF[j * W + i], nu1 = nutF[(j + 1) * W + i];
-  const float aek = v.ae[k], aem = v.ae[i > 0 ? k - 1 : k], ank = v.an[k], anm = v.an[j > 0 ? k - W : k];
-  const int id = v.comp[k];
-  const float mfd = v.mfd[k], wgt = v.wgt[k], tgtk = v.tgt[k], rate = v.rate[k], yrate = v.yrate[k];
-  const float dts = dt[k];
-  if (sol) {
+  const int u = t - NXF;
+  if (u >= NRF) return;
+  const int f = u / W, i = u % W, a = f - 1, b = f;
+  // cells a-1, a, b, b+1 of column i, clamped into the column
+  const int ja = a < 0 ? 0 : a, jb = b >= H ? H - 1 : b, jm = ja > 0 ? ja - 1 : 0, jp = jb + 1 < H ? jb + 1 : H - 1;
+  const int ka = ja * W + i, kb = jb * W + i, km = jm * W + i, kp = jp * W + i;
+  const Prim Pa = prim(Q, N, ka), Pb = prim(Q, N, kb), Pm = prim(Q, N, km), Pp = prim(Q, N, kp);
+  float nut;
+  r_face(c, H, f, Pm, Pa, Pb, Pp, v.phi[km], v.phi[ka], v.phi[kb], v.phi[kp], v.solid[ka] != 0, v.an[ka], v.lmix[ka], v.lmix[kb],
+         v.rc[ja], v.rc[jb], F, &nut);
+#pragma unroll
+  for (int q = 0; q < 5; q++) Fr[q * NRF + u] = F[q];
+  nutF[u] = nut;
+}
+
+// Everything a cell update reads besides its face fluxes.
+struct CellIn {
+  float q0[5], qb[5];
+  float dx, dr, ax0, ax1, rcj, ph, aek, aem, ank, anm, mfd, wgt, tgtk, rate, yrate, dts;
+  int id;
+  bool sol;
+};
+__device__ __forceinline__ void load_cell(const AxiView &v, const float *__restrict__ Qin, const float *Qb, const float *__restrict__ dt,
+                                          int mode, int k, int i, int j, CellIn &x) {
+  const int N = v.N, W = v.W;
+  const float *Qbb = mode == 1 ? Qb : Qin;
+#pragma unroll
+  for (int q = 0; q < 5; q++) { x.q0[q] = Qin[q * N + k]; x.qb[q] = Qbb[q * N + k]; }
+  x.sol = v.solid[k] != 0;
+  x.dx = v.dx[i]; x.dr = v.dr[j]; x.ax0 = v.rf[j]; x.ax1 = v.rf[j + 1]; x.rcj = v.rc[j]; x.ph = v.phi[k];
+  x.aek = v.ae[k]; x.aem = v.ae[i > 0 ? k - 1 : k]; x.ank = v.an[k]; x.anm = v.an[j > 0 ? k - W : k];
+  x.id = v.comp[k];
+  x.mfd = v.mfd[k]; x.wgt = v.wgt[k]; x.tgtk = v.tgt[k]; x.rate = v.rate[k]; x.yrate = v.yrate[k];
+  x.dts = dt[k];
+}
+
+// Cell update (src/cfd_cpu.js:145-192). mode 0: stage 1 with dt, 1: stage 2,
+// 2: dt only, 3: stage 1 with dt capped at *cap. fxa / fxb: x faces i and
+// i + 1, fra / frb: r faces j and j + 1, nu0 / nu1 their eddy viscosities.
+// Solid cells keep their state (modes 0 and 3 copy it to the output) and
+// write nothing else.
+__device__ __forceinline__ void cell_update(const AxiView &v, const AxiConsts &c, int k, int i, int j, const CellIn &x, const float *fxa,
+                                            const float *fxb, const float *fra, const float *frb, float nu0, float nu1, int mode,
+                                            const float *__restrict__ cap, float *Qout, float *__restrict__ dt, float *__restrict__ src,
+                                            float *__restrict__ mfxo) {
+  const int N = v.N;
+  const float *q0 = x.q0, *qb = x.qb;
+  if (x.sol) {
     if (mode == 0 || mode == 3)
 #pragma unroll
       for (int q = 0; q < 5; q++) Qout[q * N + k] = q0[q];
     return;
   }
+  const float dx = x.dx, dr = x.dr, ax0 = x.ax0, ax1 = x.ax1, ph = x.ph;
   Prim P;
   P.r = fmaxf(q0[0], 1e-4f);
   P.u = q0[1] / P.r; P.v = q0[2] / P.r;
   P.Y = clampf(q0[4] / P.r, 0.f, 1.f);
   P.g = 1.4f - 0.07f * P.Y;
   P.p = fmaxf(1.f, (P.g - 1.f) * (q0[3] - 0.5f * P.r * (P.u * P.u + P.v * P.v)));
-  float d = dts;
+  float d = x.dts;
   if (mode == 0 || mode == 2) {
     const float nu = fmaxf(nu0, nu1);
     const float cs = sqrtf(P.g * P.p / P.r);
@@ -246,36 +273,37 @@ __global__ void __launch_bounds__(kB) cell_kernel(AxiView v, const AxiConsts *__
     dt[k] = d;
     if (mode == 2) return;
   }
-  const float vol = rcj * dr;
+  const float vol = x.rcj * dr;
   float R[5];
 #pragma unroll
   for (int q = 0; q < 5; q++) R[q] = (-(fxb[q] - fxa[q]) / dx - (frb[q] * ax1 - fra[q] * ax0) / vol) / ph;
-  const float aw = i > 0 ? aem : 1.f, as = j > 0 ? anm : 0.f;
-  R[1] += P.p * (aek - aw) / (dx * ph);
-  R[2] += P.p * (ank * ax1 - as * ax0) / (vol * ph);
+  const float aw = i > 0 ? x.aem : 1.f, as = j > 0 ? x.anm : 0.f;
+  R[1] += P.p * (x.aek - aw) / (dx * ph);
+  R[2] += P.p * (x.ank * ax1 - as * ax0) / (vol * ph);
   const float mfx = 0.5f * (fxa[0] + fxb[0]);
   float sk = 0.f;
+  const int id = x.id;
   if (id) {
     const float rho = q0[0], g = P.g, cp = g * kR / (g - 1.f);
     const float T = P.p / (rho * kR), Tt = T + 0.5f * (P.u * P.u + P.v * P.v) / cp;
     const float ty = c.comps[id * 4 + 2], mf = mfx / ph;
     if (ty == 1.f || ty == 2.f) {     // blade row: tau^(wgt dx) on the upstream face enthalpy
-      const float e = c.comps[id * 4 + 1], fac = ty == 1.f ? 1.f : clampf(mf / (0.3f * mfd), 0.f, 1.f);
+      const float e = c.comps[id * 4 + 1], fac = ty == 1.f ? 1.f : clampf(mf / (0.3f * x.mfd), 0.f, 1.f);
       const float mw = fxa[0], HU = mw > 1e-9f ? fxa[3] / mw : cp * Tt;
-      const float w = (HU > 0.f ? HU : cp * Tt) * expm1f(c.comps[id * 4] * wgt * fac * dx) / dx;
+      const float w = (HU > 0.f ? HU : cp * Tt) * expm1f(c.comps[id * 4] * x.wgt * fac * dx) / dx;
       R[1] += ty == 1.f ? e * rho * w : rho * w / e;
       R[3] += mf * w;
       sk = mf * w;
-    } else if (rate > 0.f) {           // heat source relaxed to the target total temperature
+    } else if (x.rate > 0.f) {         // heat source relaxed to the target total temperature
       const float K = c.comps[id * 4 + 1] != 0.f ? c.comps[id * 4 + 1] : 1.f;
-      const float tgt = c.comps[id * 4] + (tgtk - c.comps[id * 4]) * K;
+      const float tgt = c.comps[id * 4] + (x.tgtk - c.comps[id * 4]) * K;
       const float m0 = fmaxf(mf, 0.f);
-      const float q = m0 * cp * fmaxf(0.f, tgt - Tt) * rate;
+      const float q = m0 * cp * fmaxf(0.f, tgt - Tt) * x.rate;
       R[3] += q;
       const float mfuel = q / c.hf;
       R[0] += mfuel;
       sk = mfuel;
-      R[4] += m0 * (1.f - P.Y) * yrate + mfuel;
+      R[4] += m0 * (1.f - P.Y) * x.yrate + mfuel;
     }
   }
   src[k] = sk; mfxo[k] = mfx;
@@ -291,6 +319,188 @@ __global__ void __launch_bounds__(kB) cell_kernel(AxiView v, const AxiConsts *__
   const float g = 1.4f - 0.07f * Y, ke = 0.5f * (n[1] * n[1] + n[2] * n[2]) / r;
   if ((g - 1.f) * (n[3] - ke) < 50.f) n[3] = 50.f / (g - 1.f) + ke;
   for (int q = 0; q < 5; q++) Qout[q * N + k] = n[q];
+}
+
+// One thread per cell (src/cfd_cpu.js:145-192) over the stored face fluxes.
+// The cell's loads (state, face fluxes, geometry, zone data, the stored dt)
+// are issued before the solid flag is tested.
+__global__ void __launch_bounds__(kB) cell_kernel(AxiView v, const AxiConsts *__restrict__ cp, const float *__restrict__ Qin, const float *Qb,
+                                                  float *Qout, const float *__restrict__ Fx, const float *__restrict__ Fr,
+                                                  const float *__restrict__ nutF, float *__restrict__ dt, float *__restrict__ src,
+                                                  float *__restrict__ mfxo, int mode, const float *__restrict__ cap) {
+  const AxiConsts &c = *cp;
+  const int W = v.W, N = v.N;
+  const int k = blockIdx.x * kB + threadIdx.x;
+  if (k >= N) return;
+  const int NXF = (W + 1) * v.H, NRF = W * (v.H + 1);
+  const int i = k % W, j = k / W;
+  const int fx0 = j * (W + 1) + i, fx1 = fx0 + 1, fr0 = j * W + i, fr1 = fr0 + W;
+  CellIn x;
+  load_cell(v, Qin, Qb, dt, mode, k, i, j, x);
+  float fxa[5], fxb[5], fra[5], frb[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) { fxa[q] = Fx[q * NXF + fx0]; fxb[q] = Fx[q * NXF + fx1]; fra[q] = Fr[q * NRF + fr0]; frb[q] = Fr[q * NRF + fr1]; }
+  cell_update(v, c, k, i, j, x, fxa, fxb, fra, frb, nutF[fr0], nutF[fr1], mode, cap, Qout, dt, src, mfxo);
+}
+
+// ---------------------------------------------------------------------------
+// Fused stage: one block of kFT threads per tile of kTW x kTH cells does the
+// faces and the cell updates of its tile. Phase 1: every cell of the tile and
+// its 2-cell ring (indices clamped into the grid, as the face kernel clamps)
+// is loaded once, all of a thread's loads issued before any is used, and its
+// primitives, volume fraction, solid flag, apertures and mixing length go to
+// LDS. Phase 2: the tile's x and r faces from LDS into LDS (a face on a tile's
+// edge is evaluated by both tiles, the same bits). Phase 3: every cell of the
+// tile updates from its four faces; its own loads were issued at the start.
+// Same arithmetic as face_kernel + cell_kernel (the same functions, no
+// multiply-add contraction), without the face fluxes' round trip through
+// memory and with one launch per stage instead of two. kMode 0: stage 1 with
+// dt, 1: stage 2.
+// ---------------------------------------------------------------------------
+// A tile row has kTW + 1 = 64 x faces and a face row kTW r faces: each is one
+// wavefront's work, so every wave evaluates faces of one kind at a time.
+// Tile height TH and waves per block NW are template parameters.
+constexpr int kTW = 63, kPW = kTW + 4;
+template <int TH>
+struct AxiTile {
+  static constexpr int PH = TH + 4;
+  float P[6][PH][kPW];       // r u v p Y g of cells (j0 - 2 .. j0 + TH + 1) x (i0 - 2 .. i0 + kTW + 1), indices clamped into the grid
+  float phi[PH][kPW], ae[PH][kPW], an[PH][kPW], lmix[PH][kPW];
+  uint8_t sol[PH][kPW];
+  float rc[PH];
+  float Fx[5][TH][kTW + 1];  // x faces i0 .. i0 + kTW of the tile's rows
+  float Fr[5][TH + 1][kTW];  // r faces j0 .. j0 + TH of the tile's columns
+  float nut[TH + 1][kTW];
+};
+template <int TH>
+__device__ __forceinline__ Prim lds_prim(const AxiTile<TH> &t, int r, int c) {
+  Prim P;
+  P.r = t.P[0][r][c]; P.u = t.P[1][r][c]; P.v = t.P[2][r][c]; P.p = t.P[3][r][c]; P.Y = t.P[4][r][c]; P.g = t.P[5][r][c];
+  return P;
+}
+// prim() on already-loaded conserved values
+__device__ __forceinline__ Prim prim_of(const float *q) {
+  Prim P;
+  P.r = fmaxf(q[0], 1e-4f);
+  P.u = q[1] / P.r; P.v = q[2] / P.r;
+  P.Y = clampf(q[4] / P.r, 0.f, 1.f);
+  P.g = 1.4f - 0.07f * P.Y;
+  P.p = fmaxf(1.f, (P.g - 1.f) * (q[3] - 0.5f * P.r * (P.u * P.u + P.v * P.v)));
+  return P;
+}
+// true in every lane when no lane of the wave has an open face (the wave skips the flux)
+__device__ __forceinline__ bool wave_all_closed(bool open) {
+#ifdef __AMDGCN__
+  return __builtin_amdgcn_ballot_w64(open) == 0;
+#else
+  return !open;
+#endif
+}
+
+// The block's work on tile t (LDS, passed in so a CPU build can share it between threads).
+template <int kMode, int TH, int NW>
+__device__ __forceinline__ void fused_tile(const AxiView &v, const AxiConsts *__restrict__ cp, const float *__restrict__ Qin,
+                                           const float *Qb, float *Qout, float *__restrict__ dt, float *__restrict__ src,
+                                           float *__restrict__ mfxo, int ntx, AxiTile<TH> &t) {
+  constexpr int FT = 64 * NW, PH = TH + 4, NP = kPW * PH, NU = 2 * TH + 1;
+  constexpr int PR = (NP + FT - 1) / FT, FR = (NU + NW - 1) / NW, CR = (TH + NW - 1) / NW;
+  const AxiConsts &c = *cp;
+  const int W = v.W, H = v.H, N = v.N;
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int i0 = (blockIdx.x % ntx) * kTW, j0 = (blockIdx.x / ntx) * TH;
+  // phase 1: every load of this thread's ring cells, then their LDS entries
+  float q[PR][5], ph[PR], ae[PR], an[PR], lm[PR];
+  uint8_t so[PR];
+#pragma unroll
+  for (int m = 0; m < PR; m++) {
+    const int e = tid + m * FT, ee = e < NP ? e : 0;
+    const int r = ee / kPW, cc = ee % kPW;
+    const int gi = min(max(i0 - 2 + cc, 0), W - 1), gj = min(max(j0 - 2 + r, 0), H - 1), kk = gj * W + gi;
+#pragma unroll
+    for (int z = 0; z < 5; z++) q[m][z] = Qin[z * N + kk];
+    ph[m] = v.phi[kk]; ae[m] = v.ae[kk]; an[m] = v.an[kk]; lm[m] = v.lmix[kk]; so[m] = v.solid[kk];
+  }
+  const float rcr = tid < PH ? v.rc[min(max(j0 - 2 + tid, 0), H - 1)] : 0.f;
+  // this thread's own cells (lane < kTW, rows wave, wave + NW, ...): their loads go out now, the latency overlaps the face work
+  CellIn x[CR];
+#pragma unroll
+  for (int m = 0; m < CR; m++) {
+    const int ci = i0 + lane, cj = j0 + wave + m * NW;
+    const bool own = lane < kTW && wave + m * NW < TH && ci < W && cj < H;
+    load_cell(v, Qin, Qb, dt, kMode, own ? cj * W + ci : 0, own ? ci : 0, own ? cj : 0, x[m]);
+  }
+#pragma unroll
+  for (int m = 0; m < PR; m++) {
+    const int e = tid + m * FT;
+    if (e >= NP) continue;
+    const int r = e / kPW, cc = e % kPW;
+    const Prim P = prim_of(q[m]);
+    t.P[0][r][cc] = P.r; t.P[1][r][cc] = P.u; t.P[2][r][cc] = P.v; t.P[3][r][cc] = P.p; t.P[4][r][cc] = P.Y; t.P[5][r][cc] = P.g;
+    t.phi[r][cc] = ph[m]; t.ae[r][cc] = ae[m]; t.an[r][cc] = an[m]; t.lmix[r][cc] = lm[m]; t.sol[r][cc] = so[m];
+  }
+  if (tid < PH) t.rc[tid] = rcr;
+  __syncthreads();
+  // phase 2: face units, one per wave at a time: x faces of tile row u (u < TH), r faces of face row u - TH.
+  // A wave whose faces are all closed stores zeros without evaluating a flux (the face functions give 0 there too).
+#pragma unroll
+  for (int m = 0; m < FR; m++) {
+    const int u0 = wave + m * NW;
+    float F[5];
+    if (u0 < TH) {
+      const int tr = u0, fl = lane, f = i0 + fl, j = j0 + tr;
+      const int r = tr + 2, ca = fl + 1;                 // LDS column of cell a = f - 1
+      const bool in = j < H && f <= W;
+      const float al = f - 1 < 0 ? (t.sol[r][ca + 1] ? 0.f : 1.f) : f >= W ? (t.sol[r][ca] ? 0.f : 1.f) : t.ae[r][ca];
+      if (wave_all_closed(in && al > 0.f)) {
+        if (in)
+#pragma unroll
+          for (int z = 0; z < 5; z++) t.Fx[z][tr][fl] = 0.f;
+      } else if (in) {
+        x_face(c, W, f, lds_prim(t, r, ca - 1), lds_prim(t, r, ca), lds_prim(t, r, ca + 1), lds_prim(t, r, ca + 2), t.phi[r][ca - 1],
+               t.phi[r][ca], t.phi[r][ca + 1], t.phi[r][ca + 2], t.sol[r][ca] != 0, t.sol[r][ca + 1] != 0, t.ae[r][ca], F);
+#pragma unroll
+        for (int z = 0; z < 5; z++) t.Fx[z][tr][fl] = F[z];
+      }
+    } else if (u0 < NU) {
+      const int tr = u0 - TH, cl = lane, f = j0 + tr, i = i0 + cl;
+      const int col = cl < kTW ? cl + 2 : 2, ra = tr + 1;   // LDS row of cell a = f - 1
+      const bool in = cl < kTW && i < W && f <= H;
+      const float al = f - 1 < 0 ? 0.f : f >= H ? (t.sol[ra][col] ? 0.f : 1.f) : t.an[ra][col];
+      if (wave_all_closed(in && al > 0.f)) {
+        if (in) {
+#pragma unroll
+          for (int z = 0; z < 5; z++) t.Fr[z][tr][cl] = 0.f;
+          t.nut[tr][cl] = 0.f;
+        }
+      } else if (in) {
+        float nut;
+        r_face(c, H, f, lds_prim(t, ra - 1, col), lds_prim(t, ra, col), lds_prim(t, ra + 1, col), lds_prim(t, ra + 2, col),
+               t.phi[ra - 1][col], t.phi[ra][col], t.phi[ra + 1][col], t.phi[ra + 2][col], t.sol[ra][col] != 0, t.an[ra][col],
+               t.lmix[ra][col], t.lmix[ra + 1][col], t.rc[ra], t.rc[ra + 1], F, &nut);
+#pragma unroll
+        for (int z = 0; z < 5; z++) t.Fr[z][tr][cl] = F[z];
+        t.nut[tr][cl] = nut;
+      }
+    }
+  }
+  __syncthreads();
+  // phase 3: the cell updates
+#pragma unroll
+  for (int m = 0; m < CR; m++) {
+    const int tx = lane, ty = wave + m * NW, ci = i0 + tx, cj = j0 + ty;
+    if (!(tx < kTW && ty < TH && ci < W && cj < H)) continue;
+    float fxa[5], fxb[5], fra[5], frb[5];
+#pragma unroll
+    for (int z = 0; z < 5; z++) { fxa[z] = t.Fx[z][ty][tx]; fxb[z] = t.Fx[z][ty][tx + 1]; fra[z] = t.Fr[z][ty][tx]; frb[z] = t.Fr[z][ty + 1][tx]; }
+    cell_update(v, c, cj * W + ci, ci, cj, x[m], fxa, fxb, fra, frb, t.nut[ty][tx], t.nut[ty + 1][tx], kMode, nullptr, Qout, dt, src, mfxo);
+  }
+}
+template <int kMode, int TH, int NW>
+__global__ void __launch_bounds__(64 * NW) fused_kernel(AxiView v, const AxiConsts *__restrict__ cp, const float *__restrict__ Qin,
+                                                        const float *Qb, float *Qout, float *__restrict__ dt, float *__restrict__ src,
+                                                        float *__restrict__ mfxo, int ntx) {
+  __shared__ AxiTile<TH> t;
+  fused_tile<kMode, TH, NW>(v, cp, Qin, Qb, Qout, dt, src, mfxo, ntx, t);
 }
 
 // Minimum dt over fluid cells (CPUSolver.step global / dtRatio cap, src/cfd_cpu.js:198-202).

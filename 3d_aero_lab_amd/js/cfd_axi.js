@@ -57,6 +57,10 @@ class AxiHIP {
     return Q;
   }
   finish() { loadAddon().axiSync(this.h); }
+  /* kernel form of the uncapped steps: 0 face + cell kernels, 1 .. 4 fused tile kernels (the same bits) */
+  setFused(form) { loadAddon().axiSetFused(this.h, form); }
+  /* keep the fastest kernel form for this grid (state kept): {form, msPerIter} */
+  autotune(iters = 50) { this._syncOpt(); return loadAddon().axiAutotune(this.h, iters); }
   dispose() { if (this.h) { loadAddon().axiDispose(this.h); this.h = null; } }
   /* Sections the device monitor sums (at most 4): [{col, rows: Uint8Array(H)}]
    * as cfd_post.js sectionRows builds them. */

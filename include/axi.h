@@ -41,6 +41,16 @@ const char *axi_last_error(const axi_solver *s);
 /* opt (src/cfd_cpu.js:53, 63, 198-202): cfl, order (1|2), visc (0|1), hf heating
  * value (<= 0 keeps 0.99 * 43e6), global (0|1) and dtRatio (0 = off) dt caps. */
 int axi_set_opt(axi_solver *s, double cfl, int order, int visc, double hf, int global, double dtRatio);
+/* Kernel form of the steps without a dt cap (no reference counterpart, a
+ * tuning knob): 0 a face kernel and a cell kernel per stage with the fluxes
+ * in memory; 1 .. 4 one fused kernel per stage, tiles of 63 x (3, 7, 9, 5)
+ * cells in blocks of (4, 8, 16, 4) waves with the primitives and face fluxes
+ * in LDS. The same arithmetic, the same bits. Default at create: 3 when its
+ * tiles make between half and all of one block per compute unit, else 0. */
+int axi_set_fused(axi_solver *s, int form);
+/* Times every kernel form over `iters` iterations, keeps the fastest and
+ * restores the state, diagnostics and iteration count; outputs may be null. */
+int axi_autotune(axi_solver *s, int iters, int *form, float *ms_best);
 /* setComps / setZones / setFs (src/cfd_gpu.js:252-265). */
 int axi_set_comps(axi_solver *s, const double comps[32]);
 int axi_set_zones(axi_solver *s, const axi_zones *z);

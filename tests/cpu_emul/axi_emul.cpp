@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #include "../../3d_aero_lab_amd/csrc/axi_kernels.hpp"
@@ -19,9 +21,29 @@ static void launch(int n, K k) {
     for (int t = 0; t < kB; t++) { blockIdx.x = b; threadIdx.x = t; k(); }
 }
 
+// One launch of the fused tile kernel: every block as 64 NW OS threads sharing
+// one AxiTile (NaN-filled, so a read of an unwritten slot shows up) and a barrier.
+template <int kMode, int TH, int NW>
+static void fused_launch(const AxiView &v, const AxiConsts &c, const float *Qin, const float *Qb, float *Qout, float *dt, float *src,
+                         float *mfx) {
+  const int ntx = (v.W + kTW - 1) / kTW, nty = (v.H + TH - 1) / TH;
+  for (int b = 0; b < ntx * nty; b++) {
+    std::barrier<> bar(64 * NW);
+    auto tile = std::make_unique<AxiTile<TH>>();
+    std::memset(tile.get(), 0xff, sizeof(AxiTile<TH>));
+    std::vector<std::thread> th;
+    for (int t = 0; t < 64 * NW; t++)
+      th.emplace_back([&, t, b] {
+        threadIdx.x = t; blockIdx.x = b; emul_block_barrier = &bar;
+        fused_tile<kMode, TH, NW>(v, &c, Qin, Qb, Qout, dt, src, mfx, ntx, *tile);
+      });
+    for (auto &x : th) x.join();
+  }
+}
+
 extern "C" int axi_emul_steps(int W, int H, const double *xf, const double *rf_, const uint8_t *solid, const uint8_t *comp,
                               const float *zf /* 9 planes */, const double *comps, const double *fs, double cfl, int order, int visc,
-                              double hf, float *Q /* 5N in/out */, float *src, float *mfx, int n) {
+                              double hf, float *Q /* 5N in/out */, float *src, float *mfx, int n, int fused) {
   const int N = W * H, NXF = (W + 1) * H, NRF = W * (H + 1);
   std::vector<float> rc(H), dx(W), dr(H), rf(H + 1);
   for (int i = 0; i < W; i++) dx[i] = (float)(xf[i + 1] - xf[i]);
@@ -38,6 +60,16 @@ extern "C" int axi_emul_steps(int W, int H, const double *xf, const double *rf_,
   float cap = 0;
   const int NF = NXF + NRF;
   for (int it = 0; it < n; it++) {
+    if (fused == 1) {          // tiles of 63 x 3, 4 waves (and rows from two rounds of waves)
+      fused_launch<0, 3, 2>(v, c, Q, Q, Q1.data(), dt.data(), src, mfx);
+      fused_launch<1, 3, 2>(v, c, Q1.data(), Q, Q, dt.data(), src, mfx);
+      continue;
+    }
+    if (fused == 2) {
+      fused_launch<0, 7, 8>(v, c, Q, Q, Q1.data(), dt.data(), src, mfx);
+      fused_launch<1, 7, 8>(v, c, Q1.data(), Q, Q, dt.data(), src, mfx);
+      continue;
+    }
     launch(NF, [&] { face_kernel(v, &c, Q, Fx.data(), Fr.data(), nutF.data()); });
     launch(N, [&] { cell_kernel(v, &c, Q, Q, Q1.data(), Fx.data(), Fr.data(), nutF.data(), dt.data(), src, mfx, 0, &cap); });
     launch(NF, [&] { face_kernel(v, &c, Q1.data(), Fx.data(), Fr.data(), nutF.data()); });

@@ -249,3 +249,40 @@ def test_axi_checkpoint_resume_is_exact(tmp_path):
     say("resume", whole["thrust"]["F"], resumed["thrust"]["F"])
     assert resumed["thrust"] == whole["thrust"]
     assert resumed["hist"][-1]["K4"] == whole["hist"][-1]["K4"]
+
+
+@pytest.mark.parametrize("case", ["axi_tj_xcoarse", "axi_hbtf_medium", "axi_lbtf_xcoarse"])
+def test_axi_fused_kernel_equals_face_and_cell_kernels(case):
+    """Every fused tile kernel form (primitives and face fluxes in LDS, one
+    launch per stage) and the face + cell kernels give the same bits: states,
+    dt, src and mfx after 60 iterations (graph replays and direct launches)."""
+    import aero3d
+    g, z, Q0, comps, xf, rf = load(case)
+    s = g.s
+    opt = dict(s["opt"], hf=s["hf"])
+    out = []
+    for fused in (0, 1, 2, 3, 4):
+        h = aero3d.AxiHIP(xf, rf, z, comps, s["fs"], Q0, opt)
+        h.set_fused(fused)
+        h.step(53); h.step(7)
+        out.append(h.read())
+        h.dispose()
+    for k in range(1, 5):
+        for a, b, name in zip(out[0], out[k], ("Q", "src", "mfx", "dt")):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (case, k, name)
+
+
+def test_axi_autotune_keeps_state():
+    import aero3d
+    g, z, Q0, comps, xf, rf = load("axi_tj_medium")
+    s = g.s
+    h = aero3d.AxiHIP(xf, rf, z, comps, s["fs"], Q0, dict(s["opt"], hf=s["hf"]))
+    h.step(30)
+    before = h.read()
+    form, ms = h.autotune(30)
+    say("axi autotune TJ medium: form", form, "ms per iteration", ms)
+    assert 0 <= form <= 4 and ms > 0 and h.iter == 30
+    after = h.read()
+    for a, b in zip(before, after):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    h.dispose()

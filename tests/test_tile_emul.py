@@ -55,12 +55,16 @@ def test_tile_kernel_cpu_build_small_grids(emul_lib, case, ty, zc):
 
 
 def test_axi_kernels_cpu_build_matches_oracle(tmp_path):
+    """The axisymmetric face + cell kernels and the fused tile kernel under
+    AddressSanitizer: the oracle reproduced after 2 iterations, and the fused
+    kernel (LDS tiles, one OS thread per GPU thread) bit-identical to the
+    face + cell kernels (fused_same = 0)."""
     out = str(tmp_path / "libaxi_emul.so")
     subprocess.run(["g++", "-std=c++20", "-O1", "-g", "-fPIC", "-shared", "-fsanitize=address", "-fno-omit-frame-pointer",
                     "-I", EMUL, "-o", out, os.path.join(EMUL, "axi_emul.cpp")], check=True)
     asan = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True, text=True).stdout.strip()
     env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0")
-    r = subprocess.run([sys.executable, os.path.join(EMUL, "run_axi_emul.py"), out, "axi_tj_xcoarse", "3"],
+    r = subprocess.run([sys.executable, os.path.join(EMUL, "run_axi_emul.py"), out, "axi_tj_xcoarse", "2", "fused"],
                        capture_output=True, text=True, env=env, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
     import json
