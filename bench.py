@@ -81,8 +81,9 @@ def batched_workload(seed=20261203, n=64):
 
 
 def run_converged_plain(built, conds, res_drop, max_iters, check_every, tile):
-    """Each case alone on the plain path, one after another, to res/res0 <= res_drop or max_iters.
-    Returns the wall time, the iterations and the final (CL, CD, Cm) of each case."""
+    """Each case alone on the plain path, one after another, read back every check_every iterations
+    (aero3d_readback) under the residual-drop rule (res/res0 <= res_drop, or max_iters), the server's
+    rule. Returns the wall time, the iterations and the final (CL, CD, Cm) of each case."""
     import aero3d
     t0 = time.perf_counter()
     iters, coefs = [], []
@@ -93,18 +94,9 @@ def run_converged_plain(built, conds, res_drop, max_iters, check_every, tile):
         s.set_tile(*tile)
         s.set_free(F); s.set_fix(fix); s.reset()
         s.set_force_list(fl)
-        s.residual()
-        res0 = None
-        while True:
-            s.step(check_every)
-            r = s.residual()
-            if res0 is None and r > 0:
-                res0 = r
-            if (res0 and r / res0 <= res_drop) or s.it >= max_iters or not math.isfinite(r):
-                break
-        iters.append(s.it)
-        c = s.forces(*fref)
-        coefs.append((c["CL"], c["CD"], c["Cm"]))
+        hist = s.solve(fref, aero3d.ConvRule(aero3d.RULE_RES_DROP, res_drop, max_iters), check_every)
+        iters.append(hist[-1]["it"])
+        coefs.append((hist[-1]["CL"], hist[-1]["CD"], hist[-1]["Cm"]))
         s.dispose()
     return time.perf_counter() - t0, iters, coefs
 
@@ -146,7 +138,7 @@ def run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes, m
     wall = srv.clock() - t_start
     busy, gpu, cu = srv.stats()
     srv.dispose()
-    return done, wall, busy, cu
+    return done, wall, gpu, cu
 
 
 def batched_leg(res_drop=1e-4, max_iters=2000, check_every=50):
@@ -179,8 +171,8 @@ def batched_leg(res_drop=1e-4, max_iters=2000, check_every=50):
     zc, group, pers = best[0], best[2], best[3]
     print(f"[bench] batched: {cells} cells, zchunk {zc}, group {group}, persistent {pers}; server run", file=sys.stderr, flush=True)
     reqs = [(k, M, alpha, 0.0) for k, (M, alpha) in enumerate(conds)]
-    done, wall_b, busy, cu = run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes=int(cells * 110 + (2 << 30)),
-                                        group_cells=group, persistent=pers)
+    done, wall_b, gpu_b, cu = run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes=int(cells * 110 + (2 << 30)),
+                                         group_cells=group, persistent=pers)
     print("[bench] batched: plain path, one case at a time", file=sys.stderr, flush=True)
     wall_p, iters_p, coefs_p = run_converged_plain(built, conds, res_drop, max_iters, check_every, (8, 0))
     it_b = {d["tag"]: d["iters"] for d in done}
@@ -190,6 +182,7 @@ def batched_leg(res_drop=1e-4, max_iters=2000, check_every=50):
             "cell_updates_per_s": cu / wall_b, "plain_solves_per_s": len(conds) / wall_p, "plain_wall_s": wall_p,
             "speedup_vs_plain": wall_p / wall_b, "converged": sum(d["converged"] for d in done),
             "iters_mean": sum(it_b.values()) / len(it_b), "same_iterations_as_plain": same, "cl_cd_max_rel_diff_vs_plain": dcoef,
+            "gpu_busy_frac": gpu_b / wall_b,
             "zchunk": zc, "group_cells": group, "launch": "persistent" if pers else "one block per work item",
             "criterion": f"res/res0 <= {res_drop} (density residual every {check_every} iterations) or {max_iters} iterations"}
 
@@ -215,11 +208,14 @@ SERVE_SPECS = ["sphere:30000", "sphere:120000", "sphere:500000", "transport:3000
 
 
 def serve_leg(n_requests=200, res_drop=1e-4, max_iters=2000, check_every=50):
+    """The served form on a sample of the seeded request stream (the full 10k-request stream is a long run of
+    its own: js/serve_stream.js), its capacity with every request arriving at once, and the capacity per grid
+    level against the plain path (the gain of batching by grid size)."""
     import geometry
     reqs = serve_workload()[:n_requests]
     print(f"[bench] serve: {len(reqs)} requests over {reqs[-1][3]:.1f} s of arrivals", file=sys.stderr, flush=True)
     built = geometry.build_cases(SERVE_SPECS)
-    done, wall, busy, cu = run_server(built, reqs, res_drop, max_iters, check_every, 3, arena_bytes=64 << 30)
+    done, wall, gpu, cu = run_server(built, reqs, res_drop, max_iters, check_every, 3, arena_bytes=64 << 30)
     lat = sorted(d["t_done"] - d["arrival"] for d in done)
     pct = lambda p: lat[min(len(lat) - 1, int(round(p / 100 * (len(lat) - 1))))]
     # the same requests one at a time on the plain path (arrival times ignored: its best case)
@@ -229,24 +225,35 @@ def serve_leg(n_requests=200, res_drop=1e-4, max_iters=2000, check_every=50):
     it_s = {d["tag"]: d["iters"] for d in done}
     # capacity: the same requests all arriving at once (the stream above is arrival-bound at ~20 req/s)
     print("[bench] serve: capacity run, all requests at t = 0", file=sys.stderr, flush=True)
-    cap = {}
-    for pers in (0, 1):
-        done_c, wall_c, _, cu_c = run_server(built, [(r[0], r[1], r[2], 0.0) for r in reqs], res_drop, max_iters, check_every, 3,
-                                             arena_bytes=64 << 30, persistent=pers)
-        cap[pers] = (done_c, wall_c, cu_c)
-    pers = min(cap, key=lambda p: cap[p][1])
-    done_c, wall_c, cu_c = cap[pers]
-    return {"requests": len(done), "sample": f"first {n_requests} of the 10k-request stream (seed 20261204)",
+    done_c, wall_c, gpu_c, cu_c = run_server(built, [(r[0], r[1], r[2], 0.0) for r in reqs], res_drop, max_iters, check_every, 3,
+                                             arena_bytes=64 << 30)
+    # capacity per grid level: the three bodies at one level, several requests each, all at once, against the plain path
+    print("[bench] serve: capacity per grid level", file=sys.stderr, flush=True)
+    levels = {}
+    rnd = geometry.mulberry32(20261206)
+    for cells, per_body in ((30000, 8), (120000, 4), (500000, 2), (1000000, 1)):
+        specs = [f"sphere:{cells}", f"transport:{cells}", f"interceptor:{cells}"]
+        lvl_built = [built[SERVE_SPECS.index(sp)] for sp in specs] if all(sp in SERVE_SPECS for sp in specs) else geometry.build_cases(specs)
+        lreq = [(b, 0.3 + 0.55 * rnd(), -2 + 10 * rnd(), 0.0) for b in range(3) for _ in range(per_body)]
+        dl, wl, gl, cul = run_server(lvl_built, lreq, res_drop, max_iters, check_every, 3, arena_bytes=64 << 30)
+        wp, ip, _ = run_converged_plain([lvl_built[r[0]] for r in lreq], [(r[1], r[2]) for r in lreq], res_drop, max_iters,
+                                        check_every, (8, 0))
+        it_l = {d["tag"]: d["iters"] for d in dl}
+        levels[str(cells)] = {"requests": len(lreq), "served_solves_per_s": len(lreq) / wl, "plain_solves_per_s": len(lreq) / wp,
+                              "speedup_vs_plain": wp / wl, "gpu_busy_frac": gl / wl, "served_cell_updates_per_s": cul / wl,
+                              "same_iterations_as_plain": all(it_l[k] == ip[k] for k in range(len(lreq)))}
+        print(f"[bench] serve: level {cells}: {levels[str(cells)]}", file=sys.stderr, flush=True)
+    return {"requests": len(done), "sample": f"first {n_requests} of the 10k-request stream (seed 20261204); "
+                                             "the whole stream: js/serve_stream.js",
             "capacity_solves_per_s": len(done_c) / wall_c, "capacity_cell_updates_per_s": cu_c / wall_c,
-            "capacity_speedup_vs_plain": wall_p / wall_c,
-            "capacity_launch": "persistent" if pers else "one block per work item",
-            "capacity_solves_per_s_by_launch": {"one block per work item": len(cap[0][0]) / cap[0][1], "persistent": len(cap[1][0]) / cap[1][1]},
+            "capacity_speedup_vs_plain": wall_p / wall_c, "capacity_gpu_busy_frac": gpu_c / wall_c,
             "solves_per_s": len(done) / wall, "wall_s": wall, "p50_s": pct(50), "p95_s": pct(95), "p99_s": pct(99),
-            "gpu_busy_frac": busy / wall, "cell_updates_per_s": cu / wall, "converged": sum(d["converged"] for d in done),
+            "gpu_busy_frac": gpu / wall, "cell_updates_per_s": cu / wall, "converged": sum(d["converged"] for d in done),
             "plain_solves_per_s": len(reqs) / wall_p, "plain_wall_s": wall_p, "speedup_vs_plain": wall_p / wall,
             "same_iterations_as_plain": all(it_s[k] == iters_p[k] for k in range(len(reqs))),
             "cl_cd_max_rel_diff_vs_plain": coef_rel_diff(done, coefs_p),
-            "criterion": f"res/res0 <= {res_drop} or {max_iters} iterations, residual every {check_every}"}
+            "capacity_by_grid_level": levels,
+            "criterion": f"res/res0 <= {res_drop} or {max_iters} iterations, readback every {check_every} (the first takes the snapshot)"}
 
 
 AXI_CASE = os.path.join(ROOT, "tests", "golden", "axi_tj_medium")
@@ -294,13 +301,38 @@ def axi_cpu_baseline(budget_s=10.0):
             "sample": f"{n} iterations of the TJ medium case (after 1 untimed), single thread"}
 
 
+def sphere_leg():
+    """C1: the 48 x 32 x 24 sphere case (radius 4.8 cells, M 0.4, CFL 0.5) read back every 50 iterations until
+    the reference controller's convergence rule (src/air_cfd.js:573-576) ends it: iterations and drag
+    coefficient C_d (S = pi R^2), beside the reference's own run of the same case (tests/golden/readback_sphere)
+    and the README's figure for radius 4.8 cells (README.md:99, grid not stated)."""
+    import geometry
+    import aero3d
+    grid, geo, fl, meta = geometry.build_case("sphere")
+    R = 4.8
+    M = 0.4
+    F = [1, M, 0, 0, 1 / (1.4 * 0.4) + 0.5 * M * M]
+    s = aero3d.CfdHIP(grid, geo)
+    s.opt["cfl"] = 0.5
+    s.set_free(F); s.set_fix([F, F, F]); s.reset(); s.set_force_list(fl)
+    t0 = time.perf_counter()
+    hist = s.solve((0.0, M, math.pi * R * R, 0.0, 0.0, 1.0), aero3d.ConvRule(aero3d.RULE_REFERENCE, cap=30000), 50, 2 * R * 1.12)
+    wall = time.perf_counter() - t0
+    s.dispose()
+    last = hist[-1]
+    ref = json.load(open(os.path.join(ROOT, "tests", "golden", "readback_sphere", "manifest.json")))["scalars"]["reads"][-1]
+    return {"cells": meta["N"], "converged_at": last["it"], "Cd": last["CD"], "CDi": last["CDi"], "Mmax": last["Mmax"], "wall_s": wall,
+            "reference_run_same_grid": {"converged_at": ref["it"], "Cd": ref["CD"]}, "readme_Cd_radius_4_8": 0.56,
+            "rule": "reference controller: > 6 readbacks, res/res0 < 2e-3, CL spread of the last 4 < 2e-3; readback every 50"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--legs", default="plain,batched,serve,axi", help="comma list of plain, batched, serve, axi")
+    ap.add_argument("--legs", default="plain,sphere,batched,serve,axi", help="comma list of plain, sphere, batched, serve, axi")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--tile", default="auto", help="stage-kernel shape 'ty,zchunk' or 'auto' (autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -342,6 +374,12 @@ def main():
     achieved = BYTES_PER_CELL_UPDATE / 2 * N / launch_s / 1e9
     r = meta["ref"]
     f = s.forces(alpha, M, r["S"], r["xcg"], 0, r["c"])
+    # C2: the same solve continued to 2000 iterations; its coefficients, Trefftz CDi and Mach maximum
+    it_timed = s.it
+    if s.it < 2000:
+        s.step(2000 - s.it)
+    rb = s.readback(alpha, M, r["S"], r["xcg"], 0.0, r["c"], meta["xT"] if meta.get("xT") is not None else float("nan"))
+    c2 = {"it": rb["it"], "CL": rb["CL"], "CD": rb["CD"], "Cm": rb["Cm"], "CDi": rb["CDi"], "Mmax": rb["Mmax"]}
     traffic = None
     try:     # HBM bytes per launch measured by rocprofv3 counters for this kernel shape (profiles/pmc_traffic.json)
         t = json.load(open(os.path.join(ROOT, "profiles", "pmc_traffic.json")))["plain_transport_1m"]
@@ -358,10 +396,13 @@ def main():
                    "tile": [64, ty, zc]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
-        "wall_s": wall, "it": s.it, "CL": f["CL"], "CD": f["CD"],
+        "wall_s": wall, "it_timed": it_timed, "CL_at_it_timed": f["CL"], "CD_at_it_timed": f["CD"],
+        "transport_2000_iterations": c2,
     }
     s.dispose()
     legs = a.legs.split(",")
+    if "sphere" in legs:
+        out["sphere"] = sphere_leg()
     if "batched" in legs:
         out["batched"] = batched_leg()
     if "serve" in legs:

@@ -128,3 +128,16 @@ def test_js_dropin_fighter_readme_row():
             assert abs(h["dev"][k] - h["host"][k]) / abs(h["host"][k]) < 1e-10
     assert round(res["hist"][-1]["dev"]["CL"], 4) == 0.1927 and round(res["hist"][-1]["dev"]["CD"], 5) == 0.04406
     assert abs(res["cdi"] - g.s["CDi"]) / g.s["CDi"] < 1e-4 and abs(res["cdi"] - res["cdiHost"]) / res["cdiHost"] < 1e-10
+
+
+@pytest.mark.gpu
+def test_js_server_and_batch_from_node():
+    """The served and batched forms driven from Node through the addon's async
+    calls (serverTickAsync, batchStepAsync, stepAsync): every served request
+    equals a plain CfdHIP solve read back by CfdMonitor under the same rule,
+    bit for bit; a handle with an async call in flight refuses other calls."""
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "serve_node_check.js")], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    print("[js-serve]", out, flush=True)
+    assert out["ok"], out
