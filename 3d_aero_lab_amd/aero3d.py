@@ -23,7 +23,9 @@ EXPORTS = ["aero3d_create", "aero3d_destroy", "aero3d_last_error", "aero3d_set_f
            "aero3d_set_opt", "aero3d_reset", "aero3d_step", "aero3d_iter", "aero3d_read", "aero3d_write",
            "aero3d_sync", "aero3d_set_force_list", "aero3d_forces", "aero3d_trefftz", "aero3d_residual",
            "aero3d_time_steps", "aero3d_set_tile", "aero3d_get_tile", "aero3d_autotune", "aero3d_footprint",
-           "aero3d_readback", "aero3d_readback_static"]
+           "aero3d_readback", "aero3d_readback_static", "aero3d_probe"]
+
+PROBE_STATE_KEYS = ["P", "T", "rho", "V", "M", "Tt", "Pt", "Cp", "Taw", "Vx", "Vy", "Vz"]
 
 
 class Grid(C.Structure):
@@ -37,6 +39,15 @@ class ReadbackIn(C.Structure):
 
 class ReadbackOut(C.Structure):
     _fields_ = [("it", C.c_int)] + [(k, C.c_double) for k in ("res", "CL", "CD", "Cm", "CDi", "Mmax")]
+
+
+class ProbeFs(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("T", "a", "P", "rho", "q")]
+
+
+class ProbeOut(C.Structure):
+    _fields_ = [("idx", C.c_void_p), ("w", C.c_void_p), ("ok", C.c_void_p), ("W", C.c_void_p), ("state", C.c_void_p),
+                ("vel", C.c_void_p), ("vok", C.c_void_p)]
 
 
 class Field(C.Structure):
@@ -131,6 +142,7 @@ def lib():
         L.aero3d_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
         L.aero3d_readback.argtypes = [vp, C.POINTER(ReadbackIn), C.POINTER(ReadbackOut)]
         L.aero3d_readback_static.argtypes = [C.POINTER(Field), C.POINTER(ReadbackIn), C.POINTER(ReadbackOut), vp]
+        L.aero3d_probe.argtypes = [vp, C.c_int, f64p, C.POINTER(ProbeFs), C.c_double, C.POINTER(ProbeOut)]
         _lib = L
     return _lib
 
@@ -243,6 +255,22 @@ class CfdHIP:
         o = ReadbackOut()
         self._chk(lib().aero3d_readback(self.h, C.byref(ReadbackIn(alpha, M, S, xcg, zcg, cref, xT)), C.byref(o)))
         return {k: getattr(o, k) for k, _ in ReadbackOut._fields_}
+
+    def probe(self, pts, fs=None, M=0.0):
+        """Display sampling at points pts (n x 3, full model) on the device:
+        the reference's cfdStencil / cfdSample / cfdState / cfdVelAt. fs: the
+        freestream record {T, a, P, rho, q} of the dimensional state (None:
+        no state). Returns idx (n x 8), w (n x 8), ok (n), W (n x 5), state
+        (n x 12, columns PROBE_STATE_KEYS, or None), vel (n x 3), vok (n)."""
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        n = pts.shape[0]
+        r = {"idx": np.zeros((n, 8), np.int32), "w": np.zeros((n, 8), np.float32), "ok": np.zeros(n, np.uint8),
+             "W": np.zeros((n, 5), np.float64), "state": np.zeros((n, 12), np.float64) if fs is not None else None,
+             "vel": np.zeros((n, 3), np.float64), "vok": np.zeros(n, np.uint8)}
+        o = ProbeOut(*[r[k].ctypes.data if r[k] is not None else None for k in ("idx", "w", "ok", "W", "state", "vel", "vok")])
+        f = C.byref(ProbeFs(*[fs[k] for k in ("T", "a", "P", "rho", "q")])) if fs is not None else None
+        self._chk(lib().aero3d_probe(self.h, n, pts.reshape(-1), f, M, C.byref(o)))
+        return r
 
     def solve(self, ref, rule, every=50, xT=float("nan")):
         """Step to convergence under `rule` (a ConvRule), one readback every
@@ -544,7 +572,9 @@ class AxiZones(C.Structure):
 
 AXI_EXPORTS = ["axi_create", "axi_destroy", "axi_last_error", "axi_set_opt", "axi_set_comps", "axi_set_zones", "axi_set_fs",
                "axi_set_state", "axi_step", "axi_iter", "axi_read", "axi_sync", "axi_time_steps", "axi_monitor",
-               "axi_monitor_sections", "axi_set_fused", "axi_autotune"]
+               "axi_monitor_sections", "axi_set_fused", "axi_autotune", "axi_sections", "axi_set_display_zones", "axi_derived",
+               "axi_transfer"]
+AXI_DERIVED_KEYS = ["T", "Tt", "P", "Pt", "V", "M", "rho", "Vx", "Vth", "Vr", "Y", "vort", "ds"]
 
 
 class AxiMonitorOut(C.Structure):
@@ -577,6 +607,10 @@ def alib():
         L.axi_monitor_sections.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p]
         L.axi_set_fused.argtypes = [vp, C.c_int]
         L.axi_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)]
+        L.axi_sections.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.axi_set_display_zones.argtypes = [vp, C.c_void_p, C.c_void_p]
+        L.axi_derived.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.axi_transfer.argtypes = [vp, vp, C.c_double, C.c_double]
         _alib = L
     return _alib
 
@@ -695,5 +729,33 @@ class AxiHIP:
         return {"F": o.F, "Fb": o.Fb, "Fw": o.Fw, "mfuel": o.mfuel, "pw": list(o.pw), "res_sum2": o.res_sum2,
                 "res_n": o.res_n, "res_valid": bool(o.res_valid), "bad": o.bad,
                 "sec": [{"mdot": o.sec_mdot[q], "w": o.sec_w[q], "wTt": o.sec_wTt[q]} for q in range(n)]}
+
+    def sections(self, cols, rows):
+        """Section sums of the current state (axi_sections): (nsec, 11) of mass flow,
+        w, area, sum p dA and the w-weighted Tt, Pt, T, p, |V|, M, Y."""
+        cols = np.ascontiguousarray(cols, np.int32)
+        rows = np.ascontiguousarray(rows, np.uint8).reshape(-1)
+        assert rows.size == cols.size * self.H
+        out = np.zeros(11 * cols.size)
+        self._chk(alib().axi_sections(self.h, int(cols.size), cols.ctypes.data, rows.ctypes.data, out.ctypes.data))
+        return out.reshape(-1, 11)
+
+    def set_display_zones(self, stream, mat):
+        st = np.ascontiguousarray(stream, np.int8)
+        mt = np.ascontiguousarray(mat, np.uint8)
+        assert st.shape == (self.N,) and mt.shape == (self.N,)
+        self._chk(alib().axi_set_display_zones(self.h, st.ctypes.data, mt.ctypes.data))
+
+    def derived(self, T0, P0, disp):
+        """Display fields (axi_derived): (13 x N float32 in AXI_DERIVED_KEYS order, reg, mat)."""
+        f = np.empty(13 * self.N, np.float32)
+        reg = np.empty(self.N, np.uint8)
+        mat = np.empty(self.N, np.uint8)
+        self._chk(alib().axi_derived(self.h, int(disp), T0, P0, f.ctypes.data, reg.ctypes.data, mat.ctypes.data))
+        return f.reshape(13, self.N), reg, mat
+
+    def transfer_from(self, src, T0, P0):
+        """Grid sequencing (axi_transfer): src's state carried onto this solver's grid."""
+        self._chk(alib().axi_transfer(self.h, src.h, T0, P0))
 
 EXPORTS = EXPORTS + AXI_EXPORTS

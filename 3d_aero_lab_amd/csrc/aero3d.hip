@@ -7,7 +7,7 @@
 //           current Q, stage-1 result S1, stage-2 result B, rotated like the
 //           reference's three ping-pong textures (src/air_cfd.js:421-425)
 //   masks   t u8[N], phi f32[N], ap f32[4][N]
-//   grid    dx, dy, dz f32 widths, xc f32 centres
+//   grid    dx, dy, dz f32 widths; cen f32 centres of x, y, z (NX + NY + NZ)
 //   dt      f32[N] local time step of the last stage
 //   rsnap   f32[N] density snapshot for the residual
 //   force   list q i32, cx, cz, x, z f64 and FP64 block partials
@@ -25,6 +25,7 @@
 #include "aero3d.h"
 #include "euler3d.hpp"
 #include "euler3d_tile.hpp"
+#include "probe.hpp"
 #include "reduce.hpp"
 
 using namespace aero3d;
@@ -34,6 +35,7 @@ namespace {
 std::string g_create_error;   // message of the last failed aero3d_create
 
 constexpr int kBlock = 256;
+constexpr int kProbeMaxPoints = 1 << 26;
 
 }  // namespace
 
@@ -43,6 +45,7 @@ struct aero3d_solver {
   int device = 0;
   hipStream_t stream = nullptr;
   float *dx = nullptr, *dy = nullptr, *dz = nullptr;
+  float *cen = nullptr;       // FP32 centres: x (NX), then y (NY), then z (NZ)
   std::vector<float> hxc, hdy, hdz;
   uint8_t *t = nullptr;
   float *phi = nullptr, *ap = nullptr;
@@ -63,6 +66,8 @@ struct aero3d_solver {
   double *fcx = nullptr, *fcz = nullptr, *fx = nullptr, *fz = nullptr;
   double *partials = nullptr, *sums = nullptr;
   RbSeg *dseg = nullptr;      // device copy of this solve's readback segment
+  char *probe = nullptr;      // aero3d_probe points and results, grown on demand
+  size_t probe_cap = 0;
   std::string err;
   int64_t bytes = 0;
 };
@@ -367,20 +372,25 @@ int aero3d_create(const aero3d_grid *g, const uint8_t *t, const float *phi, cons
     s->resident8 = std::max(1, ncu * std::max(1, o8));
   }
   // grid: FP32 widths and centres from FP64 faces, as the reference stores them
-  std::vector<float> hdx(s->NX), hdy(s->NY), hdz(s->NZ);
+  std::vector<float> hdx(s->NX), hdy(s->NY), hdz(s->NZ), hcen;
   s->hxc.resize(s->NX);
   for (int i = 0; i < s->NX; i++) { hdx[i] = (float)(g->fx[i + 1] - g->fx[i]); s->hxc[i] = (float)(0.5 * (g->fx[i] + g->fx[i + 1])); }
   for (int j = 0; j < s->NY; j++) hdy[j] = (float)(g->fy[j + 1] - g->fy[j]);
   for (int k = 0; k < s->NZ; k++) hdz[k] = (float)(g->fz[k + 1] - g->fz[k]);
   s->hdy = hdy; s->hdz = hdz;
+  hcen = s->hxc;
+  for (int j = 0; j < s->NY; j++) hcen.push_back((float)(0.5 * (g->fy[j] + g->fy[j + 1])));
+  for (int k = 0; k < s->NZ; k++) hcen.push_back((float)(0.5 * (g->fz[k] + g->fz[k + 1])));
   const size_t N = (size_t)s->N;
   if ((rc = dalloc(s, &s->dx, s->NX)) || (rc = dalloc(s, &s->dy, s->NY)) || (rc = dalloc(s, &s->dz, s->NZ)) ||
+      (rc = dalloc(s, &s->cen, hcen.size())) ||
       (rc = dalloc(s, &s->t, N)) || (rc = dalloc(s, &s->phi, N)) || (rc = dalloc(s, &s->ap, 4 * N)) ||
       (rc = dalloc(s, &s->state, 15 * s->ns)) || (rc = dalloc(s, &s->dt, N)) || (rc = dalloc(s, &s->rsnap, N)) ||
       (rc = dalloc(s, &s->partials, 5 * kRedMaxBlocks)) || (rc = dalloc(s, &s->sums, 16)) || (rc = dalloc(s, &s->dseg, 1)))
     return bail(rc);
   auto up = [&](void *d, const void *h, size_t b) { return hipMemcpy(d, h, b, hipMemcpyHostToDevice); };
   if (up(s->dx, hdx.data(), 4 * s->NX) || up(s->dy, hdy.data(), 4 * s->NY) || up(s->dz, hdz.data(), 4 * s->NZ) ||
+      up(s->cen, hcen.data(), 4 * hcen.size()) ||
       up(s->t, t, N) || up(s->phi, phi, 4 * N) || up(s->ap, ap0, 4 * N) || up(s->ap + N, ap1, 4 * N) ||
       up(s->ap + 2 * N, ap2, 4 * N) || up(s->ap + 3 * N, ap3, 4 * N) ||
       hipMemset(s->state, 0, sizeof(float) * 15 * s->ns) || hipMemset(s->dt, 0, 4 * N))
@@ -393,8 +403,8 @@ int aero3d_destroy(aero3d_solver *s) {
   if (!s) return 0;
   // teardown: an error here leaves nothing to recover, so return codes are not checked
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  void *ptrs[] = {s->dx, s->dy, s->dz, s->t, s->phi, s->ap, s->state, s->dt, s->rsnap, s->fq, s->fcx, s->fcz, s->fx, s->fz, s->partials, s->sums,
-                  s->dseg};
+  void *ptrs[] = {s->dx, s->dy, s->dz, s->cen, s->t, s->phi, s->ap, s->state, s->dt, s->rsnap, s->fq, s->fcx, s->fcz, s->fx, s->fz,
+                  s->partials, s->sums, s->dseg, s->probe};
   for (void *p : ptrs) if (p) (void)hipFree(p);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -539,6 +549,49 @@ int aero3d_readback(aero3d_solver *s, const aero3d_readback_in *in, aero3d_readb
   out->CL = s->fq ? o[6] : NAN; out->CD = s->fq ? o[7] : NAN; out->Cm = s->fq ? o[8] : NAN;
   out->CDi = tz ? 2 * o[kRbOut] / (Mn * Mn * in->S) : NAN;
   out->Mmax = std::sqrt(o[kRbOut + 1]);
+  return 0;
+}
+
+int aero3d_probe(aero3d_solver *s, int n, const double *xyz, const aero3d_probe_fs *fs, double M, const aero3d_probe_out *out) {
+  if (!s || n < 0 || (n > 0 && !xyz) || !out) return fail(s, AERO3D_E_ARG, "probe: null argument or n < 0");
+  if (n > kProbeMaxPoints) return fail(s, AERO3D_E_ARG, "probe: more than 2^26 points");
+  if (out->state && !fs) return fail(s, AERO3D_E_ARG, "probe: the display state needs the freestream record");
+  if (s->NX < 2 || s->NY < 2 || s->NZ < 2) return fail(s, AERO3D_E_ARG, "probe: needs at least 2 cells on each axis");
+  if (n == 0) return 0;
+  // one allocation: points, then each result array, 256-byte aligned
+  const size_t N = (size_t)n;
+  const size_t sz[8] = {24 * N, 32 * N, 32 * N, N, 40 * N, 96 * N, 24 * N, N};
+  size_t off[8], total = 0;
+  for (int c = 0; c < 8; c++) { off[c] = total; total += (sz[c] + 255) & ~(size_t)255; }
+  if (total > s->probe_cap) {
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    if (s->probe) { HIPCHK(s, hipFree(s->probe)); s->bytes -= (int64_t)s->probe_cap; s->probe = nullptr; s->probe_cap = 0; }
+    HIPCHK(s, hipMalloc((void **)&s->probe, total));
+    s->probe_cap = total;
+    s->bytes += (int64_t)total;
+  }
+  char *b = s->probe;
+  ProbeArgs a{};
+  a.NX = s->NX; a.NY = s->NY; a.NZ = s->NZ; a.n = n;
+  a.xc = s->cen; a.yc = s->cen + s->NX; a.zc = s->cen + s->NX + s->NY;
+  a.t = s->t; a.Q = buf(s, s->cur); a.ns = s->ns;
+  a.pts = (const double *)(b + off[0]);
+  if (fs) { a.fsT = fs->T; a.fsa = fs->a; a.fsP = fs->P; a.fsrho = fs->rho; a.fsq = fs->q; }
+  a.M = M;
+  a.idx = out->idx ? (int32_t *)(b + off[1]) : nullptr;
+  a.w = out->w ? (float *)(b + off[2]) : nullptr;
+  a.ok = out->ok ? (uint8_t *)(b + off[3]) : nullptr;
+  a.W = out->W ? (double *)(b + off[4]) : nullptr;
+  a.st = out->state ? (double *)(b + off[5]) : nullptr;
+  a.vel = out->vel ? (double *)(b + off[6]) : nullptr;
+  a.vok = out->vok ? (uint8_t *)(b + off[7]) : nullptr;
+  HIPCHK(s, hipMemcpyAsync(b + off[0], xyz, sz[0], hipMemcpyHostToDevice, s->stream));
+  probe_kernel<<<(n + 255) / 256, 256, 0, s->stream>>>(a);
+  HIPCHK(s, hipGetLastError());
+  void *dst[8] = {nullptr, out->idx, out->w, out->ok, out->W, out->state, out->vel, out->vok};
+  for (int c = 1; c < 8; c++)
+    if (dst[c]) HIPCHK(s, hipMemcpyAsync(dst[c], b + off[c], sz[c], hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
   return 0;
 }
 

@@ -6,6 +6,7 @@
 // ============================================================================
 #include <node_api.h>
 
+#include <algorithm>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -378,6 +379,46 @@ napi_value Readback(napi_env env, napi_callback_info info) {
   set_num(env, o, "it", out.it); set_num(env, o, "res", out.res); set_num(env, o, "CL", out.CL); set_num(env, o, "CD", out.CD);
   set_num(env, o, "Cm", out.Cm); set_num(env, o, "CDi", out.CDi); set_num(env, o, "Mmax", out.Mmax);
   return o;
+}
+
+// optional typed array: null / undefined give nullptr; otherwise type- and length-checked
+template <class T>
+bool opt_typed(napi_env env, napi_value v, napi_typedarray_type want, size_t len, T **data, const char *what) {
+  napi_valuetype ty;
+  if (napi_typeof(env, v, &ty) != napi_ok) return false;
+  if (ty == napi_null || ty == napi_undefined) { *data = nullptr; return true; }
+  size_t n;
+  return typed(env, v, want, data, &n) && check_len(env, n, len, what);
+}
+
+// probe(h, pts Float64Array(3n), fs Float64Array(5) [T, a, P, rho, q] | null, M,
+//       idx Int32Array(8n), w Float32Array(8n), ok Uint8Array(n), W Float64Array(5n),
+//       state Float64Array(12n), vel Float64Array(3n), vok Uint8Array(n))   (outputs may be null)
+napi_value Probe(napi_env env, napi_callback_info info) {
+  napi_value a[11];
+  if (!args(env, info, 11, a)) return nullptr;
+  aero3d_solver *s = handle(env, a[0]);
+  if (!s) return nullptr;
+  double *pts;
+  size_t np;
+  if (!typed(env, a[1], napi_float64_array, &pts, &np)) return nullptr;
+  if (np % 3) { napi_throw_range_error(env, nullptr, "aero3d addon: probe points need 3 coordinates each"); return nullptr; }
+  const size_t n = np / 3;
+  double *fsv;
+  if (!opt_typed(env, a[2], napi_float64_array, 5, &fsv, "probe freestream record")) return nullptr;
+  aero3d_probe_out o{};
+  if (!opt_typed(env, a[4], napi_int32_array, 8 * n, &o.idx, "probe idx") ||
+      !opt_typed(env, a[5], napi_float32_array, 8 * n, &o.w, "probe w") ||
+      !opt_typed(env, a[6], napi_uint8_array, n, &o.ok, "probe ok") ||
+      !opt_typed(env, a[7], napi_float64_array, 5 * n, &o.W, "probe W") ||
+      !opt_typed(env, a[8], napi_float64_array, 12 * n, &o.state, "probe state") ||
+      !opt_typed(env, a[9], napi_float64_array, 3 * n, &o.vel, "probe vel") ||
+      !opt_typed(env, a[10], napi_uint8_array, n, &o.vok, "probe vok"))
+    return nullptr;
+  aero3d_probe_fs fs{};
+  if (fsv) fs = {fsv[0], fsv[1], fsv[2], fsv[3], fsv[4]};
+  int rc = aero3d_probe(s, (int)std::min(n, (size_t)1 << 30), pts, fsv ? &fs : nullptr, num(env, a[3]), &o);
+  return rc ? throw_rc(env, s, rc) : undef(env);
 }
 
 // ---------------------------------------------------------------------------
@@ -1032,6 +1073,55 @@ napi_value AxiMonitorSections(napi_env env, napi_callback_info info) {
   int rc = axi_monitor_sections(r->s, (int)n, cols, rows);
   return rc ? axi_throw(env, r->s, rc) : undef(env);
 }
+// axiSections(h, cols Int32Array(n), rows Uint8Array(n H), sums Float64Array(11 n))
+napi_value AxiSections(napi_env env, napi_callback_info info) {
+  napi_value a[4];
+  if (!args(env, info, 4, a)) return nullptr;
+  RecAxi *r = recaxi(env, a[0]);
+  int32_t *cols; uint8_t *rows; double *o; size_t n, m, no;
+  if (!r || !typed(env, a[1], napi_int32_array, &cols, &n) || !typed(env, a[2], napi_uint8_array, &rows, &m) ||
+      !check_len(env, m, n * (size_t)r->H, "section rows") || !typed(env, a[3], napi_float64_array, &o, &no) ||
+      !check_len(env, no, 11 * n, "section sums"))
+    return nullptr;
+  int rc = axi_sections(r->s, (int)n, cols, rows, o);
+  return rc ? axi_throw(env, r->s, rc) : undef(env);
+}
+// axiSetDisplayZones(h, stream Int8Array(N), mat Uint8Array(N))
+napi_value AxiSetDisplayZones(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!args(env, info, 3, a)) return nullptr;
+  RecAxi *r = recaxi(env, a[0]);
+  int8_t *st; uint8_t *mat; size_t n, m;
+  if (!r || !typed(env, a[1], napi_int8_array, &st, &n) || !check_len(env, n, r->N, "stream") ||
+      !typed(env, a[2], napi_uint8_array, &mat, &m) || !check_len(env, m, r->N, "mat"))
+    return nullptr;
+  int rc = axi_set_display_zones(r->s, st, mat);
+  return rc ? axi_throw(env, r->s, rc) : undef(env);
+}
+// axiDerived(h, disp, T0, P0, fields Float32Array(13 N) | null, reg Uint8Array(N) | null, mat Uint8Array(N) | null)
+napi_value AxiDerived(napi_env env, napi_callback_info info) {
+  napi_value a[7];
+  if (!args(env, info, 7, a)) return nullptr;
+  RecAxi *r = recaxi(env, a[0]);
+  if (!r) return nullptr;
+  float *f; uint8_t *reg, *mat;
+  if (!opt_typed(env, a[4], napi_float32_array, 13 * r->N, &f, "display fields") ||
+      !opt_typed(env, a[5], napi_uint8_array, r->N, &reg, "reg") || !opt_typed(env, a[6], napi_uint8_array, r->N, &mat, "mat"))
+    return nullptr;
+  int rc = axi_derived(r->s, (int)num(env, a[1]), num(env, a[2]), num(env, a[3]), f, reg, mat);
+  return rc ? axi_throw(env, r->s, rc) : undef(env);
+}
+// axiTransfer(hdst, hsrc, T0, P0)
+napi_value AxiTransfer(napi_env env, napi_callback_info info) {
+  napi_value a[4];
+  if (!args(env, info, 4, a)) return nullptr;
+  RecAxi *d = recaxi(env, a[0]);
+  if (!d) return nullptr;
+  RecAxi *s = recaxi(env, a[1]);
+  if (!s) return nullptr;
+  int rc = axi_transfer(d->s, s->s, num(env, a[2]), num(env, a[3]));
+  return rc ? axi_throw(env, d->s, rc) : undef(env);
+}
 // axiMonitor(h, comps Float64Array(32), P0, out Float64Array(29)):
 // F Fb Fw mfuel pw[8] res_sum2 res_n res_valid nsec (mdot w wTt) x 4 bad
 napi_value AxiMonitor(napi_env env, napi_callback_info info) {
@@ -1089,7 +1179,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"create", Create}, {"dispose", Dispose}, {"setFree", SetFree}, {"setFix", SetFix}, {"setOpt", SetOpt},
       {"reset", Reset}, {"step", Step}, {"iter", Iter}, {"read", Read}, {"write", Write}, {"sync", Sync},
       {"setForceList", SetForceList}, {"forces", Forces}, {"trefftz", Trefftz}, {"residual", Residual},
-      {"timeSteps", TimeSteps}, {"autotune", Autotune}, {"readback", Readback}, {"stepAsync", StepAsync},
+      {"timeSteps", TimeSteps}, {"autotune", Autotune}, {"readback", Readback}, {"probe", Probe}, {"stepAsync", StepAsync},
       {"batchCreate", BatchCreate}, {"batchDispose", BatchDispose}, {"batchAddGeometry", BatchAddGeometry},
       {"batchRemoveGeometry", BatchRemoveGeometry}, {"batchAdd", BatchAdd}, {"batchRemove", BatchRemove}, {"batchStep", BatchStep},
       {"batchStepAsync", BatchStepAsync}, {"batchSync", BatchSync}, {"batchIter", BatchIter}, {"batchRead", BatchRead},
@@ -1100,7 +1190,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"serverTickAsync", ServerTickAsync}, {"serverPoll", ServerPoll}, {"serverClock", ServerClock}, {"serverStats", ServerStats},
       {"axiCreate", AxiCreate}, {"axiDispose", AxiDispose}, {"axiSetOpt", AxiSetOpt}, {"axiSetComps", AxiSetComps},
       {"axiSetZones", AxiSetZones}, {"axiSetFs", AxiSetFs}, {"axiSetState", AxiSetState}, {"axiStep", AxiStep},
-      {"axiRead", AxiRead}, {"axiSync", AxiSync}, {"axiSetFused", AxiSetFused}, {"axiAutotune", AxiAutotune}, {"axiMonitorSections", AxiMonitorSections}, {"axiMonitor", AxiMonitor}};
+      {"axiRead", AxiRead}, {"axiSync", AxiSync}, {"axiSetFused", AxiSetFused}, {"axiAutotune", AxiAutotune}, {"axiMonitorSections", AxiMonitorSections}, {"axiMonitor", AxiMonitor},
+      {"axiSections", AxiSections}, {"axiSetDisplayZones", AxiSetDisplayZones}, {"axiDerived", AxiDerived}, {"axiTransfer", AxiTransfer}};
   for (auto &f : fns) {
     napi_value fn;
     NAPI_OK(napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));

@@ -33,6 +33,7 @@
 #include "axi.h"
 
 #include "axi_kernels.hpp"
+#include "axi_post.hpp"
 
 using namespace axik;
 
@@ -70,6 +71,12 @@ struct axi_solver {
   uint8_t *sec_rows = nullptr;
   double *mon_part = nullptr, *mon_out = nullptr;
   MonIn *mon_in = nullptr;
+  // post-processing: FP64 x centres, the display zone codes, a scratch buffer grown on demand
+  double *xcD = nullptr;
+  int8_t *dstream = nullptr;
+  uint8_t *dmat = nullptr;
+  char *post = nullptr;
+  size_t post_cap = 0;
   std::string err;
 };
 
@@ -216,18 +223,18 @@ int axi_create(const axi_grid *g, const axi_zones *z, const double comps[32], co
       al(&s->dt, N) || al(&s->src, N) || al(&s->mfx, N) || al(&s->partial, 1024) || al(&s->cap, 1) || al(&s->dc, 1) ||
       al(&s->rcD, s->H) || al(&s->drD, s->H) || al(&s->dxD, s->W) || al(&s->rsnap, N) || al(&s->sec_col, kMonMaxSec) ||
       al(&s->sec_rows, (size_t)kMonMaxSec * s->H) || al(&s->mon_part, (size_t)kMonBlocks * kMonK) ||
-      al(&s->mon_out, kMonK + 3 * kMonMaxSec) || al(&s->mon_in, 1))
+      al(&s->mon_out, kMonK + kSecK * kMonMaxSec) || al(&s->mon_in, 1) || al(&s->xcD, s->W))
     return bail(afail(s, -1002, "axi_create: hipMalloc failed"));
   if (hipMemcpy(s->rc, rc.data(), 4 * rc.size(), hipMemcpyHostToDevice) || hipMemcpy(s->dx, dx.data(), 4 * dx.size(), hipMemcpyHostToDevice) ||
       hipMemcpy(s->dr, dr.data(), 4 * dr.size(), hipMemcpyHostToDevice) || hipMemcpy(s->rf, rf.data(), 4 * rf.size(), hipMemcpyHostToDevice) ||
       hipMemset(s->src, 0, 4 * N) || hipMemset(s->mfx, 0, 4 * N) || hipMemset(s->dt, 0, 4 * N))
     return bail(afail(s, -1001, "axi_create: upload failed"));
   {  // the host grid's double centres and widths (buildCFDGrid, src/cfd_grid.js:36-41)
-    std::vector<double> rcD(s->H), drD(s->H), dxD(s->W);
-    for (int i = 0; i < s->W; i++) dxD[i] = g->xf[i + 1] - g->xf[i];
+    std::vector<double> rcD(s->H), drD(s->H), dxD(s->W), xcD(s->W);
+    for (int i = 0; i < s->W; i++) { dxD[i] = g->xf[i + 1] - g->xf[i]; xcD[i] = 0.5 * (g->xf[i] + g->xf[i + 1]); }
     for (int j = 0; j < s->H; j++) { rcD[j] = 0.5 * (g->rf[j] + g->rf[j + 1]); drD[j] = g->rf[j + 1] - g->rf[j]; }
     if (hipMemcpy(s->rcD, rcD.data(), 8 * rcD.size(), hipMemcpyHostToDevice) || hipMemcpy(s->drD, drD.data(), 8 * drD.size(), hipMemcpyHostToDevice) ||
-        hipMemcpy(s->dxD, dxD.data(), 8 * dxD.size(), hipMemcpyHostToDevice))
+        hipMemcpy(s->dxD, dxD.data(), 8 * dxD.size(), hipMemcpyHostToDevice) || hipMemcpy(s->xcD, xcD.data(), 8 * xcD.size(), hipMemcpyHostToDevice))
       return bail(afail(s, -1001, "axi_create: upload failed"));
   }
   {  // default kernel form: tiles of 63 x 9 cells in blocks of 16 waves when they make between half and all
@@ -251,7 +258,8 @@ int axi_destroy(axi_solver *s) {
   if (s->exec) (void)hipGraphExecDestroy(s->exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   void *p[] = {s->rc, s->dx, s->dr, s->rf, s->solid, s->comp, s->zf, s->Q, s->Q1, s->Fx, s->Fr, s->nutF, s->dt, s->src, s->mfx, s->partial, s->cap, s->dc,
-               s->rcD, s->drD, s->dxD, s->rsnap, s->sec_col, s->sec_rows, s->mon_part, s->mon_out, s->mon_in};
+               s->rcD, s->drD, s->dxD, s->rsnap, s->sec_col, s->sec_rows, s->mon_part, s->mon_out, s->mon_in, s->xcD, s->dstream, s->dmat,
+               s->post};
   for (void *x : p) if (x) (void)hipFree(x);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -343,10 +351,11 @@ int axi_monitor(axi_solver *s, const double comps[32], double P0, axi_monitor_ou
   const int nb = std::max(1, std::min(kMonBlocks, (s->N + kB - 1) / kB));
   monitor_kernel<<<nb, kB, 0, s->stream>>>(v, s->rcD, s->drD, s->dxD, s->Q, s->src, s->mfx, s->rsnap, s->mon_in, s->mon_part);
   monitor_finish<<<1, 64, 0, s->stream>>>(s->mon_part, nb, s->mon_out);
-  if (s->nsec > 0) section_kernel<<<1, 64, 0, s->stream>>>(v, s->rcD, s->drD, s->Q, s->mfx, s->nsec, s->sec_col, s->sec_rows, s->mon_out);
+  if (s->nsec > 0)
+    section_kernel<<<1, kSecBlock, 0, s->stream>>>(v, s->rcD, s->drD, s->Q, s->mfx, s->nsec, s->sec_col, s->sec_rows, s->mon_out + kMonK);
   ACHK(s, hipGetLastError());
-  double o[kMonK + 3 * kMonMaxSec];
-  ACHK(s, hipMemcpyAsync(o, s->mon_out, sizeof(double) * (kMonK + 3 * s->nsec), hipMemcpyDeviceToHost, s->stream));
+  double o[kMonK + kSecK * kMonMaxSec];
+  ACHK(s, hipMemcpyAsync(o, s->mon_out, sizeof(double) * (kMonK + kSecK * s->nsec), hipMemcpyDeviceToHost, s->stream));
   ACHK(s, hipMemcpyAsync(s->rsnap, s->Q, 4 * (size_t)s->N, hipMemcpyDeviceToDevice, s->stream));   // density of this readback
   ACHK(s, hipStreamSynchronize(s->stream));
   s->snap_valid = true;
@@ -357,8 +366,91 @@ int axi_monitor(axi_solver *s, const double comps[32], double P0, axi_monitor_ou
   out->bad = (int)o[13];
   out->nsec = s->nsec;
   for (int q = 0; q < s->nsec; q++) {
-    out->sec_mdot[q] = o[kMonK + 3 * q]; out->sec_w[q] = o[kMonK + 3 * q + 1]; out->sec_wTt[q] = o[kMonK + 3 * q + 2];
+    const double *sq = o + kMonK + kSecK * q;
+    out->sec_mdot[q] = sq[0]; out->sec_w[q] = sq[1]; out->sec_wTt[q] = sq[4];
   }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// post-processing on the device (section averages, display fields, transfer)
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kMaxSections = 256;
+
+// the post scratch buffer, at least `bytes` (grown after the stream drained)
+int post_buf(axi_solver *s, size_t bytes) {
+  if (bytes <= s->post_cap) return 0;
+  ACHK(s, hipStreamSynchronize(s->stream));
+  if (s->post) { ACHK(s, hipFree(s->post)); s->post = nullptr; s->post_cap = 0; }
+  ACHK(s, hipMalloc((void **)&s->post, bytes));
+  s->post_cap = bytes;
+  return 0;
+}
+size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+int axi_sections(axi_solver *s, int nsec, const int *col, const uint8_t *rows, double *sums) {
+  if (!s || nsec < 0 || nsec > kMaxSections || (nsec > 0 && (!col || !rows || !sums)))
+    return afail(s, -1, "sections: bad argument (0 <= nsec <= 256)");
+  for (int q = 0; q < nsec; q++)
+    if (col[q] < 0 || col[q] >= s->W) return afail(s, -1, "sections: column out of range");
+  if (nsec == 0) return 0;
+  const size_t bc = up256(sizeof(int) * nsec), br = up256((size_t)nsec * s->H), bo = sizeof(double) * kSecK * nsec;
+  if (int rc = post_buf(s, bc + br + bo)) return rc;
+  int *dcol = (int *)s->post;
+  uint8_t *drows = (uint8_t *)(s->post + bc);
+  double *dout = (double *)(s->post + bc + br);
+  ACHK(s, hipMemcpyAsync(dcol, col, sizeof(int) * nsec, hipMemcpyHostToDevice, s->stream));
+  ACHK(s, hipMemcpyAsync(drows, rows, (size_t)nsec * s->H, hipMemcpyHostToDevice, s->stream));
+  section_kernel<<<(nsec + kSecBlock - 1) / kSecBlock, kSecBlock, 0, s->stream>>>(view(s), s->rcD, s->drD, s->Q, s->mfx, nsec, dcol, drows, dout);
+  ACHK(s, hipGetLastError());
+  ACHK(s, hipMemcpyAsync(sums, dout, bo, hipMemcpyDeviceToHost, s->stream));
+  ACHK(s, hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int axi_set_display_zones(axi_solver *s, const int8_t *stream, const uint8_t *mat) {
+  if (!s || !stream || !mat) return afail(s, -1, "set_display_zones: null argument");
+  const size_t N = s->N;
+  if (!s->dstream) {
+    ACHK(s, hipMalloc((void **)&s->dstream, N));
+    ACHK(s, hipMalloc((void **)&s->dmat, N));
+  }
+  ACHK(s, hipMemcpyAsync(s->dstream, stream, N, hipMemcpyHostToDevice, s->stream));
+  ACHK(s, hipMemcpyAsync(s->dmat, mat, N, hipMemcpyHostToDevice, s->stream));
+  ACHK(s, hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int axi_derived(axi_solver *s, int disp, double T0, double P0, float *fields, uint8_t *reg, uint8_t *mat) {
+  if (!s || (disp != 0 && disp != 1)) return afail(s, -1, "derived: bad argument (disp 0 or 1)");
+  if (!s->dstream) return afail(s, -3, "derived: axi_set_display_zones first");
+  const size_t N = s->N, bf = up256(sizeof(float) * kDerivedFields * N), bb = up256(N);
+  if (int rc = post_buf(s, bf + 2 * bb)) return rc;
+  float *df = (float *)s->post;
+  uint8_t *dreg = (uint8_t *)(s->post + bf), *dmat = (uint8_t *)(s->post + bf + bb);
+  derived_kernel<<<(s->N + kB - 1) / kB, kB, 0, s->stream>>>(view(s), s->xcD, s->rcD, s->Q, s->dstream, s->dmat, disp, T0, P0, df, dreg,
+                                                              dmat);
+  ACHK(s, hipGetLastError());
+  if (fields) ACHK(s, hipMemcpyAsync(fields, df, sizeof(float) * kDerivedFields * N, hipMemcpyDeviceToHost, s->stream));
+  if (reg) ACHK(s, hipMemcpyAsync(reg, dreg, N, hipMemcpyDeviceToHost, s->stream));
+  if (mat) ACHK(s, hipMemcpyAsync(mat, dmat, N, hipMemcpyDeviceToHost, s->stream));
+  ACHK(s, hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int axi_transfer(axi_solver *dst, const axi_solver *src, double T0, double P0) {
+  if (!dst || !src || dst == src) return afail(dst, -1, "transfer: two different solvers");
+  hipPointerAttribute_t a{}, b{};
+  if (hipPointerGetAttributes(&a, src->Q) != hipSuccess || hipPointerGetAttributes(&b, dst->Q) != hipSuccess || a.device != b.device)
+    return afail(dst, -1, "transfer: the solvers are on different devices");
+  ACHK(dst, hipStreamSynchronize(src->stream));   // the source's steps have run
+  InterpSrc A{src->W, src->H, src->xcD, src->rcD, src->solid, src->Q};
+  interp_kernel<<<(dst->N + kB - 1) / kB, kB, 0, dst->stream>>>(A, dst->W, dst->H, dst->xcD, dst->rcD, T0, P0, dst->Q);
+  ACHK(dst, hipGetLastError());
+  ACHK(dst, hipStreamSynchronize(dst->stream));
+  dst->snap_valid = false;        // as after setState: the next monitor has no previous readback
   return 0;
 }
 

@@ -536,8 +536,8 @@ __global__ void dtcap_kernel(const float *__restrict__ partial, int nb, const Ax
 //                   (src/cfd_post.js:44-75), the density residual of the
 //                   readback (src/app_cfd.js:368-371) and the count of
 //                   non-physical cells (src/app_cfd.js:377): block partial sums.
-//   section_kernel  mass flow and mass-weighted Tt sums of cfdSection
-//                   (src/cfd_post.js:83-102) at the control stations, one
+//   section_kernel  the sums of cfdSection (src/cfd_post.js:83-102) at
+//                   sections (the control stations, station averages), one
 //                   thread per section, rows in the host's order.
 // Per-cell terms are the host's double operations in the host's order
 // (contraction off, NaN-propagating max / min / clamp as in JavaScript); only
@@ -639,26 +639,36 @@ __global__ void monitor_finish(const double *__restrict__ partials, int nb, doub
   out[c] = a;
 }
 
-// out[kMonK + 3 s + {0, 1, 2}] = mass flow, sum of max(mf, 0), sum of max(mf, 0) Tt at column col[s] over rows[s H + j] != 0
+// cfdSection's sums (src/cfd_post.js:83-102) for section s at column col[s]
+// over rows[s H + j] != 0, in the host's row order, with the solver's mean
+// axial face mass flux as the cell mass flow; written to out[kSecK s + c]:
+//   0 mass flow, 1 sum of max(mf, 0) (the weight w), 2 area A, 3 sum p dA,
+//   4..10 w-weighted sums of Tt, Pt, T, p, |V|, M, Y
+constexpr int kSecK = 11;
+constexpr int kSecBlock = 64;     // threads per block of section_kernel
 __global__ void section_kernel(AxiView v, const double *__restrict__ rcD, const double *__restrict__ drD, const float *__restrict__ Q,
                                const float *__restrict__ mfx, int nsec, const int *__restrict__ col, const uint8_t *__restrict__ rows,
                                double *__restrict__ out) {
 #pragma clang fp contract(off)
-  const int s = threadIdx.x;
+  const int s = blockIdx.x * kSecBlock + threadIdx.x;
   if (s >= nsec) return;
   const int W = v.W, H = v.H, N = v.N, i = col[s];
-  double m = 0.0, w = 0.0, wTt = 0.0;
+  double acc[kSecK];
+  for (int c = 0; c < kSecK; c++) acc[c] = 0.0;
   for (int j = 0; j < H; j++) {
-    if (!rows[s * H + j]) continue;
+    if (!rows[(size_t)s * H + j]) continue;
     const int k = j * W + i;
     const PrimD q = prim_d(Q, N, k);
+    const double al = jmin(1.0, 0.5 * ((i > 0 ? (double)v.ae[k - 1] : 1.0) + (double)v.ae[k]));
+    const double dA = 2.0 * kPi * rcD[j] * drD[j] * al;
     const double T = q.p / (q.r * kRGas), cp = q.g * kRGas / (q.g - 1.0), V2 = q.u * q.u + q.v * q.v, Tt = T + V2 / (2.0 * cp);
+    const double Pt = q.p * pow(Tt / T, q.g / (q.g - 1.0));
     const double mf = (double)mfx[k] * 2.0 * kPi * rcD[j] * drD[j], wk = jmax(mf, 0.0);
-    m += mf; w += wk; wTt += wk * Tt;
+    acc[0] += mf; acc[1] += wk; acc[2] += dA; acc[3] += q.p * dA;
+    acc[4] += wk * Tt; acc[5] += wk * Pt; acc[6] += wk * T; acc[7] += wk * q.p; acc[8] += wk * sqrt(V2);
+    acc[9] += wk * sqrt(V2 / (q.g * q.p / q.r)); acc[10] += wk * q.Y;
   }
-  out[kMonK + 3 * s] = m;
-  out[kMonK + 3 * s + 1] = w;
-  out[kMonK + 3 * s + 2] = wTt;
+  for (int c = 0; c < kSecK; c++) out[(size_t)kSecK * s + c] = acc[c];
 }
 
 }  // namespace axik

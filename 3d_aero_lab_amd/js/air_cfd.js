@@ -8,9 +8,9 @@
  *   CfdHIP                   solver object with the CfdCPU / CfdGPU protocol
  *                            (setFree, setFix, reset, step, read, dispose; opt)
  *                            backed by the HIP library      (src/air_cfd.js:157-259, 367-438)
- *   cfdForces                surface pressure force          (src/air_cfd.js:443-452)
- *   cfdTrefftz, cfdResidual  induced drag and density residual of the
- *                            controller's readback          (src/air_cfd.js:552-570)
+ * The post-processing the reference runs on read() output (cfdForces, the
+ * readback's residual / Trefftz sums, display sampling) runs on the device:
+ * CfdHIP.forces / trefftz / residual / readback / probe of hip_solver.js.
  *
  * Grid and mask construction stays in JS doubles: the masks are compared
  * byte-for-byte with the reference, so every expression that feeds a
@@ -365,151 +365,9 @@ function forceListArrays(list) {
   return { n, q, cx, cz, x, z };
 }
 
-/* Pressure force on the body from a host state (src/air_cfd.js:443-452):
- * -(p - p_inf) times the aperture jump, moments about (xcg, zcg), rotated to
- * wind axes and doubled for the half model. */
-function cfdForces(g, list, H, alphaDeg, M, S, xcg, zcg, cref) {
-  let Fx = 0, Fz = 0, My = 0;
-  const pinf = 1 / CG;
-  for (const f of list) {
-    const q = f.q, r = H.r[q], u = H.ru[q] / r, v = H.rv[q] / r, w = H.rw[q] / r;
-    const dp = (CG - 1) * (H.E[q] - 0.5 * r * (u * u + v * v + w * w)) - pinf;
-    const fx = -dp * f.cx, fz = -dp * f.cz;
-    Fx += fx; Fz += fz; My += (f.z - zcg) * fx - (f.x - xcg) * fz;
-  }
-  const a = alphaDeg * DEG, k = 2 / (0.5 * M * M * S);
-  return { CL: k * (Fz * Math.cos(a) - Fx * Math.sin(a)), CD: k * (Fx * Math.cos(a) + Fz * Math.sin(a)), Cm: k * My / cref };
-}
-
-/* Bracketing index and weight of v in ascending centres c (src/air_cfd.js:453). */
-function cfdLocate(c, v) {
-  const n = c.length;
-  if (v <= c[0]) return [0, 0];
-  if (v >= c[n - 1]) return [n - 2, 1];
-  let lo = 0, hi = n - 1;
-  while (hi - lo > 1) { const m = (lo + hi) >> 1; if (c[m] <= v) lo = m; else hi = m; }
-  return [lo, (v - c[lo]) / (c[hi] - c[lo])];
-}
-
-/* Trefftz-plane induced drag CDi = 2 sum (v^2 + vn^2) dy dz / (M^2 S) over
- * fluid cells of the x-interpolated plane at xT (src/air_cfd.js:560-570). */
-function cfdTrefftz(g, t, H, xT, alphaDeg, M, S) {
-  const [i0, tx] = cfdLocate(g.X.c, xT), a = alphaDeg * DEG, ca = Math.cos(a), sa = Math.sin(a), Mn = Math.max(M, 0.05);
-  let sum = 0;
-  for (let k = 0; k < g.NZ; k++) for (let j = 0; j < g.NY; j++) {
-    let u = 0, v = 0, w = 0, ws = 0;
-    for (let di = 0; di < 2; di++) {
-      const q = cidx(g, i0 + di, j, k);
-      if (t[q] !== 0) continue;
-      const ww = di ? tx : 1 - tx, r = H.r[q];
-      u += ww * H.ru[q] / r; v += ww * H.rv[q] / r; w += ww * H.rw[q] / r; ws += ww;
-    }
-    if (ws < 1e-6) continue;
-    u /= ws; v /= ws; w /= ws;
-    const vn = w * ca - u * sa;
-    sum += (v * v + vn * vn) * g.Y.d[j] * g.Z.d[k];
-  }
-  return 2 * sum / (Mn * Mn * S);
-}
 /* Trefftz plane of an aircraft: 0.12 L behind the later of the fuselage end
  * and the tail trailing edge (src/air_cfd.js:561). */
 function trefftzX(ac) { return Math.max(ac.fus.L, ac.htail.xLE(ac.htail.semi) + ac.htail.ct) + 0.12 * ac.fus.L; }
-
-/* Density residual between two reads, per iteration: RMS of the density change
- * over every third cell that is fluid (src/air_cfd.js:553-554). */
-function cfdResidual(t, r, prevR, nIt) {
-  let sum = 0, n = 0;
-  for (let q = 0; q < r.length; q += 3) {
-    if (t[q] !== 0) continue;
-    const d = r[q] - prevR[q];
-    sum += d * d; n++;
-  }
-  return Math.sqrt(sum / Math.max(1, n)) / nIt;
-}
-
-/* ---- display sampling (src/air_cfd.js:454-490, 637-646) ---- */
-
-/* Trilinear weights at (x, |y|, z) over the 8 surrounding cell centres, fluid
- * and fixed-state cells only, renormalised; when they hold (almost) no weight,
- * the nearest fluid cell of the 4 x 4 x 4 neighbourhood takes weight 1.
- * idx / w: the caller's arrays (Int32 / Float32 in the reference's callers),
- * entries o .. o+7. Returns false when no fluid cell is near. */
-function cfdStencil(g, t, x, y, z, idx, w, o) {
-  const ay = Math.abs(y);
-  const [i, tx] = cfdLocate(g.X.c, x), [j, ty] = cfdLocate(g.Y.c, ay), [k, tz] = cfdLocate(g.Z.c, z);
-  let ws = 0;
-  for (let c = 0; c < 8; c++) {
-    const di = c & 1, dj = (c >> 1) & 1, dk = c >> 2, q = cidx(g, i + di, j + dj, k + dk);
-    const wc = (di ? tx : 1 - tx) * (dj ? ty : 1 - ty) * (dk ? tz : 1 - tz);
-    const use = t[q] === 0 || t[q] >= 3;
-    idx[o + c] = q; w[o + c] = use ? wc : 0; ws += use ? wc : 0;
-  }
-  if (ws > 1e-6) { for (let c = 0; c < 8; c++) w[o + c] /= ws; return true; }
-  let best = -1, bd = Infinity;
-  for (let dk = -1; dk <= 2; dk++) for (let dj = -1; dj <= 2; dj++) for (let di = -1; di <= 2; di++) {
-    const a = i + di, b = j + dj, c = k + dk;
-    if (a < 0 || b < 0 || c < 0 || a >= g.NX || b >= g.NY || c >= g.NZ) continue;
-    const q = cidx(g, a, b, c);
-    if (t[q] !== 0) continue;
-    const d2 = (g.X.c[a] - x) ** 2 + (g.Y.c[b] - ay) ** 2 + (g.Z.c[c] - z) ** 2;
-    if (d2 < bd) { bd = d2; best = q; }
-  }
-  for (let c = 0; c < 8; c++) { idx[o + c] = best < 0 ? 0 : best; w[o + c] = best < 0 ? 0 : (c === 0 ? 1 : 0); }
-  return best >= 0;
-}
-
-/* Primitives [rho, u, v, w, p] of the stencil-weighted conserved state of H
- * (read() arrays); v flips sign for points below the symmetry plane (ysgn -1).
- * Returns null when the stencil has no weight. */
-function cfdSample(H, idx, w, o, ysgn, out) {
-  let r = 0, ru = 0, rv = 0, rw = 0, E = 0, ws = 0;
-  for (let c = 0; c < 8; c++) {
-    const wc = w[o + c];
-    if (!wc) continue;
-    const q = idx[o + c];
-    r += wc * H.r[q]; ru += wc * H.ru[q]; rv += wc * H.rv[q]; rw += wc * H.rw[q]; E += wc * H.E[q]; ws += wc;
-  }
-  if (!ws) return null;
-  r = Math.max(r, 1e-6);
-  const u = ru / r, v = rv / r * ysgn, wv = rw / r;
-  out[0] = r; out[1] = u; out[2] = v; out[3] = wv;
-  out[4] = Math.max((CG - 1) * (E - 0.5 * r * (u * u + v * v + wv * wv)), 1e-7);
-  return out;
-}
-
-/* Dimensional state of nondimensional primitives W given the freestream
- * record fs = {T, a, P, rho, q} (src/air_cfd.js:484-490). */
-function cfdState(fs, W, o) {
-  o = o || {};
-  const T = CG * W[4] / W[0] * fs.T, a = Math.sqrt(1.4 * 287.05 * T), V = Math.hypot(W[1], W[2], W[3]) * fs.a;
-  o.P = W[4] * CG * fs.P; o.T = T; o.rho = W[0] * fs.rho; o.V = V; o.M = V / a;
-  o.Tt = T * (1 + 0.2 * o.M * o.M); o.Pt = o.P * Math.pow(1 + 0.2 * o.M * o.M, 3.5);
-  o.Cp = fs.q > 1 ? (o.P - fs.P) / fs.q : 0; o.Taw = T * (1 + 0.89 * 0.2 * o.M * o.M);
-  o.Vx = W[1] * fs.a; o.Vy = W[2] * fs.a; o.Vz = W[3] * fs.a;
-  return o;
-}
-
-/* Velocity at (x, y, z) in units of the freestream speed (M), null outside
- * the cell-centre box or without fluid weight (src/air_cfd.js:637-646); the
- * reference keeps g, t, H and M on its CFD3 record, here they are arguments. */
-function cfdVelAt(g, t, H, M, x, y, z, out, scratch) {
-  if (x < g.X.c[0] || x > g.X.c[g.NX - 1] || Math.abs(y) > g.Y.c[g.NY - 1] || z < g.Z.c[0] || z > g.Z.c[g.NZ - 1]) return null;
-  const S = scratch || (cfdVelAt._s || (cfdVelAt._s = { idx: new Int32Array(8), w: new Float32Array(8), W: [0, 0, 0, 0, 0] }));
-  const { idx, w, W } = S;
-  const [i, tx] = cfdLocate(g.X.c, x), [j, ty] = cfdLocate(g.Y.c, Math.abs(y)), [k, tz] = cfdLocate(g.Z.c, z);
-  let ws = 0;
-  for (let c = 0; c < 8; c++) {
-    const di = c & 1, dj = (c >> 1) & 1, dk = c >> 2, q = cidx(g, i + di, j + dj, k + dk), tq = t[q];
-    const wc = (tq === 0 || tq >= 3) ? (di ? tx : 1 - tx) * (dj ? ty : 1 - ty) * (dk ? tz : 1 - tz) : 0;
-    idx[c] = q; w[c] = wc; ws += wc;
-  }
-  if (ws < 1e-6) return null;
-  for (let c = 0; c < 8; c++) w[c] /= ws;
-  if (!cfdSample(H, idx, w, 0, y < 0 ? -1 : 1, W)) return null;
-  const Minf = Math.max(M || 0, 1e-3);
-  out[0] = W[1] / Minf; out[1] = W[2] / Minf; out[2] = W[3] / Minf;
-  return out;
-}
 
 /* Freestream conserved state at Mach M and incidence alpha (src/air_cfd.js:523-524). */
 function freestream(M, alphaDeg) {
@@ -519,6 +377,5 @@ function freestream(M, alphaDeg) {
 
 module.exports = {
   CG, CFD_GRIDS, SUB, cfdAxis, axisFromFaces, gridFromFaces, cfdGrid, cidx, classifyCells, finishMasks,
-  cfdGeomBody, aircraftSolid, cfdGeom, cfdForceList, forceListArrays, cfdForces, cfdLocate, cfdTrefftz, trefftzX, cfdResidual, freestream,
-  cfdStencil, cfdSample, cfdState, cfdVelAt,
+  cfdGeomBody, aircraftSolid, cfdGeom, cfdForceList, forceListArrays, trefftzX, freestream,
 };

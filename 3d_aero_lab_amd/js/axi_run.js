@@ -2,7 +2,9 @@
 /* ============================================================================
  * axi_run.js — the reference's long axisymmetric run (tools/longrun.js:2-14)
  * on AxiHIP: n iterations in chunks of 250 with the exit-temperature control
- * after every chunk, thrust and station averages at the end.
+ * after every chunk, thrust and station averages at the end. Every readback
+ * is the device's (AxiHIP.monitor / sections, FP64 sums): only the numbers
+ * come back, the state once at the end (for a checkpoint).
  *
  *   node axi_run.js <case dir> <iters> [report every] [spec] [load=<file>] [save=<file>]
  * <case dir> holds the solver inputs (grid faces xf / rf, zones z_*, comps,
@@ -43,8 +45,10 @@ function run(dir, iters, every, fromSpec, ckpt) {
     if (L.comps) comps = L.comps;
   }
   const S = new AxiHIP(C.grid, C.zones, comps, s.fs, Q0, Object.assign({}, s.opt, { hf: s.hf }));
+  const ctlSt = P.controlStations(s.ctl);
+  S.monitorSections(ctlSt.map(st => P.sectionRows(C.grid, C.zones, st.x, st.si)));
   const hist = [];
-  let it = 0, Q = null;
+  let it = 0, f = null;
   const t0 = process.hrtime.bigint();
   let tStep = 0n;
   while (it < iters) {
@@ -53,18 +57,15 @@ function run(dir, iters, every, fromSpec, ckpt) {
     S.step(n); S.finish();
     tStep += process.hrtime.bigint() - a;
     it += n;
-    Q = S.read(Q);
-    const f0 = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, 0, { S: S.src, F: S.mfx });
-    P.cfdControl(s.type || 'tj', s.ctl, C.grid, C.zones, Q, S.mfx, comps, f0.pw, s.ctl.etam, 250);
+    // the readback on the device: thrust, blade-row powers and the control sections (FP64)
+    f = S.monitor(comps, s.ctl.P0);
+    P.controlFromSections(s.type || 'tj', s.ctl, st => P.sectionFromSums(f.sec[st.id === 4 ? 0 : 1]), comps, f.pw, s.ctl.etam, 250);
     S.setComps(comps);
-    if (it % every === 0 || it >= iters) {
-      const f = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, s.hf, { S: S.src, F: S.mfx });
-      hist.push({ it, F: f.F, Fb: f.Fb, Fw: f.Fw, mfuel: f.mfuel, K4: comps[17] });
-    }
+    if (it % every === 0 || it >= iters) hist.push({ it, F: f.F, Fb: f.Fb, Fw: f.Fw, mfuel: f.mfuel, K4: comps[17] });
   }
   const wall = Number(process.hrtime.bigint() - t0) / 1e9;
-  const f = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, s.hf, { S: S.src, F: S.mfx });
-  const st = P.stations(C.grid, C.zones, Q, S.mfx, s.ctl.order, s.type || 'tj');
+  const st = P.stations(S, C.grid, C.zones, s.ctl.order, s.type || 'tj');
+  const Q = S.read();
   S.dispose();
   if (ckpt && ckpt.save) saveCheckpoint(ckpt.save, Q, comps);
   return { cells: C.grid.N, iters, hist, thrust: { F: f.F, Fb: f.Fb, Fw: f.Fw, mfuel: f.mfuel }, F1D: s.ctl.F1D, stations: st,

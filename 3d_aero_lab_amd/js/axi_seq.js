@@ -7,19 +7,18 @@
  * start of the current grid), convergence when the residual is below 2e-5 or
  * the thrust has stayed within 0.3 % for 1800 iterations with the residual
  * below 6e-5. With grid sequencing, the run starts on the coarse grid and
- * moves to the target grid (interpState, control gains carried) once the
- * coarse grid has converged after 2500 iterations, or after 9000.
+ * moves to the target grid (the state carried over on the device,
+ * AxiHIP.transferFrom = interpState; control gains carried) once the coarse
+ * grid has converged after 2500 iterations, or after 9000.
  *
- *   node axi_seq.js <coarse case dir> <target case dir> [every] [cap] [host]   (sequenced)
- *   node axi_seq.js - <target case dir> [every] [cap] [host]                   (target grid only)
+ *   node axi_seq.js <coarse case dir> <target case dir> [every] [cap]   (sequenced)
+ *   node axi_seq.js - <target case dir> [every] [cap]                   (target grid only)
  * Case directories as written by tests/golden/make_golden_axi.js (setup arrays
  * and initial state). Prints one JSON line: per-grid iterations, residual and
  * thrust histories, final thrust, wall time.
- * By default each readback is the device monitor (AxiHIP.monitor: thrust,
- * blade-row powers, residual sums, the control sections and the divergence
- * count in FP64, 29 numbers back); `host` reads the whole state back and
- * computes the same quantities with cfdForces / cfdSection on the host, as
- * the application does.
+ * Each readback is the device monitor (AxiHIP.monitor: thrust, blade-row
+ * powers, residual sums, the control sections and the divergence count in
+ * FP64, 29 numbers back) where the application reads the whole state back.
  * Divergence (src/app_cfd.js:377-378, 405-409): a readback that finds a cell
  * k = 0, 7, 14, ... with density or energy not positive abandons the run; it
  * starts again from the initial state and the setup's constants (the whole
@@ -27,68 +26,36 @@
  * first-order reconstruction, at most `maxRestarts` times.
  * ========================================================================== */
 const P = require('./cfd_post.js');
-const { interpState } = require('./cfd_grid.js');
 const { AxiHIP } = require('./cfd_axi.js');
 const { loadCase } = require('./axi_run.js');
-
-/* density residual of the reference's readback: RMS over every second
- * non-solid cell of the relative density change, per iteration (src/app_cfd.js:368-371) */
-function residual(zones, Q0, prev, dIt) {
-  let s2 = 0, n = 0;
-  for (let k = 0; k < Q0.length; k += 2) {
-    if (zones.solid[k]) continue;
-    const d = (Q0[k] - prev[k]) / prev[k];
-    if (d === d) { s2 += d * d; n++; }
-  }
-  return Math.sqrt(s2 / Math.max(1, n)) / dIt;
-}
-
-/* cells k = 0, 7, 14, ... with energy or density not positive (the divergence
- * check of src/app_cfd.js:377) */
-function badCells(Q) {
-  let bad = 0;
-  for (let k = 0; k < Q[0].length; k += 7) if (!(Q[3][k] > 0) || !(Q[0][k] > 0)) bad++;
-  return bad;
-}
 
 function makeAxiHIP(C, comps, Q0, opt) {
   return new AxiHIP(C.grid, C.zones, comps, C.s.fs, Q0, Object.assign({}, C.s.opt, { hf: C.s.hf }, opt));
 }
 
-/* Runs one grid until the reference's criteria; returns the final state, or
- * null when a readback found non-physical cells. */
-function runGrid(C, comps, Q0, seq, every, cap, log, host, opt, make) {
+/* Runs one grid until the reference's criteria (from the state of solver
+ * `from` when given, carried onto this grid). Returns the solver (the caller
+ * disposes it), or null when a readback found non-physical cells. */
+function runGrid(C, comps, from, seq, every, cap, log, opt, make) {
   const s = C.s;
-  const S = make(C, comps, Q0, opt);
+  const S = make(C, comps, C.Q0, opt);
+  if (from) S.transferFrom(from, s.fl);
   const minIt = seq ? 2500 : 3000;
-  let it = 0, prev = null, res = NaN, Q = null, f = null;
+  let it = 0, res = NaN, f = null;
   const lastF = [];
-  if (!host) S.monitorSections(P.controlStations(s.ctl).map(st => P.sectionRows(C.grid, C.zones, st.x, st.si)));
+  S.monitorSections(P.controlStations(s.ctl).map(st => P.sectionRows(C.grid, C.zones, st.x, st.si)));
   const t0 = process.hrtime.bigint();
   for (;;) {
     S.step(every); it += every;
     const gate = log.itTotal + it > 1500 && it > 300;
-    if (host) {
-      Q = S.read(Q);
-      if (prev) res = residual(C.zones, Q[0], prev, every);
-      prev = Float32Array.from(Q[0]);
-      if (badCells(Q)) { S.dispose(); log.itTotal += it; return null; }
-      f = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, s.hf, { S: S.src, F: S.mfx });
-      if (gate) {
-        const f0 = P.cfdForces(C.grid, C.zones, Q, comps, s.ctl.P0, 0, { S: S.src, F: S.mfx });
-        P.cfdControl(s.type || 'tj', s.ctl, C.grid, C.zones, Q, S.mfx, comps, f0.pw, s.ctl.etam, every);
-        S.setComps(comps);
-      }
-    } else {
-      // with the solver's sources the forces do not depend on the heating value
-      const m = S.monitor(comps, s.ctl.P0);
-      if (m.resValid) res = Math.sqrt(m.resSum2 / Math.max(1, m.resN)) / every;
-      if (m.bad) { S.dispose(); log.itTotal += it; return null; }
-      f = m;
-      if (gate) {
-        P.controlFromSections(s.type || 'tj', s.ctl, st => P.sectionFromSums(m.sec[st.id === 4 ? 0 : 1]), comps, m.pw, s.ctl.etam, every);
-        S.setComps(comps);
-      }
+    // with the solver's sources the forces do not depend on the heating value
+    const m = S.monitor(comps, s.ctl.P0);
+    if (m.resValid) res = Math.sqrt(m.resSum2 / Math.max(1, m.resN)) / every;
+    if (m.bad) { S.dispose(); log.itTotal += it; return null; }
+    f = m;
+    if (gate) {
+      P.controlFromSections(s.type || 'tj', s.ctl, st => P.sectionFromSums(m.sec[st.id === 4 ? 0 : 1]), comps, m.pw, s.ctl.etam, every);
+      S.setComps(comps);
     }
     log.hist.push({ grid: C.grid.W + 'x' + C.grid.H, it: log.itTotal + it, res, F: f.F });
     lastF.push([it, f.F]);
@@ -102,30 +69,31 @@ function runGrid(C, comps, Q0, seq, every, cap, log, host, opt, make) {
     if (!seq && it >= cap) break;
   }
   const wall = Number(process.hrtime.bigint() - t0) / 1e9;
-  if (!host) Q = S.read(Q);                       // the final state, once
-  S.dispose();
   log.itTotal += it;
   log.grids.push({ W: C.grid.W, H: C.grid.H, iters: it, res, F: f.F, wall_s: wall });
-  return Q;
+  return S;
 }
 
 /* One attempt of the whole run (coarse grid first when sequenced); null on divergence. */
-function attempt(coarseDir, T, every, cap, host, opt, make, load, log) {
+function attempt(coarseDir, T, every, cap, opt, make, load, log) {
   const comps = Float64Array.from(T.comps);
-  let Q0 = T.Q0;
+  let from = null;
   if (coarseDir && coarseDir !== '-') {
     const Cc = load(coarseDir), cc = Float64Array.from(Cc.comps);
-    const Qc = runGrid(Cc, cc, Cc.Q0, true, every, cap, log, host, opt, make);
-    if (!Qc) return null;
-    const Q64 = [0, 1, 2, 3, 4].map(q => Float64Array.from(Qc[q]));
-    Q0 = interpState(Cc.grid, Cc.zones, Q64, T.grid, T.zones, T.s.fl);
+    from = runGrid(Cc, cc, null, true, every, cap, log, opt, make);
+    if (!from) return null;
     comps[17] = cc[17]; comps[29] = cc[29];                 // control gains carried (src/app_cfd.js:281)
   }
-  return runGrid(T, comps, Q0, false, every, cap, log, host, opt, make);
+  const S = runGrid(T, comps, from, false, every, cap, log, opt, make);
+  if (from) from.dispose();
+  if (!S) return null;
+  const Q = S.read();                                       // the final state, once
+  S.dispose();
+  return Q;
 }
 
 /* o: {makeSolver(C, comps, Q0, opt), loadCase(dir), maxRestarts} (defaults: AxiHIP, the fixture loader, 3). */
-function run(coarseDir, targetDir, every, cap, host, o = {}) {
+function run(coarseDir, targetDir, every, cap, o = {}) {
   const make = o.makeSolver || makeAxiHIP, load = o.loadCase || loadCase, maxRestarts = o.maxRestarts === undefined ? 3 : o.maxRestarts;
   const T = load(targetDir);
   const opt = { cfl: T.s.opt.cfl, order: T.s.opt.order };
@@ -134,21 +102,20 @@ function run(coarseDir, targetDir, every, cap, host, o = {}) {
   let log;
   for (;;) {
     log = { itTotal: 0, hist: [], grids: [] };
-    if (attempt(coarseDir, T, every, cap, host, opt, make, load, log)) break;
+    if (attempt(coarseDir, T, every, cap, opt, make, load, log)) break;
     restarts.push({ at: log.itTotal, cfl: opt.cfl, order: opt.order });
     if (restarts.length > maxRestarts) {
-      return { mode: host ? 'host' : 'device', diverged: true, restarts, wall_s: Number(process.hrtime.bigint() - t0) / 1e9 };
+      return { diverged: true, restarts, wall_s: Number(process.hrtime.bigint() - t0) / 1e9 };
     }
     opt.cfl = Math.max(0.3, opt.cfl - 0.2); opt.order = 1;   // src/app_cfd.js:407
   }
   const wall = Number(process.hrtime.bigint() - t0) / 1e9;
   const last = log.grids[log.grids.length - 1];
-  return { mode: host ? 'host' : 'device', grids: log.grids, iters: log.itTotal, F: last.F, res: last.res, wall_s: wall, hist: log.hist,
-    restarts, opt };
+  return { grids: log.grids, iters: log.itTotal, F: last.F, res: last.res, wall_s: wall, hist: log.hist, restarts, opt };
 }
 
 if (require.main === module) {
-  const [coarse, target, every, cap, mode] = process.argv.slice(2);
-  console.log(JSON.stringify(run(coarse, target, +(every || 250), +(cap || 60000), mode === 'host')));
+  const [coarse, target, every, cap] = process.argv.slice(2);
+  console.log(JSON.stringify(run(coarse, target, +(every || 250), +(cap || 60000))));
 }
-module.exports = { run, residual, badCells };
+module.exports = { run };

@@ -45,7 +45,7 @@ class AxiHIP {
     }
   }
   setComps(c) { this.comps = Float64Array.from(c); loadAddon().axiSetComps(this.h, this.comps); }
-  setZones(z) { this.z = z; loadAddon().axiSetZones(this.h, z, this.W * this.H); }
+  setZones(z) { this.z = z; this._dispZones = false; loadAddon().axiSetZones(this.h, z, this.W * this.H); }
   setFs(fs) { this.fs = fs; loadAddon().axiSetFs(this.h, Float64Array.from([fs.r, fs.u, fs.p])); }
   setState(Q) { loadAddon().axiSetState(this.h, flat5(Q, this.W * this.H)); }
   step(n) { this._syncOpt(); loadAddon().axiStep(this.h, n || 1); this.iter += n || 1; }
@@ -83,7 +83,41 @@ class AxiHIP {
     return { F: o[0], Fb: o[1], Fw: o[2], mfuel: o[3], pw: o.slice(4, 12), resSum2: o[12], resN: o[13], resValid: o[14] !== 0, sec,
       bad: o[28] };
   }
+  /* Section sums of the current state on the device for [{col, rows}] (any
+   * number up to 256): per section a Float64Array(11) of mass flow, w, area,
+   * sum p dA and the w-weighted Tt, Pt, T, p, |V|, M, Y (cfd_post.js
+   * sectionAverages / stationAverages turn them into averages). */
+  sections(list) {
+    const n = list.length, cols = Int32Array.from(list.map(x => x.col)), rows = new Uint8Array(n * this.H), o = new Float64Array(11 * n);
+    list.forEach((x, q) => rows.set(x.rows, q * this.H));
+    loadAddon().axiSections(this.h, cols, rows, o);
+    return list.map((x, q) => o.subarray(11 * q, 11 * q + 11));
+  }
+  /* Display fields of the current state on the device (cfdDerived,
+   * src/cfd_post.js:10-40): {W, H, N, reg, mat, T, Tt, P, Pt, V, M, rho, Vx,
+   * Vth, Vr, Y, vort, ds} with Float32Array fields. fl = {T0, P0}; disp: cells
+   * under half full show as solid. G: a previous result to refill. */
+  derived(fl, disp, G) {
+    const N = this.W * this.H;
+    if (!this._dispZones) {
+      loadAddon().axiSetDisplayZones(this.h, Int8Array.from(this.z.stream), Uint8Array.from(this.z.mat));
+      this._dispZones = true;
+    }
+    if (!G || G.N !== N) {
+      G = { W: this.W, H: this.H, N, reg: new Uint8Array(N), mat: new Uint8Array(N), _f: new Float32Array(13 * N) };
+      DERIVED_KEYS.forEach((k, c) => { G[k] = G._f.subarray(c * N, (c + 1) * N); });
+    }
+    loadAddon().axiDerived(this.h, disp ? 1 : 0, fl.T0, fl.P0, G._f, G.reg, G.mat);
+    return G;
+  }
+  /* Grid sequencing on the device (interpState, src/app_cfd.js:321-358): the
+   * state of solver `src` (another grid) carried onto this solver's grid and
+   * made its current state. fl = {T0, P0} of the fallback at rest. */
+  transferFrom(src, fl) { loadAddon().axiTransfer(this.h, src.h, fl.T0, fl.P0); }
 }
+
+/* field order of AxiHIP.derived's planes */
+const DERIVED_KEYS = ['T', 'Tt', 'P', 'Pt', 'V', 'M', 'rho', 'Vx', 'Vth', 'Vr', 'Y', 'vort', 'ds'];
 
 /* Checkpoint of a long run in the reference's format (save and setup's load,
  * tools/cfdlib.js:16-21): the five state planes as one raw Float64 file and
@@ -111,4 +145,4 @@ function loadCheckpoint(file, N, ncomps) {
   return { Q, comps };
 }
 
-module.exports = { AxiHIP, axiGrid, saveCheckpoint, loadCheckpoint };
+module.exports = { AxiHIP, axiGrid, saveCheckpoint, loadCheckpoint, DERIVED_KEYS };

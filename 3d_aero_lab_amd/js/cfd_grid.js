@@ -4,8 +4,9 @@
  * grid, the cut-cell / stream / blade-row / heat-source zones and the
  * component constants (buildCFDGrid, buildZones, updateHeatZones,
  * streamMassFlows, compParams; src/cfd_grid.js:11-276), producing the typed
- * arrays AxiHIP takes, and the state transfer between grids of the grid
- * sequencing (interpState, src/app_cfd.js:321-358).
+ * arrays AxiHIP takes. The state transfer between grids of the grid
+ * sequencing (interpState, src/app_cfd.js:321-358) runs on the device
+ * (AxiHIP.transferFrom).
  *
  * Input: an engine description as data (`axispec`: stream walls and liners as
  * polyline points, body polygons, blade rows, combustor / afterburner extents,
@@ -449,51 +450,6 @@ function compParams(type, op, eff) {
   return c;
 }
 
-/* Fractional cell index of v in ascending centres, extrapolated linearly
- * outside (fracIdx, src/app_cfd.js:321-326). */
-function fracIdx(arr, v) {
-  const n = arr.length;
-  if (v <= arr[0]) return (v - arr[0]) / (arr[1] - arr[0]);
-  if (v >= arr[n - 1]) return n - 1 + (v - arr[n - 1]) / (arr[n - 1] - arr[n - 2]);
-  let lo = 0, hi = n - 1;
-  while (hi - lo > 1) { const m = (lo + hi) >> 1; if (arr[m] <= v) lo = m; else hi = m; }
-  return lo + (v - arr[lo]) / (arr[hi] - arr[lo]);
-}
-
-/* State of grid A carried onto grid B for grid sequencing (interpState,
- * src/app_cfd.js:331-358): bilinear in the cell centres over A's non-solid
- * cells; where none has weight, the nearest non-solid cell of a 7 x 7 window;
- * where there is none, quiescent freestream (fl.P0, fl.T0). Float64 planes. */
-function interpState(gA, zA, QA, gB, zB, fl) {
-  const N = gB.N, Q = [0, 1, 2, 3, 4].map(() => new Float64Array(N));
-  for (let j = 0; j < gB.H; j++) {
-    const fj = fracIdx(gA.rc, gB.rc[j]);
-    for (let i = 0; i < gB.W; i++) {
-      const fi = fracIdx(gA.xc, gB.xc[i]);
-      const i0 = clamp(Math.floor(fi), 0, gA.W - 1), j0 = clamp(Math.floor(fj), 0, gA.H - 1);
-      const tx = clamp(fi - i0, 0, 1), ty = clamp(fj - j0, 0, 1);
-      let wsum = 0;
-      const acc = [0, 0, 0, 0, 0];
-      for (const [di, dj, w] of [[0, 0, (1 - tx) * (1 - ty)], [1, 0, tx * (1 - ty)], [0, 1, (1 - tx) * ty], [1, 1, tx * ty]]) {
-        const ii = Math.min(i0 + di, gA.W - 1), jj = Math.min(j0 + dj, gA.H - 1), k = jj * gA.W + ii;
-        if (zA.solid[k] || w <= 0) continue;
-        for (let q = 0; q < 5; q++) acc[q] += w * QA[q][k];
-        wsum += w;
-      }
-      const kB = j * gB.W + i;
-      if (wsum > 0) { for (let q = 0; q < 5; q++) Q[q][kB] = acc[q] / wsum; continue; }
-      let best = -1, bd = 1e9;
-      for (let dj = -3; dj <= 3; dj++) for (let di = -3; di <= 3; di++) {
-        const ii = clamp(i0 + di, 0, gA.W - 1), jj = clamp(j0 + dj, 0, gA.H - 1), k = jj * gA.W + ii;
-        if (!zA.solid[k] && di * di + dj * dj < bd) { bd = di * di + dj * dj; best = k; }
-      }
-      if (best < 0) { Q[0][kB] = fl.P0 / (287.05 * fl.T0); Q[3][kB] = fl.P0 / 0.4; continue; }
-      for (let q = 0; q < 5; q++) Q[q][kB] = QA[q][best];
-    }
-  }
-  return Q;
-}
-
 /* Whole setup from an engine description: grid, zones, component constants. */
 function axiSetup(spec, level) {
   const grid = buildCFDGrid(spec, level || spec.level);
@@ -502,4 +458,4 @@ function axiSetup(spec, level) {
   return { grid, zones, comps };
 }
 
-module.exports = { GRID_LEVELS, COMP_ID, axisFaces, buildCFDGrid, engine, zoneColumns, buildZones, updateHeatZones, streamMassFlows, compParams, axiSetup, fracIdx, interpState };
+module.exports = { GRID_LEVELS, COMP_ID, axisFaces, buildCFDGrid, engine, zoneColumns, buildZones, updateHeatZones, streamMassFlows, compParams, axiSetup };

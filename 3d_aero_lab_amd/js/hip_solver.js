@@ -12,7 +12,7 @@
  *   const H = s.read();                    // {r, ru, rv, rw, E, dt}: Float32Array(N), reused buffers
  *   s.dispose();
  * Extras: setForceList(list), forces(alpha, M, S, xcg, zcg, cref), trefftz(xT, alpha, M, S),
- * residual(), readback(ref, xT), autotune(iters), timeSteps(n).
+ * residual(), readback(ref, xT), probe(pts, fs, M), autotune(iters), timeSteps(n).
  *
  * CfdMonitor: the readback controller state of the reference (CFD3.readback,
  * src/air_cfd.js:550-578) over a solver's device readback: residual, force
@@ -68,10 +68,28 @@ class CfdHIP {
     return loadAddon().readback(this.h, ref.alpha || 0, ref.M, ref.S, ref.xcg || 0, ref.zcg || 0, ref.cref || 1,
       xT === undefined ? NaN : xT);
   }
+  /* Display sampling of the current state on the device (the reference's
+   * cfdStencil / cfdSample / cfdState / cfdVelAt, src/air_cfd.js:453-490, 637-646)
+   * at the points pts = [x0, y0, z0, x1, ...] of the full model. fs: the
+   * freestream record {T, a, P, rho, q} of the dimensional state (null: none);
+   * M: the freestream Mach number of the velocities. Returns {n, idx (8n), w (8n),
+   * ok (n), W (5n: rho, u, v, w, p; NaN where !ok), state (12n, PROBE_KEYS order,
+   * or null), vel (3n, in units of the freestream speed; NaN where !vok), vok (n)}. */
+  probe(pts, fs, M) {
+    const P = pts instanceof Float64Array ? pts : Float64Array.from(pts), n = P.length / 3;
+    const o = { n, idx: new Int32Array(8 * n), w: new Float32Array(8 * n), ok: new Uint8Array(n), W: new Float64Array(5 * n),
+      state: fs ? new Float64Array(12 * n) : null, vel: new Float64Array(3 * n), vok: new Uint8Array(n) };
+    loadAddon().probe(this.h, P, fs ? Float64Array.of(fs.T, fs.a, fs.P, fs.rho, fs.q) : null, M || 0, o.idx, o.w, o.ok, o.W,
+      o.state, o.vel, o.vok);
+    return o;
+  }
   autotune(iters = 10) { this._syncOpt(); return loadAddon().autotune(this.h, iters); }
   timeSteps(n) { this._syncOpt(); const ms = loadAddon().timeSteps(this.h, n); this.it += n; return ms; }
   dispose() { if (this.h) { loadAddon().dispose(this.h); this.h = null; } }
 }
+
+/* columns of CfdHIP.probe's state, the fields of the reference's cfdState */
+const PROBE_KEYS = ['P', 'T', 'rho', 'V', 'M', 'Tt', 'Pt', 'Cp', 'Taw', 'Vx', 'Vy', 'Vz'];
 
 function forceArrays(list) {
   const n = list.length, q = new Int32Array(n), cx = new Float64Array(n), cz = new Float64Array(n), x = new Float64Array(n), z = new Float64Array(n);
@@ -202,4 +220,4 @@ class Server {
   dispose() { if (this.h) { loadAddon().serverDispose(this.h); this.h = null; } }
 }
 
-module.exports = { CfdHIP, CfdMonitor, ConvRule, RULE_RES_DROP, RULE_REFERENCE, Batch, Server, forceArrays, loadAddon };
+module.exports = { CfdHIP, CfdMonitor, ConvRule, RULE_RES_DROP, RULE_REFERENCE, Batch, Server, forceArrays, loadAddon, PROBE_KEYS };

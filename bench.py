@@ -271,12 +271,12 @@ def axi_leg(iters=14000):
     out["roofline_frac"] = out["cell_updates_per_s"] * 182 / (HBM_PEAK_GBS * 1e9)
     out["thrust_kN"] = out["thrust"]["F"] / 1000
     out["published_thrust_kN"] = 56.9
-    # the application's convergence loop with and without grid sequencing (js/axi_seq.js)
-    # (readback on the device monitor; the last run reads the whole state back and post-processes on the host)
+    # the application's convergence loop with and without grid sequencing (js/axi_seq.js;
+    # readbacks on the device monitor, the state carried between grids on the device)
     seq = {}
     coarse_dir = os.path.join(ROOT, "tests", "golden", "axi_tj_coarse")
-    for mode, coarse, extra in (("sequenced", coarse_dir, []), ("direct", "-", []), ("sequenced_host_readback", coarse_dir, ["host"])):
-        r = subprocess.run(["node", os.path.join(ROOT, "3d_aero_lab_amd", "js", "axi_seq.js"), coarse, AXI_CASE, "250", "60000", *extra],
+    for mode, coarse in (("sequenced", coarse_dir), ("direct", "-")):
+        r = subprocess.run(["node", os.path.join(ROOT, "3d_aero_lab_amd", "js", "axi_seq.js"), coarse, AXI_CASE, "250", "60000"],
                            capture_output=True, text=True, timeout=600, check=True)
         d = json.loads(r.stdout.strip().splitlines()[-1])
         seq[mode] = {"grids": [{k: g[k] for k in ("W", "H", "iters")} for g in d["grids"]], "thrust_kN": d["F"] / 1000,

@@ -110,6 +110,35 @@ typedef struct {
 } aero3d_readback_out;
 int aero3d_readback(aero3d_solver *s, const aero3d_readback_in *in, aero3d_readback_out *out);
 
+/* Display sampling of the current state at n points, on the device: the
+ * reference's cfdStencil + cfdSample + cfdState (src/air_cfd.js:453-490) and
+ * cfdVelAt (:637-646), which its pages run on read() output.
+ * xyz: 3n coordinates of the full model (y of either sign; v is mirrored
+ * below the symmetry plane). Per point: the trilinear stencil over fluid and
+ * fixed-state cells, renormalised, or the nearest fluid cell of the 4 x 4 x 4
+ * block when it holds no weight (idx / w: 8 entries each, corner bit 0 = +x,
+ * bit 1 = +y, bit 2 = +z; ok: found); the primitives rho, u, v, w, p of the
+ * blended conserved state (W: 5, NaN where not found); the dimensional state
+ * P, T, rho, V, M, Tt, Pt, Cp, Taw, Vx, Vy, Vz from the freestream record fs
+ * (state: 12); the velocity in units of the freestream speed (divided by
+ * max(M, 1e-3)) inside the box of cell centres, without the nearest-cell
+ * fallback (vel: 3, NaN where vok is 0). Any output pointer may be NULL (not
+ * copied back); state needs fs. Stencils, primitives and velocities equal
+ * the reference's bit for bit (FP32 weights, FP64 sums in its order); the
+ * dimensional state agrees to a few ulps (sqrt, pow). n <= 2^26; each axis
+ * needs >= 2 cells. Waits. */
+typedef struct { double T, a, P, rho, q; } aero3d_probe_fs;   /* the reference's fs record: freestream T, a, P, rho, q */
+typedef struct {
+  int32_t *idx;      /* 8n  */
+  float *w;          /* 8n  */
+  uint8_t *ok;       /* n   */
+  double *W;         /* 5n  */
+  double *state;     /* 12n */
+  double *vel;       /* 3n  */
+  uint8_t *vok;      /* n   */
+} aero3d_probe_out;
+int aero3d_probe(aero3d_solver *s, int n, const double *xyz, const aero3d_probe_fs *fs, double M, const aero3d_probe_out *out);
+
 /* No reference counterpart: a variant of aero3d_readback for callers that
  * cannot allocate device memory (code under stream capture, serving loops
  * that own a fixed arena). It works on device arrays the caller owns (no

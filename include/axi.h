@@ -96,6 +96,37 @@ int axi_monitor(axi_solver *s, const double comps[32], double P0, axi_monitor_ou
  * (1 where cfdSection counts row j: non-solid and in the section's stream). */
 int axi_monitor_sections(axi_solver *s, int nsec, const int *col, const uint8_t *rows);
 
+/* Section sums of the current state on the device, for cfdSection
+ * (src/cfd_post.js:83-102; the control loop's section averages) and the
+ * station averages (tools/stations.js:8-16): per section q, column col[q]
+ * and rows[q*H + j] = 1 where row j counts (non-solid, in the section's
+ * stream). The cell mass flow is the solver's mean axial face mass flux times
+ * the annulus area. sums[11 q + c], FP64 in the host's row order:
+ *   0 mass flow, 1 w = sum of max(mf, 0), 2 area (aperture-weighted),
+ *   3 sum p dA, 4..10 the w-weighted sums of Tt, Pt, T, p, |V|, M, Y.
+ * cfdSection's averages are the weighted sums over (w or 1), its P without
+ * positive flow pA / (A or 1); a station's Tt, Pt the weighted sums over w.
+ * nsec <= 256. Waits. */
+int axi_sections(axi_solver *s, int nsec, const int *col, const uint8_t *rows, double *sums);
+/* Display fields of the current state on the device (cfdDerived,
+ * src/cfd_post.js:10-40). axi_set_display_zones gives the zone codes they
+ * need once (stream: Int8, -1 outside the engine; mat: material of solid
+ * cells). disp = 1: cells under half full count as solid. fields: 13 planes
+ * of N floats, in the order T, Tt, P, Pt, V, M, rho, Vx, Vth, Vr, Y, vort, ds
+ * (NaN on solid cells, Vth 0); reg: 1 solid, 0 outside, 2 + stream inside;
+ * mat: the material code on solid cells, 0 elsewhere. Any output may be
+ * null. Every value is the reference's double arithmetic rounded to FP32
+ * once (Pt and ds through pow / log, within an ulp of a JS engine's). Waits. */
+int axi_set_display_zones(axi_solver *s, const int8_t *stream, const uint8_t *mat);
+int axi_derived(axi_solver *s, int disp, double T0, double P0, float *fields, uint8_t *reg, uint8_t *mat);
+/* Grid sequencing (interpState, src/app_cfd.js:321-358): the state of src
+ * carried onto dst's grid on the device and made dst's current state
+ * (bilinear in the cell centres over src's non-solid cells; where none has
+ * weight, the nearest non-solid cell of a 7 x 7 window; where there is none,
+ * P0 / (287.05 T0) density and P0 / 0.4 energy at rest). FP64 arithmetic,
+ * rounded to FP32 once. Both solvers on one device; waits for src's stream. */
+int axi_transfer(axi_solver *dst, const axi_solver *src, double T0, double P0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -130,13 +130,13 @@ function caseXcoarse() {
 }
 
 /* display fields (cfdDerived, src/cfd_post.js:10-40) of the solver's current
- * state, with and without the display solid rule, as sha256 of each array */
+ * state, with and without the display solid rule */
 const DER_KEYS = ['reg', 'mat', 'T', 'Tt', 'P', 'Pt', 'V', 'M', 'rho', 'Vx', 'Vth', 'Vr', 'Y', 'vort', 'ds'];
 function derived(w, C) {
   for (const disp of [false, true]) {
     const z = Object.assign({}, C.zones);   // cfdDerived caches its display mask on the zones record
     const G = A.cfdDerived(C.grid, z, C.S.Q, C.fl, null, disp);
-    for (const k of DER_KEYS) w.arr(`der${disp ? 1 : 0}_${k}`, G[k], true);
+    for (const k of DER_KEYS) w.arr(`der${disp ? 1 : 0}_${k}`, G[k]);
   }
 }
 
@@ -233,7 +233,8 @@ function caseSetups() {
 /* interpState (grid sequencing, src/app_cfd.js:331-358 with fracIdx :321-326),
  * taken as text from the reference's app module and evaluated here (the rest
  * of that module needs a browser page): the xcoarse reference state carried
- * onto the medium grid for the TJ and the HBTF, as sha256 of the five planes. */
+ * onto the medium grid for the TJ and the HBTF, as sha256 of the five planes
+ * (Float64, and rounded to Float32 as a solver's state holds them). */
 function refInterp() {
   const src = fs.readFileSync(path.join(REF, 'src', 'app_cfd.js'), 'utf8');
   const grab = name => {
@@ -253,6 +254,7 @@ function caseInterp() {
     globalThis.CUR = { sol: { fl: B.fl } };      // the all-solid fallback reads the freestream from the app state
     const Q = I.interpState(A0.grid, A0.zones, A0.S.Q, B.grid, B.zones);
     out[type] = { from: 'xcoarse', to: 'medium', iters, sha256: Q.map(a => sha(Buffer.from(a.buffer))),
+      sha256_f32: Q.map(a => sha(Buffer.from(Float32Array.from(a).buffer))),
       sample: Q.map(a => Array.from(a.subarray(0, 64))) };
   }
   fs.mkdirSync(path.join(OUT, 'axi_interp'), { recursive: true });

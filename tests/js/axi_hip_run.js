@@ -1,7 +1,8 @@
 'use strict';
 /* The reference's long-run flow (tools/longrun.js:7-12) on AxiHIP: step in
- * chunks of 250, thrust from the solver's own sources after each chunk, and
- * (with ctl=1) the exit-temperature control every 250 iterations.
+ * chunks of 250, thrust from the solver's own sources after each chunk (the
+ * device monitor), and (with ctl=1) the exit-temperature control every 250
+ * iterations on the monitor's control section.
  *   node tests/js/axi_hip_run.js <golden dir> <iters> <report every> <ctl 0|1>  */
 const fs = require('fs'), path = require('path'), zlib = require('zlib');
 const PKG = path.join(__dirname, '..', '..', '3d_aero_lab_amd', 'js');
@@ -19,22 +20,19 @@ for (const k of ['solid', 'phi', 'ae', 'an', 'comp', 'wgt', 'lmix', 'tgt', 'rate
 const comps = Float64Array.from(arr('comps'));
 const Q0 = [0, 1, 2, 3, 4].map(q => arr(`Q0_${q}`));
 const S = new AxiHIP(grid, zones, comps, s.fs, Q0, Object.assign({}, s.opt, { hf: s.hf }));
+S.monitorSections(P.controlStations(s.ctl).map(st => P.sectionRows(grid, zones, st.x, st.si)));
 const hist = [];
 const t0 = Date.now();
 let it = 0;
 while (it < iters) {
   const n = Math.min(ctlOn ? 250 : every, iters - it);
   S.step(n); it += n;
-  let Q = null;
+  const f = S.monitor(comps, s.ctl.P0);
   if (ctlOn) {
-    Q = S.read();
-    const f0 = P.cfdForces(grid, zones, Q, comps, s.ctl.P0, 0, { S: S.src, F: S.mfx });
-    P.cfdControl(s.type || 'tj', s.ctl, grid, zones, Q, S.mfx, comps, f0.pw, s.ctl.etam, 250);
+    P.controlFromSections(s.type || 'tj', s.ctl, st => P.sectionFromSums(f.sec[st.id === 4 ? 0 : 1]), comps, f.pw, s.ctl.etam, 250);
     S.setComps(comps);
   }
   if (it % every === 0 || it === iters) {
-    if (!Q) Q = S.read();
-    const f = P.cfdForces(grid, zones, Q, comps, s.ctl.P0, s.hf, { S: S.src, F: S.mfx });
     hist.push({ it, F: f.F, Fb: f.Fb, Fw: f.Fw, mfuel: f.mfuel, K4: comps[17], s: (Date.now() - t0) / 1000 });
     console.error(JSON.stringify(hist[hist.length - 1]));
   }

@@ -1,7 +1,8 @@
 'use strict';
 /* The reference's CPU driver flow (tools/test_cfd3.js:17-24) on the HIP solver:
  * aircraft -> cfdGrid -> cfdGeom -> cfdForceList -> new CfdHIP -> setFree /
- * setFix([F,F,F]) / reset -> step(50) + cfdForces, printed as JSON lines.
+ * setFix([F,F,F]) / reset -> step(50) + force coefficients (device FP64
+ * reduction, the reference's cfdForces on the device), printed as JSON.
  *   node tests/js/cfd3_hip_run.js <type> <cells> <iters> <M> <alpha> <delta> <cfl>
  */
 const path = require('path');
@@ -21,10 +22,8 @@ s.setForceList(list);
 const out = [];
 for (let it = 50; it <= iters; it += 50) {
   s.step(50);
-  const host = A.cfdForces(g, list, s.read(), alpha, M, C.ref.S, C.ref.xcg, 0, C.ref.c);   // host FP64 sum on read()
-  const dev = s.forces(alpha, M, C.ref.S, C.ref.xcg, 0, C.ref.c);                           // device FP64 reduction
-  out.push({ it, host, dev });
+  out.push({ it, dev: s.forces(alpha, M, C.ref.S, C.ref.xcg, 0, C.ref.c) });
 }
-const cdi = s.trefftz(C.xT, alpha, M, C.ref.S), cdiHost = A.cfdTrefftz(g, geo.t, s.read(), C.xT, alpha, M, C.ref.S);
+const cdi = s.trefftz(C.xT, alpha, M, C.ref.S);
 s.dispose();
-console.log(JSON.stringify({ N: g.N, hist: out, cdi, cdiHost }));
+console.log(JSON.stringify({ N: g.N, hist: out, cdi }));

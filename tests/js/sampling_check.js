@@ -1,30 +1,47 @@
 'use strict';
-/* Display sampling of 3d_aero_lab_amd/js/air_cfd.js (cfdStencil, cfdSample,
- * cfdState, cfdVelAt) on the reference's fighter state and probe points of
- * tests/golden/sampling; prints which outputs equal the reference's bit for bit.
- *   node tests/js/sampling_check.js <golden dir> */
+/* Display sampling on the device through the Node addon (CfdHIP.probe of
+ * 3d_aero_lab_amd/js/hip_solver.js) on the reference's fighter state and the
+ * probe points of tests/golden/sampling; prints which outputs equal the
+ * reference's (cfdStencil / cfdSample / cfdState / cfdVelAt) bit for bit and
+ * the largest relative difference of the fields that go through sqrt / pow.
+ *   node tests/js/sampling_check.js <sampling dir> <fighter30k dir> */
 const fs = require('fs'), path = require('path'), zlib = require('zlib');
 const A = require(path.join(__dirname, '..', '..', '3d_aero_lab_amd', 'js', 'air_cfd.js'));
-const dir = process.argv[2];
-const m = JSON.parse(fs.readFileSync(path.join(dir, 'manifest.json'), 'utf8'));
+const { CfdHIP, PROBE_KEYS } = require(path.join(__dirname, '..', '..', '3d_aero_lab_amd', 'js', 'hip_solver.js'));
 const T = { f4: Float32Array, f8: Float64Array, u1: Uint8Array, i4: Int32Array };
-const arr = n => { const a = m.arrays[n]; const b = zlib.gunzipSync(fs.readFileSync(path.join(dir, a.file))); return new T[a.dtype](b.buffer.slice(b.byteOffset, b.byteOffset + b.byteLength)); };
-const s = m.scalars;
-const g = A.gridFromFaces(arr('fx'), arr('fy'), arr('fz'));
-const t = arr('t'), H = { r: arr('it200_r'), ru: arr('it200_ru'), rv: arr('it200_rv'), rw: arr('it200_rw'), E: arr('it200_E') };
-const pts = arr('pts'), n = pts.length / 3;
-const idx = new Int32Array(8 * n), w = new Float32Array(8 * n), ok = new Uint8Array(n), got = new Uint8Array(n);
-const Wv = new Float64Array(5 * n), st = new Float64Array(12 * n), vel = new Float64Array(3 * n), vok = new Uint8Array(n);
-const W = [0, 0, 0, 0, 0], o = {}, vo = [0, 0, 0];
-for (let p = 0; p < n; p++) {
-  const x = pts[3 * p], y = pts[3 * p + 1], z = pts[3 * p + 2];
-  ok[p] = A.cfdStencil(g, t, x, y, z, idx, w, 8 * p) ? 1 : 0;
-  if (A.cfdSample(H, idx, w, 8 * p, y < 0 ? -1 : 1, W)) {
-    got[p] = 1; for (let c = 0; c < 5; c++) Wv[5 * p + c] = W[c];
-    A.cfdState(s.fs, W, o); s.keys.forEach((k, c) => { st[12 * p + c] = o[k]; });
-  }
-  if (A.cfdVelAt(g, t, H, s.M, x, y, z, vo)) { vok[p] = 1; vel[3 * p] = vo[0]; vel[3 * p + 1] = vo[1]; vel[3 * p + 2] = vo[2]; }
+function loader(dir) {
+  const m = JSON.parse(fs.readFileSync(path.join(dir, 'manifest.json'), 'utf8'));
+  const arr = n => { const a = m.arrays[n]; const b = zlib.gunzipSync(fs.readFileSync(path.join(dir, a.file))); return new T[a.dtype](b.buffer.slice(b.byteOffset, b.byteOffset + b.byteLength)); };
+  return { m, s: m.scalars, arr };
 }
-const same = (a, b) => Buffer.from(a.buffer, a.byteOffset, a.byteLength).equals(Buffer.from(b.buffer, b.byteOffset, b.byteLength));
-console.log(JSON.stringify({ n, idx: same(idx, arr('idx')), w: same(w, arr('w')), ok: same(ok, arr('ok')), got: same(got, arr('got')),
-  W: same(Wv, arr('W')), state: same(st, arr('state')), vel: same(vel, arr('vel')), vok: same(vok, arr('vok')) }));
+const S = loader(process.argv[2]), F = loader(process.argv[3]);
+const g = A.gridFromFaces(F.arr('fx'), F.arr('fy'), F.arr('fz'));
+const geo = { t: F.arr('t'), phi: F.arr('phi'), ap: [0, 1, 2, 3].map(d => F.arr('ap' + d)) };
+const s = new CfdHIP(g, geo);
+const H = { r: S.arr('it200_r'), ru: S.arr('it200_ru'), rv: S.arr('it200_rv'), rw: S.arr('it200_rw'), E: S.arr('it200_E') };
+// write() re-applies the fixed states (cell types 3, 4, 5): give it the run's own, as the state holds them
+const fixOf = ty => { const k = Math.max(0, geo.t.indexOf(ty)); return [H.r[k], H.ru[k], H.rv[k], H.rw[k], H.E[k]]; };
+s.setFix([fixOf(3), fixOf(4), fixOf(5)]);
+s.write(H);
+const pts = S.arr('pts'), n = pts.length / 3;
+const o = s.probe(pts, S.s.fs, S.s.M);
+s.dispose();
+const bytes = a => Buffer.from(a.buffer, a.byteOffset, a.byteLength);
+const same = (a, b) => bytes(a).equals(bytes(b));
+const ok = S.arr('ok'), vok = S.arr('vok'), W = S.arr('W'), st = S.arr('state'), vel = S.arr('vel');
+let Wsame = true, velSame = true, stExact = true;
+const worst = {};
+const exact = new Set(['P', 'T', 'rho', 'Cp', 'Vx', 'Vy', 'Vz']);
+for (let p = 0; p < n; p++) {
+  if (ok[p]) {
+    for (let c = 0; c < 5; c++) if (!Object.is(o.W[5 * p + c], W[5 * p + c])) Wsame = false;
+    PROBE_KEYS.forEach((k, c) => {
+      const a = o.state[12 * p + c], b = st[12 * p + c];
+      if (exact.has(k)) { if (!Object.is(a, b)) stExact = false; }
+      else worst[k] = Math.max(worst[k] || 0, Math.abs(a - b) / Math.abs(b));
+    });
+  }
+  if (vok[p]) for (let c = 0; c < 3; c++) if (!Object.is(o.vel[3 * p + c], vel[3 * p + c])) velSame = false;
+}
+console.log(JSON.stringify({ n, idx: same(o.idx, S.arr('idx')), w: same(o.w, S.arr('w')), ok: same(o.ok, ok), got: same(o.ok, S.arr('got')),
+  W: Wsame, stateExact: stExact, vel: velSame, vok: same(o.vok, vok), keys: PROBE_KEYS.join() === S.s.keys.join(), worst }));

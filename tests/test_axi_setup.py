@@ -3,8 +3,7 @@ buildZones, updateHeatZones, streamMassFlows, compParams; src/cfd_grid.js:11-276
 against the reference's own setups: grid faces, every zone array and the
 component constants byte-identical (sha256) for the turbojet, the high- and
 low-bypass turbofans at all four grid levels (tests/golden/axi_setups) and for
-the five setups the solver fixtures carry; the grid-sequencing state transfer
-(interpState) likewise."""
+the five setups the solver fixtures carry."""
 import json
 import os
 import shutil
@@ -26,18 +25,6 @@ def test_axi_setup_byte_identical(case):
     assert res
     bad = {name: [k for k, ok in v.items() if not ok] for name, v in res.items() if not all(v.values())}
     assert not bad, bad
-
-
-@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
-def test_interp_state_byte_identical():
-    """Grid-sequencing state transfer (interpState, src/app_cfd.js:331-358) of
-    the reference's xcoarse TJ (500 iterations) and HBTF (100 iterations) states
-    onto the medium grids: all five planes byte-identical to the reference's."""
-    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "axi_interp_check.js"), os.path.join(ROOT, "tests", "golden")],
-                       capture_output=True, text=True, timeout=300)
-    assert r.returncode == 0, r.stderr[-2000:]
-    res = json.loads(r.stdout)
-    assert set(res) == {"tj", "hbtf"} and all(all(v) for v in res.values()), res
 
 
 @pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")

@@ -179,7 +179,7 @@ def test_axi_longrun_matches_reference_history(setup, longrun, iters, published,
 def test_axi_grid_sequencing_converges():
     """The reference application's convergence loop (src/app_cfd.js:270-297,
     361-404) on the HIP solver, TJ medium: started on the coarse grid and moved
-    by interpState, and started on the medium grid directly. Both runs reach the
+    by the device transfer (interpState), and started on the medium grid directly. Both runs reach the
     application's convergence test (density residual below 2e-5); the sequenced
     one needs fewer medium-grid iterations; both thrusts lie within 2 % of the
     published 56.9 kN (README.md:73) - the converged states differ, so the bar
@@ -198,37 +198,6 @@ def test_axi_grid_sequencing_converges():
         assert abs(res[mode]["F"] - 56.9e3) / 56.9e3 < 0.02
     assert len(res["seq"]["grids"]) == 2 and res["seq"]["grids"][0]["W"] < 450
     assert res["seq"]["grids"][-1]["iters"] < res["direct"]["grids"][-1]["iters"]
-    # the same loop with the host readback (whole state back, cfdForces /
-    # cfdSection on the host): the same iterations and the same thrust to
-    # round-off of the grid sums
-    r = subprocess.run(["node", os.path.join(ROOT, "3d_aero_lab_amd", "js", "axi_seq.js"), os.path.join(ROOT, "tests", "golden", "axi_tj_coarse"),
-                        os.path.join(ROOT, "tests", "golden", "axi_tj_medium"), "250", "60000", "host"],
-                       capture_output=True, text=True, timeout=300)
-    assert r.returncode == 0, r.stderr[-2000:]
-    host = json.loads(r.stdout.strip().splitlines()[-1])
-    say("host readback", [(x["W"], x["H"], x["iters"], round(x["F"], 1)) for x in host["grids"]], round(host["wall_s"], 2),
-        "device", round(res["seq"]["wall_s"], 2))
-    assert host["mode"] == "host" and res["seq"]["mode"] == "device"
-    assert [x["iters"] for x in host["grids"]] == [x["iters"] for x in res["seq"]["grids"]]
-    assert abs(host["F"] - res["seq"]["F"]) / abs(host["F"]) < 1e-6
-
-
-@pytest.mark.parametrize("case,iters", [("axi_tj_medium", 300), ("axi_hbtf_medium", 300), ("axi_lbtf_xcoarse", 300)])
-def test_axi_device_monitor_matches_host(case, iters):
-    """axi_monitor (FP64 on the device) against cfdForces, the readback
-    residual and cfdSection computed on the host from read() of the same state:
-    grid sums to 1e-12 (only their order differs), the control sections (one
-    thread, the host's row order) bit for bit, and no residual before a second
-    monitor call."""
-    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "axi_monitor_check.js"), os.path.join(ROOT, "tests", "golden", case),
-                        str(iters)], capture_output=True, text=True, timeout=300)
-    assert r.returncode == 0, r.stderr[-2000:]
-    d = json.loads(r.stdout.strip().splitlines()[-1])
-    say("monitor", case, d)
-    assert d["first_res_valid"] is False and d["second_res_valid"] is True
-    assert d["forces"] < 1e-12 and d["residual"] < 1e-12, d
-    assert d["sections_exact"], d
-    assert d["nsec"] == (2 if case != "axi_hbtf_medium" else 1)
 
 
 def test_axi_checkpoint_resume_is_exact(tmp_path):
