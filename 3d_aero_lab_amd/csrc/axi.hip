@@ -350,9 +350,9 @@ int axi_monitor(axi_solver *s, const double comps[32], double P0, axi_monitor_ou
   const AxiView v = view(s);
   const int nb = std::max(1, std::min(kMonBlocks, (s->N + kB - 1) / kB));
   monitor_kernel<<<nb, kB, 0, s->stream>>>(v, s->rcD, s->drD, s->dxD, s->Q, s->src, s->mfx, s->rsnap, s->mon_in, s->mon_part);
-  monitor_finish<<<1, 64, 0, s->stream>>>(s->mon_part, nb, s->mon_out);
+  monitor_finish<<<1, kMonFinishThreads, 0, s->stream>>>(s->mon_part, nb, s->mon_out);
   if (s->nsec > 0)
-    section_kernel<<<1, kSecBlock, 0, s->stream>>>(v, s->rcD, s->drD, s->Q, s->mfx, s->nsec, s->sec_col, s->sec_rows, s->mon_out + kMonK);
+    section_kernel<<<s->nsec, kSecBlock, 0, s->stream>>>(v, s->rcD, s->drD, s->Q, s->mfx, s->nsec, s->sec_col, s->sec_rows, s->mon_out + kMonK);
   ACHK(s, hipGetLastError());
   double o[kMonK + kSecK * kMonMaxSec];
   ACHK(s, hipMemcpyAsync(o, s->mon_out, sizeof(double) * (kMonK + kSecK * s->nsec), hipMemcpyDeviceToHost, s->stream));
@@ -403,7 +403,7 @@ int axi_sections(axi_solver *s, int nsec, const int *col, const uint8_t *rows, d
   double *dout = (double *)(s->post + bc + br);
   ACHK(s, hipMemcpyAsync(dcol, col, sizeof(int) * nsec, hipMemcpyHostToDevice, s->stream));
   ACHK(s, hipMemcpyAsync(drows, rows, (size_t)nsec * s->H, hipMemcpyHostToDevice, s->stream));
-  section_kernel<<<(nsec + kSecBlock - 1) / kSecBlock, kSecBlock, 0, s->stream>>>(view(s), s->rcD, s->drD, s->Q, s->mfx, nsec, dcol, drows, dout);
+  section_kernel<<<nsec, kSecBlock, 0, s->stream>>>(view(s), s->rcD, s->drD, s->Q, s->mfx, nsec, dcol, drows, dout);
   ACHK(s, hipGetLastError());
   ACHK(s, hipMemcpyAsync(sums, dout, bo, hipMemcpyDeviceToHost, s->stream));
   ACHK(s, hipStreamSynchronize(s->stream));

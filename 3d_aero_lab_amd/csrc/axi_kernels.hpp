@@ -630,45 +630,66 @@ __global__ void __launch_bounds__(kB) monitor_kernel(AxiView v, const double *__
     }
 }
 
-// partials of nb blocks -> out[0..kMonK), block order
-__global__ void monitor_finish(const double *__restrict__ partials, int nb, double *__restrict__ out) {
-  const int c = threadIdx.x;
-  if (c >= kMonK) return;
+// partials of nb blocks -> out[0..kMonK): one wave per component, lane l
+// sums blocks l, l + 64, ..., then a fixed shuffle tree (the same order at
+// every call)
+constexpr int kMonFinishThreads = 64 * kMonK;
+__global__ void __launch_bounds__(kMonFinishThreads) monitor_finish(const double *__restrict__ partials, int nb, double *__restrict__ out) {
+  const int c = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double a = 0.0;
-  for (int b = 0; b < nb; b++) a += partials[b * kMonK + c];
-  out[c] = a;
+  for (int b = lane; b < nb; b += 64) a += partials[b * kMonK + c];
+  for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+  if (lane == 0) out[c] = a;
 }
 
-// cfdSection's sums (src/cfd_post.js:83-102) for section s at column col[s]
-// over rows[s H + j] != 0, in the host's row order, with the solver's mean
-// axial face mass flux as the cell mass flow; written to out[kSecK s + c]:
+// cfdSection's sums (src/cfd_post.js:83-102) for section s = blockIdx.x at
+// column col[s] over rows[s H + j] != 0, with the solver's mean axial face
+// mass flux as the cell mass flow; written to out[kSecK s + c]:
 //   0 mass flow, 1 sum of max(mf, 0) (the weight w), 2 area A, 3 sum p dA,
 //   4..10 w-weighted sums of Tt, Pt, T, p, |V|, M, Y
+// The rows' terms are computed in parallel, kSecBlock rows at a time, into
+// LDS; thread c then adds term c over the rows in row order, as the host's
+// sequential loop does, so the sums are the host's to the bit.
 constexpr int kSecK = 11;
-constexpr int kSecBlock = 64;     // threads per block of section_kernel
-__global__ void section_kernel(AxiView v, const double *__restrict__ rcD, const double *__restrict__ drD, const float *__restrict__ Q,
-                               const float *__restrict__ mfx, int nsec, const int *__restrict__ col, const uint8_t *__restrict__ rows,
-                               double *__restrict__ out) {
+constexpr int kSecBlock = 256;    // threads per block of section_kernel (one block per section)
+__global__ void __launch_bounds__(kSecBlock) section_kernel(AxiView v, const double *__restrict__ rcD, const double *__restrict__ drD,
+                                                            const float *__restrict__ Q, const float *__restrict__ mfx, int nsec,
+                                                            const int *__restrict__ col, const uint8_t *__restrict__ rows,
+                                                            double *__restrict__ out) {
 #pragma clang fp contract(off)
-  const int s = blockIdx.x * kSecBlock + threadIdx.x;
+  __shared__ double term[kSecK][kSecBlock];
+  const int s = blockIdx.x, t = threadIdx.x;
   if (s >= nsec) return;
   const int W = v.W, H = v.H, N = v.N, i = col[s];
-  double acc[kSecK];
-  for (int c = 0; c < kSecK; c++) acc[c] = 0.0;
-  for (int j = 0; j < H; j++) {
-    if (!rows[(size_t)s * H + j]) continue;
-    const int k = j * W + i;
-    const PrimD q = prim_d(Q, N, k);
-    const double al = jmin(1.0, 0.5 * ((i > 0 ? (double)v.ae[k - 1] : 1.0) + (double)v.ae[k]));
-    const double dA = 2.0 * kPi * rcD[j] * drD[j] * al;
-    const double T = q.p / (q.r * kRGas), cp = q.g * kRGas / (q.g - 1.0), V2 = q.u * q.u + q.v * q.v, Tt = T + V2 / (2.0 * cp);
-    const double Pt = q.p * pow(Tt / T, q.g / (q.g - 1.0));
-    const double mf = (double)mfx[k] * 2.0 * kPi * rcD[j] * drD[j], wk = jmax(mf, 0.0);
-    acc[0] += mf; acc[1] += wk; acc[2] += dA; acc[3] += q.p * dA;
-    acc[4] += wk * Tt; acc[5] += wk * Pt; acc[6] += wk * T; acc[7] += wk * q.p; acc[8] += wk * sqrt(V2);
-    acc[9] += wk * sqrt(V2 / (q.g * q.p / q.r)); acc[10] += wk * q.Y;
+  double acc = 0.0;                 // thread t < kSecK: the running sum of term t
+  for (int j0 = 0; j0 < H; j0 += kSecBlock) {
+    const int j = j0 + t;
+    const bool on = j < H && rows[(size_t)s * H + j] != 0;
+    if (!on) {
+      // a row that does not count adds +0: the sums start at +0 and never become -0
+      // (x + (-x) rounds to +0), so acc + 0 == acc to the bit
+      for (int c = 0; c < kSecK; c++) term[c][t] = 0.0;
+    } else {
+      const int k = j * W + i;
+      const PrimD q = prim_d(Q, N, k);
+      const double al = jmin(1.0, 0.5 * ((i > 0 ? (double)v.ae[k - 1] : 1.0) + (double)v.ae[k]));
+      const double dA = 2.0 * kPi * rcD[j] * drD[j] * al;
+      const double T = q.p / (q.r * kRGas), cp = q.g * kRGas / (q.g - 1.0), V2 = q.u * q.u + q.v * q.v, Tt = T + V2 / (2.0 * cp);
+      const double Pt = q.p * pow(Tt / T, q.g / (q.g - 1.0));
+      const double mf = (double)mfx[k] * 2.0 * kPi * rcD[j] * drD[j], wk = jmax(mf, 0.0);
+      term[0][t] = mf; term[1][t] = wk; term[2][t] = dA; term[3][t] = q.p * dA;
+      term[4][t] = wk * Tt; term[5][t] = wk * Pt; term[6][t] = wk * T; term[7][t] = wk * q.p; term[8][t] = wk * sqrt(V2);
+      term[9][t] = wk * sqrt(V2 / (q.g * q.p / q.r)); term[10][t] = wk * q.Y;
+    }
+    __syncthreads();
+    if (t < kSecK) {
+      const int n = min(kSecBlock, H - j0);
+#pragma unroll 8
+      for (int r = 0; r < n; r++) acc += term[t][r];
+    }
+    __syncthreads();
   }
-  for (int c = 0; c < kSecK; c++) out[(size_t)kSecK * s + c] = acc[c];
+  if (t < kSecK) out[(size_t)kSecK * s + t] = acc;
 }
 
 }  // namespace axik

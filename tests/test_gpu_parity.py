@@ -119,6 +119,46 @@ def test_transport_1m_matches_oracle():
         assert rel(fd[k], fo[k]) < COEF_TOL, (k, fd[k], fo[k])
 
 
+@pytest.mark.timeout(900)
+def test_transport_1m_coefficients_after_2000_iterations():
+    """The 1.0M-cell transport carried to 2000 iterations, as the headline
+    configuration is quoted (CL / CD / Cm of the reference's driver flow at
+    2000 iterations): the GPU's coefficients against the oracle's within 1e-4.
+    The oracle takes about 80 s on 16 cores; progress every 250 iterations."""
+    import math
+    import os
+    import time
+    import aero3d
+    import geometry
+    grid, geo, fl, meta = geometry.build_case("transport:1000000")
+    M, alpha = 0.78, 2.0
+    a = math.radians(alpha)
+    F = [1.0, M * math.cos(a), 0.0, M * math.sin(a), 1 / (1.4 * 0.4) + 0.5 * M * M]
+    fix = meta["fix"]
+    r = meta["ref"]
+    fref = (alpha, M, r["S"], r["xcg"], 0.0, r["c"])
+    s = aero3d.CfdHIP(grid, geo)
+    s.opt["cfl"] = 0.5
+    s.set_free(F); s.set_fix(fix); s.reset(); s.set_force_list(fl)
+    s.step(2000)
+    Q = s.read()
+    fd = s.forces(*fref)
+    s.dispose()
+    say("transport 1M, 2000 iterations: GPU", fd)
+    o = O.Oracle3D(grid, geo, threads=min(16, os.cpu_count() or 1))
+    o.set_free(F); o.set_fix(fix); o.set_opt(0.5, 2); o.reset()
+    t0 = time.perf_counter()
+    for done in range(250, 2001, 250):
+        o.step(250)
+        say("oracle at", done, "iterations,", round(time.perf_counter() - t0, 1), "s")
+    Qo = o.read()
+    fo = O.forces(Qo, fl, *fref)
+    d = rel_diff(Q, Qo, geo["t"])
+    say("transport 1M, 2000 iterations: state", d, "coefficients GPU", fd, "oracle", fo)
+    for k in ("CL", "CD", "Cm"):
+        assert rel(fd[k], fo[k]) < COEF_TOL, (k, fd[k], fo[k])
+
+
 def test_device_reductions_match_oracle_on_same_state():
     """Forces, Trefftz and residual in FP64 on the device equal the oracle's
     sums over the identical FP32 state to summation-order rounding."""
