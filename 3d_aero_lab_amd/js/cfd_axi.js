@@ -100,6 +100,7 @@ class AxiHIP {
   derived(fl, disp, G) {
     const N = this.W * this.H;
     if (!this._dispZones) {
+      if (!this.z.stream || !this.z.mat) throw new Error('AxiHIP.derived: the zones need stream and mat (buildZones)');
       loadAddon().axiSetDisplayZones(this.h, Int8Array.from(this.z.stream), Uint8Array.from(this.z.mat));
       this._dispZones = true;
     }

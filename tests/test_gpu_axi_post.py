@@ -9,6 +9,10 @@ display Pt and ds within one FP32 ulp); the thrust and the density residual
 of the monitor within 1e-12 (grid-wide sums in another order); the transfer
 onto another grid bit for bit with the oracle's result rounded to FP32.
 """
+import json
+import os
+import shutil
+import subprocess
 import sys
 
 import numpy as np
@@ -19,6 +23,7 @@ from axi_case import load
 from golden_io import Golden
 
 pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 STATION_TYPES = {"axi_tj_xcoarse": "tj", "axi_hbtf_xcoarse": "hbtf", "axi_lbtf_xcoarse": "lbtf", "axi_tj_medium": "tj"}
 STATIONS = {   # tools/stations.js:8-16
@@ -141,3 +146,49 @@ def test_axi_transfer_equals_oracle(typ, iters):
     B.step(10)
     assert np.isfinite(B.read()[0]).all()
     A.dispose(); B.dispose()
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+@pytest.mark.parametrize("case,tag", [("axi_tj_xcoarse", "it500"), ("axi_lbtf_xcoarse", "it100")])
+def test_display_fields_and_stations_from_node(case, tag, tmp_path):
+    """AxiHIP.derived and the station averages of js/cfd_post.js (over
+    AxiHIP.sections) through the Node addon, against the oracle on the same
+    FP32 states, to the bars above."""
+    g, z, Q0, comps, xf, rf = load(case)
+    z["mat"] = g.arr("z_mat")
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "axi_display_dump.js"), g.dir, tag, str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    grid = OA.grid_arrays(xf, rf)
+    W, H, N = grid["W"], grid["H"], grid["W"] * grid["H"]
+    Q = np.stack([g.arr(f"{tag}_Q{q}") for q in range(5)]).astype(np.float32)
+    for disp in (0, 1):
+        f = np.fromfile(tmp_path / f"der{disp}.f32", np.float32).reshape(13, N)
+        reg = np.fromfile(tmp_path / f"reg{disp}.u8", np.uint8)
+        mat = np.fromfile(tmp_path / f"mat{disp}.u8", np.uint8)
+        fo, rego, mato = OA.derived(grid, z, Q, g.s["fl"]["T0"], g.s["fl"]["P0"], disp)
+        assert np.array_equal(reg, rego) and np.array_equal(mat, mato)
+        for c, key in enumerate(OA.DERIVED_KEYS):
+            a, b = f[c], fo[c]
+            if key in ("Pt", "ds"):
+                fin = np.isfinite(b)
+                assert np.array_equal(fin, np.isfinite(a)), key
+                assert (np.abs(a[fin].astype(np.float64) - b[fin]) <= np.spacing(np.abs(b[fin])).astype(np.float64)).all(), (disp, key)
+            else:
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (disp, key)
+    # stations after one step, from the stepped state and mass flux the script read back
+    Q1 = np.fromfile(tmp_path / "q1.f32", np.float32).reshape(5, N)
+    mfx = np.fromfile(tmp_path / "mfx.f32", np.float32)
+    st = json.load(open(tmp_path / "stations.json"))
+    order = g.s["ctl"]["order"]
+    typ = STATION_TYPES[case]
+    assert [s["id"] for s in st] == [sid for sid, _, _ in STATIONS[typ]]
+    for s, (sid, x, sn) in zip(st, STATIONS[typ]):
+        col = col_at(grid["xc"], x)
+        o = dict(zip(OA.SECTION_SUMS, OA.section_sums(grid, z, Q1, mfx, col, rows_of(z, W, H, col, order.index(sn)))))
+        assert s["mdot"] == o["mdot"]
+        if o["w"]:
+            assert s["Tt"] == o["Tt"] / o["w"] and rel(s["Pt"], o["Pt"] / o["w"]) <= 2e-15, (sid, s, o)
+        else:
+            assert s["Tt"] is None and s["Pt"] is None       # 0 / 0 -> NaN, JSON null
+    say(case, "display fields and", len(st), "stations through Node")
