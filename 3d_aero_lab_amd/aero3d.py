@@ -573,7 +573,8 @@ class AxiZones(C.Structure):
 AXI_EXPORTS = ["axi_create", "axi_destroy", "axi_last_error", "axi_set_opt", "axi_set_comps", "axi_set_zones", "axi_set_fs",
                "axi_set_state", "axi_step", "axi_iter", "axi_read", "axi_sync", "axi_time_steps", "axi_monitor",
                "axi_monitor_sections", "axi_set_fused", "axi_autotune", "axi_sections", "axi_set_display_zones", "axi_derived",
-               "axi_transfer"]
+               "axi_transfer", "axi_field_create", "axi_field_destroy", "axi_field_last_error", "axi_field_sample",
+               "axi_initial_state"]
 AXI_DERIVED_KEYS = ["T", "Tt", "P", "Pt", "V", "M", "rho", "Vx", "Vth", "Vr", "Y", "vort", "ds"]
 
 
@@ -611,6 +612,12 @@ def alib():
         L.axi_set_display_zones.argtypes = [vp, C.c_void_p, C.c_void_p]
         L.axi_derived.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
         L.axi_transfer.argtypes = [vp, vp, C.c_double, C.c_double]
+        L.axi_field_create.argtypes = [f64p, C.c_int, C.POINTER(vp)]
+        L.axi_field_destroy.argtypes = [vp]
+        L.axi_field_last_error.argtypes = [vp]
+        L.axi_field_last_error.restype = C.c_char_p
+        L.axi_field_sample.argtypes = [vp, f64p, f64p, f64p, C.c_int, f64p]
+        L.axi_initial_state.argtypes = [vp, C.c_int, C.c_int, f64p, f64p, u8p, C.c_void_p, C.c_int, f64p]
         _alib = L
     return _alib
 
@@ -757,5 +764,118 @@ class AxiHIP:
     def transfer_from(self, src, T0, P0):
         """Grid sequencing (axi_transfer): src's state carried onto this solver's grid."""
         self._chk(alib().axi_transfer(self.h, src.h, T0, P0))
+
+# ---------------------------------------------------------------------------
+# Engine field model (FieldModel, src/field.js:255-488) and initialState
+# (src/cfd_grid.js:279-305) on the device.
+
+FIELD_SAMPLE_KEYS = ["region", "T", "P", "Tt", "Pt", "V", "Vx", "Vth", "M", "a", "rho", "g", "cp", "R"]
+_STREAM_NAME = {"inlet": 0, "bypass": 1, "core": 2, "ab": 3}
+_ENGINE_TYPE = {"tj": 0, "hbtf": 1, "lbtf": 2}
+FM_HEAD = 240
+
+
+def field_table(d):
+    """The flat table of axi_field_create (layout: AXI_FM_* in axi.h) from a field
+    model description: the FieldModel constructor's products and the geometry it
+    samples, as written by the model's owner (JSON: streams, kn, bodies, c, fl,
+    comb, mixer, ab, lip, outer, outerEnd, cold, exits, Uface, maxdT, hotGas)."""
+    T = [0.0] * FM_HEAD
+    nan = float("nan")
+    num = lambda v: nan if v is None else float(v)
+
+    def put(vals):
+        off = len(T)
+        T.extend(float(v) for v in vals)
+        return off
+
+    def poly(pts):
+        return put([c for p in pts for c in p]), len(pts)
+
+    order = d["order"]
+    T[0], T[1], T[2], T[3] = 1.0, _ENGINE_TYPE[d["type"]], d["s"], len(order)
+    for i, k in enumerate(order):
+        st = d["streams"][k]
+        lo, hi = poly(st["lo"]), poly(st["hi"])
+        kn = put([c for q in d["kn"][k] for c in q])
+        T[4 + 10 * i: 14 + 10 * i] = [_STREAM_NAME[k], st["x0"], st["x1"], lo[0], lo[1], hi[0], hi[1], kn, len(d["kn"][k]), 0]
+    T[44] = len(d["bodies"])
+    for b, body in enumerate(d["bodies"]):
+        xs, rs = [p[0] for p in body["poly"]], [p[1] for p in body["poly"]]
+        off, n = poly(body["poly"])
+        T[45 + 6 * b: 51 + 6 * b] = [min(xs), max(xs), min(rs), max(rs), off, n]
+    cb = d["comb"]
+    li, lo = poly(cb["linerIn"]), poly(cb["linerOut"])
+    T[141:152] = [order.index(cb["stream"]), cb["x0"], cb["xd"], cb["x1"], cb["turbEnd"], cb["n"], 1.0 if cb["cannular"] else 0.0,
+                  li[0], li[1], lo[0], lo[1]]
+    mx = d.get("mixer")
+    T[152:156] = [1.0, mx["x"], mx["coreFrac"], mx["lobes"]] if mx else [0.0] * 4
+    ab = d.get("ab")
+    if ab:
+        rg = put(ab["rings"])
+        T[156:162] = [1.0, order.index(ab["stream"]), ab["xFH"], ab["x1"], rg, len(ab["rings"])]
+    c = d["c"]
+    T[162:169] = [num(c["Tt3"]), num(c["Tt4"]), num(c["Tt5"]), num(c["Tt6A"]), num(c["Tt7"]), num(c["Tt13"]), 1.0 if c["abOn"] else 0.0]
+    fl = d["fl"]
+    T[169:174] = [fl["V0"], fl["T0"], fl["P0"], fl["Tt0"], fl["Pt0"]]
+    ol = poly(d["outer"])
+    T[174:180] = [d["lip"]["x"], d["lip"]["r"], d["lip"]["rOut"], ol[0], ol[1], d["outerEnd"]]
+    T[180:182] = [d["cold"]["g"], d["cold"]["cp"]]
+    T[182] = len(d["exits"])
+    for q, e in enumerate(d["exits"]):
+        T[183 + 12 * q: 195 + 12 * q] = [e["x"], e["R"], e["spread"], e["Qm"], e["Qh"], 1.0 if e["under"] else 0.0, e["Ls"],
+                                         e["P"], e["Pt"], e["g"], e["cp"], e["dTt"]]
+    hg = d.get("hotGas")
+    T[231:236] = [d["Uface"], d["maxdT"], 1.0 if hg else 0.0, hg["g"] if hg else 0.0, hg["cp"] if hg else 0.0]
+    return np.array(T, np.float64)
+
+
+class FieldModelHIP:
+    """The engine field model on the device (axi_field_*): sample() is
+    FieldModel.prototype.sample at many points, initial_state() the solver's
+    initialState. `desc` is a field model description (field_table)."""
+
+    def __init__(self, desc):
+        self.table = field_table(desc)
+        self.order = list(desc["order"])
+        h = C.c_void_p()
+        rc = alib().axi_field_create(self.table, int(self.table.size), C.byref(h))
+        if rc != 0:
+            raise Aero3DError(rc, alib().axi_field_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise Aero3DError(rc, alib().axi_field_last_error(self.h).decode())
+
+    def dispose(self):
+        if getattr(self, "h", None):
+            alib().axi_field_destroy(self.h)
+            self.h = None
+
+    __del__ = dispose
+
+    def sample(self, x, r, th=None):
+        """(n, 14) in FIELD_SAMPLE_KEYS order."""
+        x = np.ascontiguousarray(x, np.float64)
+        r = np.ascontiguousarray(r, np.float64)
+        th = np.zeros_like(x) if th is None else np.ascontiguousarray(th, np.float64)
+        assert x.shape == r.shape == th.shape and x.ndim == 1
+        out = np.empty(14 * x.size)
+        self._chk(alib().axi_field_sample(self.h, x, r, th, int(x.size), out))
+        return out.reshape(-1, 14)
+
+    def initial_state(self, xf, rf, solid, stream, uniform=False):
+        """initialState on the grid of faces xf, rf: (5, W*H) float64."""
+        xf = np.ascontiguousarray(xf, np.float64)
+        rf = np.ascontiguousarray(rf, np.float64)
+        W, H = xf.size - 1, rf.size - 1
+        solid = np.ascontiguousarray(solid, np.uint8)
+        stream = np.ascontiguousarray(stream, np.int8)
+        assert solid.shape == (W * H,) and stream.shape == (W * H,)
+        Q = np.empty(5 * W * H)
+        self._chk(alib().axi_initial_state(self.h, W, H, xf, rf, solid, stream.ctypes.data, int(bool(uniform)), Q))
+        return Q.reshape(5, W * H)
+
 
 EXPORTS = EXPORTS + AXI_EXPORTS

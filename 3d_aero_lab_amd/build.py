@@ -34,7 +34,7 @@ def _run(cmd):
 
 def build_hip():
     os.makedirs(LIB, exist_ok=True)
-    src = [os.path.join(CSRC, f) for f in ("aero3d.hip", "aero3d_batch.hip", "axi.hip")]
+    src = [os.path.join(CSRC, f) for f in ("aero3d.hip", "aero3d_batch.hip", "axi.hip", "field.hip")]
     tmp = LIBAERO + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, *src])
     os.replace(tmp, LIBAERO)

@@ -70,8 +70,9 @@ bool typed(napi_env env, napi_value v, napi_typedarray_type want, T **data, size
 // solver and its device memory.
 struct Rec3 { aero3d_solver *s; size_t N; bool busy = false; };
 struct RecAxi { axi_solver *s; size_t N; int W, H; };
+struct RecField { axi_field *s; };
 std::mutex g_mu;
-std::unordered_set<const void *> g_rec3, g_recaxi;
+std::unordered_set<const void *> g_rec3, g_recaxi, g_recfield;
 
 void fin3(napi_env, void *d, void *) {
   Rec3 *r = static_cast<Rec3 *>(d);
@@ -85,11 +86,18 @@ void finaxi(napi_env, void *d, void *) {
   if (r->s) axi_destroy(r->s);
   delete r;
 }
+void finfield(napi_env, void *d, void *) {
+  RecField *r = static_cast<RecField *>(d);
+  { std::lock_guard<std::mutex> l(g_mu); g_recfield.erase(r); }
+  if (r->s) axi_field_destroy(r->s);
+  delete r;
+}
 
 // A handle with asynchronous work in flight may not be used (the library's
 // handles are single-threaded); the axisymmetric solver has no async calls.
 template <class R> bool busy_of(const R *r) { return r->busy; }
 template <> bool busy_of<RecAxi>(const RecAxi *) { return false; }
+template <> bool busy_of<RecField>(const RecField *) { return false; }
 
 template <class R>
 R *record(napi_env env, napi_value v, const std::unordered_set<const void *> &live, const char *what) {
@@ -1174,6 +1182,76 @@ napi_value AxiSync(napi_env env, napi_callback_info info) {
   return rc ? axi_throw(env, s, rc) : undef(env);
 }
 
+// ---- engine field model (axi_field_*, axi_initial_state)
+RecField *recfield(napi_env env, napi_value v) { return record<RecField>(env, v, g_recfield, "a field model handle"); }
+napi_value field_throw(napi_env env, axi_field *f, int rc) {
+  std::string msg = "libaero3d field error " + std::to_string(rc) + ": " + axi_field_last_error(f);
+  napi_throw_error(env, nullptr, msg.c_str());
+  return nullptr;
+}
+// fieldCreate(table Float64Array) -> external handle
+napi_value FieldCreate(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!args(env, info, 1, a)) return nullptr;
+  double *t; size_t n;
+  if (!typed(env, a[0], napi_float64_array, &t, &n)) return nullptr;
+  if (n > (1u << 26)) { napi_throw_range_error(env, nullptr, "fieldCreate: table too long"); return nullptr; }
+  axi_field *f = nullptr;
+  int rc = axi_field_create(t, (int)n, &f);
+  if (rc) return field_throw(env, nullptr, rc);
+  RecField *r = new RecField{f};
+  { std::lock_guard<std::mutex> l(g_mu); g_recfield.insert(r); }
+  napi_value ext;
+  if (napi_create_external(env, r, finfield, nullptr, &ext) != napi_ok) {
+    finfield(env, r, nullptr);
+    napi_throw_error(env, nullptr, "aero3d addon: Node-API call failed");
+    return nullptr;
+  }
+  return ext;
+}
+napi_value FieldDispose(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!args(env, info, 1, a)) return nullptr;
+  RecField *r = recfield(env, a[0]);
+  if (!r) return nullptr;
+  axi_field_destroy(r->s);
+  r->s = nullptr;
+  return undef(env);
+}
+// fieldSample(h, x, r, th, out Float64Array(14 n))
+napi_value FieldSample(napi_env env, napi_callback_info info) {
+  napi_value a[5];
+  if (!args(env, info, 5, a)) return nullptr;
+  RecField *r = recfield(env, a[0]);
+  double *x, *rr, *th, *o; size_t nx, nr, nt, no;
+  if (!r || !typed(env, a[1], napi_float64_array, &x, &nx) || !typed(env, a[2], napi_float64_array, &rr, &nr) ||
+      !typed(env, a[3], napi_float64_array, &th, &nt) || !typed(env, a[4], napi_float64_array, &o, &no) ||
+      !check_len(env, nr, nx, "fieldSample r") || !check_len(env, nt, nx, "fieldSample th") || !check_len(env, no, 14 * nx, "fieldSample out"))
+    return nullptr;
+  if (nx > (1u << 26)) { napi_throw_range_error(env, nullptr, "fieldSample: at most 2^26 points"); return nullptr; }
+  int rc = axi_field_sample(r->s, x, rr, th, (int)nx, o);
+  return rc ? field_throw(env, r->s, rc) : undef(env);
+}
+// initialState(h, xf, rf, solid Uint8Array(N), stream Int8Array(N), uniform, Q Float64Array(5 N))
+napi_value InitialState(napi_env env, napi_callback_info info) {
+  napi_value a[7];
+  if (!args(env, info, 7, a)) return nullptr;
+  RecField *r = recfield(env, a[0]);
+  double *xf, *rf, *Q; uint8_t *solid; int8_t *stream; size_t nx, nr, ns, nt, nq;
+  if (!r || !typed(env, a[1], napi_float64_array, &xf, &nx) || !typed(env, a[2], napi_float64_array, &rf, &nr) ||
+      !typed(env, a[3], napi_uint8_array, &solid, &ns) || !typed(env, a[4], napi_int8_array, &stream, &nt) ||
+      !typed(env, a[6], napi_float64_array, &Q, &nq))
+    return nullptr;
+  if (nx < 2 || nr < 2 || nx > (1u << 20) || nr > (1u << 20)) { napi_throw_range_error(env, nullptr, "initialState: sizes"); return nullptr; }
+  const size_t N = (nx - 1) * (nr - 1);
+  if (!check_len(env, ns, N, "initialState solid") || !check_len(env, nt, N, "initialState stream") || !check_len(env, nq, 5 * N, "initialState Q"))
+    return nullptr;
+  bool uni = false;
+  napi_get_value_bool(env, a[5], &uni);
+  int rc = axi_initial_state(r->s, (int)nx - 1, (int)nr - 1, xf, rf, solid, stream, uni ? 1 : 0, Q);
+  return rc ? field_throw(env, r->s, rc) : undef(env);
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct { const char *name; napi_callback cb; } fns[] = {
       {"create", Create}, {"dispose", Dispose}, {"setFree", SetFree}, {"setFix", SetFix}, {"setOpt", SetOpt},
@@ -1191,7 +1269,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"axiCreate", AxiCreate}, {"axiDispose", AxiDispose}, {"axiSetOpt", AxiSetOpt}, {"axiSetComps", AxiSetComps},
       {"axiSetZones", AxiSetZones}, {"axiSetFs", AxiSetFs}, {"axiSetState", AxiSetState}, {"axiStep", AxiStep},
       {"axiRead", AxiRead}, {"axiSync", AxiSync}, {"axiSetFused", AxiSetFused}, {"axiAutotune", AxiAutotune}, {"axiMonitorSections", AxiMonitorSections}, {"axiMonitor", AxiMonitor},
-      {"axiSections", AxiSections}, {"axiSetDisplayZones", AxiSetDisplayZones}, {"axiDerived", AxiDerived}, {"axiTransfer", AxiTransfer}};
+      {"axiSections", AxiSections}, {"axiSetDisplayZones", AxiSetDisplayZones}, {"axiDerived", AxiDerived}, {"axiTransfer", AxiTransfer},
+      {"fieldCreate", FieldCreate}, {"fieldDispose", FieldDispose}, {"fieldSample", FieldSample}, {"initialState", InitialState}};
   for (auto &f : fns) {
     napi_value fn;
     NAPI_OK(napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));

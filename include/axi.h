@@ -127,6 +127,61 @@ int axi_derived(axi_solver *s, int disp, double T0, double P0, float *fields, ui
  * rounded to FP32 once. Both solvers on one device; waits for src's stream. */
 int axi_transfer(axi_solver *dst, const axi_solver *src, double T0, double P0);
 
+/* ---------------------------------------------------------------------------
+ * Engine field model (FieldModel, src/field.js:255-488) and the solver's
+ * initial state (initialState, src/cfd_grid.js:279-305) on the device.
+ *
+ * The model is given as its tables: what the FieldModel constructor derives
+ * from the engine cycle (stream knots after assignMach, the processed exits,
+ * the inlet face velocity, the hot gas) and the engine geometry it samples
+ * (stream walls, body polygons, combustor, mixer, afterburner, lip and outer
+ * line). One flat array of doubles: a header of AXI_FM_HEAD entries at the
+ * indices below, then a data area that the header's offsets index (counts
+ * and offsets are integer-valued doubles, undefined cycle values are NaN).
+ * -------------------------------------------------------------------------*/
+enum {
+  AXI_FM_VERSION = 0,   /* 1 */
+  AXI_FM_TYPE = 1,      /* 0 turbojet, 1 high-bypass turbofan, 2 low-bypass turbofan */
+  AXI_FM_S = 2,         /* engine scale s */
+  AXI_FM_NSTREAM = 3,   /* streams in the model's classification order, 1..4 */
+  AXI_FM_STREAM = 4,    /* 4 x 10: name (0 inlet, 1 bypass, 2 core, 3 ab), x0, x1, lower wall (offset,
+                           points), upper wall (offset, points), knots (offset, count), 0; a wall is
+                           (x, r) pairs, a knot 10 doubles: x, Tt, Pt, M, swirl deg, g, cp, R, mdot, noz */
+  AXI_FM_NBODY = 44,    /* solid polygons, 0..16 */
+  AXI_FM_BODY = 45,     /* 16 x 6: x min, x max, r min, r max, polygon (offset, points) */
+  AXI_FM_COMB = 141,    /* stream index, x0, xd, x1, turbEnd, n, cannular, inner liner (offset, points),
+                           outer liner (offset, points) */
+  AXI_FM_MIXER = 152,   /* present, x, coreFrac, lobes */
+  AXI_FM_AB = 156,      /* present, stream index, xFH, x1, rings (offset, count) */
+  AXI_FM_CYCLE = 162,   /* Tt3, Tt4, Tt5, Tt6A, Tt7, Tt13, abOn */
+  AXI_FM_FL = 169,      /* freestream V0, T0, P0, Tt0, Pt0 */
+  AXI_FM_LIP = 174,     /* lip x, r, rOut, outer line (offset, points), outerEnd */
+  AXI_FM_COLD = 180,    /* cold gas g, cp */
+  AXI_FM_NEXIT = 182,   /* exits, 0..4, in the model's order (largest radius first) */
+  AXI_FM_EXIT = 183,    /* 4 x 12: x, R, spread, Qm, Qh, under, Ls, P, Pt, g, cp, dTt */
+  AXI_FM_HOT = 231,     /* Uface, maxdT, hot gas present, hot gas g, cp */
+  AXI_FM_HEAD = 240
+};
+typedef struct axi_field axi_field;
+/* new FieldModel(...): the table is checked (version, counts, every offset and
+ * extent inside the array) and kept on the device. */
+int axi_field_create(const double *tab, int ntab, axi_field **out);
+int axi_field_destroy(axi_field *f);
+const char *axi_field_last_error(const axi_field *f);
+/* FieldModel.prototype.sample (src/field.js:318-488) at n points (x, r, th):
+ * out[14 p + c] = region (0 outside the engine, 1 solid, 2 + stream index),
+ * then T, P, Tt, Pt, V, Vx, Vth, M, a, rho, g, cp, R (NaN for a solid point).
+ * FP64, the reference's operations in its order; pow / exp / sin / cos / tan /
+ * log within an ulp of a JS engine's. Waits. */
+int axi_field_sample(axi_field *f, const double *x, const double *r, const double *th, int n, double *out);
+/* initialState(grid, zones, spec, model, fl, uniform) (src/cfd_grid.js:279-305)
+ * on the grid with faces xf (W + 1) and rf (H + 1) and the zones' solid (u8)
+ * and stream (i8, index into the model's stream order, -1 outside): Q = 5
+ * planes of N = W*H doubles [rho, rho u, 0, E, rho Y]. W, H >= 1 and
+ * W*H <= 2^28. Waits. */
+int axi_initial_state(axi_field *f, int W, int H, const double *xf, const double *rf, const uint8_t *solid,
+                      const int8_t *stream, int uniform, double *Q);
+
 #ifdef __cplusplus
 }
 #endif

@@ -96,6 +96,35 @@ function caseSpecs() {
   console.log('wrote engine_specs.json');
 }
 
+/* CFD3.applyCond (src/air_cfd.js:521-538) of the stock aircraft at a cruise
+ * point, run as the reference's own method over a stand-in solver that keeps
+ * what setFree / setFix receive. The engine's operating point and field model
+ * are the application's (src/app_air.js:351-354: EngineDeck.at at full
+ * throttle, FieldModel of the aircraft's engine spec); the model is written
+ * as data (field_desc.js) beside the freestream and the three fixed states. */
+function caseApplyCond() {
+  const { fieldDesc } = require('./field_desc.js');
+  const w = new Writer('applycond');
+  const out = {};
+  for (const [type, h, M, alpha] of [['airliner', 11000, 0.78, 2], ['fighter', 6000, 0.85, 4], ['interceptor', 9000, 0.9, 3]]) {
+    const eng = engineFor(type), P = eng.P;
+    const ac = buildAircraft(type, {}, eng);
+    const deck = new EngineDeck(P.type, P.design, P.eff, { Tt7: P.Tt7, nozzleType: P.nozzleType });
+    const op = deck.at(h, M, 1, false);
+    const fm = new FieldModel(ac.spec, { op: op.op, fl: flightState(h, M), ref: deck.ref }, deck.p(h, M, op.op.thr || 1, false));
+    const fs = fsState(h, M);
+    let F = null, fix = null;
+    globalThis.CUR = { R: { fs, alpha, fm }, ac };
+    globalThis.S = { M };
+    CFD3.solver = { setFree(x) { F = Array.from(x); }, setFix(x) { fix = x.map(a => Array.from(a)); } };
+    CFD3.applyCond(true);
+    out[type] = { h, M, alpha, fs: { P: fs.P, T: fs.T, a: fs.a }, F, fix, model: fieldDesc(ac.spec, fm, GAS.cold) };
+    console.log(type, JSON.stringify(fix));
+  }
+  w.set('cases', out);
+  w.close();
+}
+
 /* A duck-typed aircraft whose only solid is the sphere: no engines, lifting
  * surfaces too thin to be volumes and outside the domain (no baffle rows). */
 function sphereAsAircraft(body) {
@@ -312,5 +341,5 @@ const want = process.argv.slice(2);
 const all = { specs: caseSpecs, sphere: caseSphere, transport1m: () => caseTransport('transport1m', 1.0e6, 0, [], false),
   transport60k: () => caseTransport('transport60k', 60e3, 100, [2, 100], true), fighter30k: caseFighter, interceptor: caseInterceptor,
   sampling: caseSampling, readback_sphere: caseReadbackSphere, readback_fighter: caseReadbackFighter,
-  readback_transport: caseReadbackTransport };
+  readback_transport: caseReadbackTransport, applycond: caseApplyCond };
 for (const k of (want.length ? want : Object.keys(all))) { const t0 = Date.now(); all[k](); console.log(`${k}: ${((Date.now() - t0) / 1000).toFixed(1)} s`); }

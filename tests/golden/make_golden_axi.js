@@ -9,7 +9,8 @@
  *   node tests/golden/make_golden_axi.js [case ...]
  * cases: xcoarse medium longrun (TJ), hbtf_xcoarse hbtf_medium lbtf_xcoarse,
  * longrun_hbtf longrun_lbtf (the README.md:71-75 validation rows), setups,
- * interp, tj_coarse, checkpoint, options
+ * interp, tj_coarse, checkpoint, options, field (the field model: samples and
+ * initial states)
  *
  * The reference's own harness (tools/cfdlib.js:6-20) uses `??`, which this
  * Node (v12) cannot parse; the same setup sequence is written out here:
@@ -28,9 +29,10 @@ const crypto = require('crypto');
 const REF = process.env.AERO_REF || '/root/reference';
 const code = ['physics', 'geometry', 'field', 'cfd_grid', 'cfd_cpu', 'cfd_post']
   .map(f => fs.readFileSync(path.join(REF, 'src', f + '.js'), 'utf8').replace(/^if \(typeof module[^\n]*\n/gm, '')).join('\n');
-vm.runInThisContext(code + '\nglobalThis.AXI={cfdDerived,cfdControl,solveEngine,ENGINE_TYPES,DEFAULT_EFF,buildSpec,FieldModel,buildCFDGrid,buildZones,compParams,initialState,CPUSolver,cfdForces,cfdSection,spoolRatios,RGAS};');
+vm.runInThisContext(code + '\nglobalThis.AXI={cfdDerived,cfdControl,solveEngine,ENGINE_TYPES,DEFAULT_EFF,buildSpec,FieldModel,buildCFDGrid,buildZones,compParams,initialState,CPUSolver,cfdForces,cfdSection,spoolRatios,RGAS,GAS};');
 const A = globalThis.AXI;
 const OUT = __dirname;
+const { fieldDesc } = require('./field_desc.js');
 
 function setup(type, level) {
   const T = A.ENGINE_TYPES[type];
@@ -45,7 +47,7 @@ function setup(type, level) {
   const fsv = { r: fl.P0 / (A.RGAS * fl.T0), u: fl.V0, p: fl.P0 };
   const hf = P.eff.etb * P.eff.LHV;
   const S = new A.CPUSolver(grid, zones, comps, fsv, Q0, { cfl: 0.6, order: 2, visc: true, hf });
-  return { P, sol, op, fl, spec, grid, zones, comps, base: Float64Array.from(comps), Q0, fsv, hf, S, type };
+  return { P, sol, op, fl, spec, grid, zones, comps, base: Float64Array.from(comps), Q0, fsv, hf, S, type, m: model };
 }
 
 function sha(buf) { return crypto.createHash('sha256').update(buf).digest('hex'); }
@@ -306,6 +308,63 @@ function caseOptions() {
   w.close();
 }
 
+/* The engine field model as data and its outputs: the tables that the
+ * FieldModel constructor derives from the 1D cycle (src/field.js:255-294:
+ * stream knots after assignMach, processed exits, inlet face velocity, hot
+ * gas) with the geometry it samples (stream walls, bodies, combustor, mixer,
+ * afterburner, lip, outer line), FieldModel.sample (:318-488) at seeded
+ * points, and initialState (src/cfd_grid.js:279-305) on all twelve setups. */
+function fieldModelOf(type, ab) {
+  const T = A.ENGINE_TYPES[type];
+  const P = { type, design: Object.assign({}, T.design), eff: Object.assign({}, A.DEFAULT_EFF), alt: 0, M0: 0.25, thr: 1, ab: !!ab, Tt7: type === 'lbtf' ? 2100 : 2000, nozzleType: T.nozzle };
+  const sol = A.solveEngine(P), nz = sol.op.noz;
+  const spec = A.buildSpec(type, 1, { r8: Math.sqrt(nz.Athroat / Math.PI), r9: Math.sqrt(Math.max(nz.A, nz.Athroat) / Math.PI), A9: sol.ref.noz && sol.ref.noz.A, A19: sol.ref.nozB && sol.ref.nozB.A });
+  return { spec, sol, P, m: new A.FieldModel(spec, sol, P) };
+}
+function mulberry32(a) {
+  return function () { a |= 0; a = a + 0x6D2B79F5 | 0; let t = Math.imul(a ^ a >>> 15, 1 | a); t = t + Math.imul(t ^ t >>> 7, 61 | t) ^ t; return ((t ^ t >>> 14) >>> 0) / 4294967296; };
+}
+const SAMPLE_KEYS = ['T', 'P', 'Tt', 'Pt', 'V', 'Vx', 'Vth', 'M', 'a', 'rho', 'g', 'cp', 'R'];
+function caseField() {
+  const w = new Writer('axi_field');
+  const models = {};
+  for (const [name, type, ab] of [['tj', 'tj', false], ['hbtf', 'hbtf', false], ['lbtf', 'lbtf', false], ['tj_ab', 'tj', true], ['lbtf_ab', 'lbtf', true]]) {
+    const F = fieldModelOf(type, ab), sp = F.spec, m = F.m, d = fieldDesc(sp, m, A.GAS.cold);
+    models[name] = d;
+    // seeded points: a third over the engine, a third inside the streams (every
+    // wall distance), a third over the whole domain (jets, plume, shock cells)
+    const rnd = mulberry32(20261205 + name.length * 7 + (ab ? 1 : 0)), n = 3000;
+    const X = new Float64Array(n), R = new Float64Array(n), TH = new Float64Array(n), out = new Float64Array(n * 14);
+    const xb = sp.domain.xb, rb = sp.domain.rb, len = sp.view.len, Rm = sp.view.Rmax;
+    for (let p = 0; p < n; p++) {
+      const kind = p % 3;
+      let x, r;
+      if (kind === 0) { x = -0.3 * sp.s + rnd() * (len + 0.6 * sp.s); r = rnd() * 1.15 * Rm; }
+      else if (kind === 1) { const st = sp.streams[sp.order[Math.floor(rnd() * sp.order.length)]]; x = st.x0 + rnd() * (st.x1 - st.x0); const lo = st.lo.at(x), hi = st.hi.at(x); r = lo + rnd() * (hi - lo); }
+      else { x = xb[0] + rnd() * (xb[xb.length - 1] - xb[0]); r = rnd() * rb[rb.length - 1]; }
+      const th = rnd() < 0.5 ? 0 : rnd() * 2 * Math.PI;
+      X[p] = x; R[p] = r; TH[p] = th;
+      const o = m.sample(x, r, th, {});
+      out[p * 14] = o.solid ? 1 : o.region === 'ext' ? 0 : 2 + sp.order.indexOf(o.stream);
+      for (let q = 0; q < 13; q++) out[p * 14 + 1 + q] = o.solid ? NaN : o[SAMPLE_KEYS[q]];
+    }
+    w.arr(`${name}_x`, X); w.arr(`${name}_r`, R); w.arr(`${name}_th`, TH); w.arr(`${name}_out`, out);
+  }
+  w.set('models', models);
+  // initialState on the twelve setups (the model at the setup's operating point)
+  const q0 = {};
+  for (const type of ['tj', 'hbtf', 'lbtf']) for (const level of ['xcoarse', 'coarse', 'medium', 'fine']) {
+    const C = setup(type, level), N = C.grid.N, stride = 17, ns = Math.ceil(N / stride);
+    const smp = new Float64Array(5 * ns);
+    for (let q = 0; q < 5; q++) for (let k = 0; k < ns; k++) smp[q * ns + k] = C.Q0[q][k * stride];
+    w.arr(`q0s_${type}_${level}`, smp);
+    q0[`${type}_${level}`] = { stride, sha256: C.Q0.map(a => sha(Buffer.from(a.buffer))), sha256_f32: C.Q0.map(a => sha(Buffer.from(Float32Array.from(a).buffer))),
+      uniform_sha256: A.initialState(C.grid, C.zones, C.spec, C.m, C.fl, true).map(a => sha(Buffer.from(a.buffer))) };
+  }
+  w.set('initial_state', q0);
+  w.close();
+}
+
 const want = process.argv.slice(2);
 const all = {
   xcoarse: caseXcoarse, medium: caseMedium,
@@ -315,6 +374,6 @@ const all = {
   tj_coarse: () => caseSetupOnly('tj', 'coarse'), checkpoint: caseCheckpoint,
   longrun_hbtf: () => caseLongrun('hbtf', 'medium', 14000, 'axi_hbtf_medium_longrun'),
   longrun_lbtf: () => caseLongrun('lbtf', 'xcoarse', 10000, 'axi_lbtf_xcoarse_longrun'),
-  options: caseOptions,
+  options: caseOptions, field: caseField,
 };
 for (const k of (want.length ? want : ['xcoarse', 'medium'])) { const t0 = Date.now(); all[k](); console.log(`${k}: ${((Date.now() - t0) / 1000).toFixed(1)} s`); }

@@ -146,23 +146,31 @@ def test_axi_graph_replay_equals_direct_launches():
     ("axi_tj_medium", "axi_tj_medium_longrun", 14000, 56.9e3, "arrays"),       # README.md:73
     ("axi_hbtf_medium", "axi_hbtf_medium_longrun", 14000, 89.2e3, "arrays"),   # README.md:74
     ("axi_lbtf_xcoarse", "axi_lbtf_xcoarse_longrun", 10000, 68.8e3, "arrays"),  # README.md:75
-    ("axi_tj_medium", "axi_tj_medium_longrun", 14000, 56.9e3, "spec"),         # setup built by js/cfd_grid.js
+    ("axi_tj_medium", "axi_tj_medium_longrun", 14000, 56.9e3, "spec"),         # setup built by js/cfd_grid.js and the device field model
 ])
-def test_axi_longrun_matches_reference_history(setup, longrun, iters, published, src):
+def test_axi_longrun_matches_reference_history(setup, longrun, iters, published, src, tmp_path):
     """The reference's long runs (README.md:69-75, tools/longrun.js:2-14:
     exit-temperature control every 250 iterations) on the HIP solver: thrust
     and control gain at every 2000-iteration checkpoint against the
     reference's own run of the same setup (tests/golden/<longrun>), and the
     final thrust against the published validation number. src "spec": grid,
     zones and constants built from the engine description by js/cfd_grid.js
-    instead of taken from the reference's setup arrays. Bar 2e-4 relative
+    instead of taken from the reference's setup arrays, and the initial state
+    computed on the device from the engine's field model (axi_initial_state).
+    Bar 2e-4 relative
     on the history (FP32 device arithmetic against JS doubles over a transient
     with thrust swinging by tens of percent), 5e-3 on the published number
     (quoted to 0.1 kN)."""
     ref = json.load(open(os.path.join(ROOT, "tests", "golden", longrun, "manifest.json")))["scalars"]["hist"]
     say("long run", setup, iters, "iterations")
+    extra = []
+    if src == "spec":
+        model = tmp_path / "field_model.json"
+        gm = json.load(open(os.path.join(ROOT, "tests", "golden", setup, "manifest.json")))["scalars"]["type"]
+        model.write_text(json.dumps(json.load(open(os.path.join(ROOT, "tests", "golden", "axi_field", "manifest.json")))["scalars"]["models"][gm]))
+        extra = [f"model={model}"]
     r = subprocess.run(["node", os.path.join(ROOT, "3d_aero_lab_amd", "js", "axi_run.js"),
-                        os.path.join(ROOT, "tests", "golden", setup), str(iters), "2000", src],
+                        os.path.join(ROOT, "tests", "golden", setup), str(iters), "2000", src] + extra,
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     out = json.loads(r.stdout.strip().splitlines()[-1])
