@@ -5,7 +5,8 @@
  *        reference's own outputs: initialState on all twelve setups built from their
  *        engine descriptions (cfd_grid.js) at the reference's stride samples, the
  *        fixed states of CFD3.applyCond for the stock aircraft, FieldModel.sample at
- *        the seeded points through FieldHIP.sample / sampleMany. */
+ *        the seeded points through FieldHIP.sample / sampleMany.
+ *   node tests/js/field_check.js fix <aircraft>   -> {F, fix} of fixedStates on the device */
 const fs = require('fs'), path = require('path'), zlib = require('zlib'), crypto = require('crypto');
 const ROOT = path.join(__dirname, '..', '..');
 const FM = require(path.join(ROOT, '3d_aero_lab_amd', 'js', 'field_model.js'));
@@ -18,7 +19,12 @@ const rel = (a, b) => (a === b || (a !== a && b !== b) ? 0 : Math.abs(a - b) / M
 const F = manifest('axi_field'), AC = manifest('applycond');
 const mode = process.argv[2];
 const out = {};
-if (mode === 'tables') {
+if (mode === 'fix') {
+  const c = AC.scalars.cases[process.argv[3]];
+  const fm = new FM.FieldHIP(c.model);
+  Object.assign(out, FM.fixedStates(fm, c.fs, c.M, c.alpha));
+  fm.dispose();
+} else if (mode === 'tables') {
   for (const [k, d] of Object.entries(F.scalars.models)) out[k] = sha(FM.fieldTable(d));
   for (const [k, c] of Object.entries(AC.scalars.cases)) out['applycond_' + k] = sha(FM.fieldTable(c.model));
 } else {

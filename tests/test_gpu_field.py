@@ -110,3 +110,32 @@ def test_field_errors_are_reported():
     ok = fm.sample(np.array([1.0]), np.array([0.2]))
     assert ok.shape == (1, 14) and np.isfinite(ok[0, 0])
     fm.dispose()
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_transport_with_engine_fixed_states_matches_oracle():
+    """CFD3.applyCond end to end: the airliner's freestream and its inlet-sink
+    and jet fixed states from the engine field model on the device (Node,
+    fixedStates), set on the 60k-cell transport's solver, 20 iterations on the
+    GPU against the oracle with the same states (bar 1e-5 per cell)."""
+    import aero3d
+    import oracle3d
+    from parity import rel_diff
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "field_check.js"), "fix", "airliner"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    d = json.loads(r.stdout)
+    F, fix = d["F"], d["fix"]
+    ref = Golden("applycond").s["cases"]["airliner"]
+    assert np.allclose(np.array(fix), np.array(ref["fix"]), rtol=1e-13, atol=0) and np.allclose(F, ref["F"], rtol=1e-15)
+    assert len({tuple(f) for f in fix}) == 3          # three distinct engine states, none the freestream
+    g = Golden("transport60k")
+    assert (g.geo()["t"] >= 3).sum() > 0               # the grid has fixed-state cells
+    s = aero3d.CfdHIP(g.grid(), g.geo())
+    s.opt["cfl"] = 0.5
+    s.set_free(F); s.set_fix(fix); s.reset(); s.step(20)
+    o = oracle3d.Oracle3D(g.grid(), g.geo())
+    o.set_free(F); o.set_fix(fix); o.set_opt(0.5, 2); o.reset(); o.step(20)
+    dd = rel_diff(s.read(), o.read(), g.geo()["t"])
+    say("transport60k, engine fixed states, 20 iterations: max rel diff", dd)
+    assert max(dd.values()) < 1e-5, dd
+    s.dispose()
