@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libaero3d.so")
+# AERO3D_LIB names another build of the library (default: the in-tree one)
+LIB_PATH = os.environ.get("AERO3D_LIB") or os.path.join(_HERE, "lib", "libaero3d.so")
 
 f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")

@@ -131,9 +131,17 @@ __device__ __forceinline__ void hllc_fast(const float *L, const float *R, float 
 // Face flux of axis d between left cell (PL, TL, slope sL) and right cell
 // (PR, TR, slope sR) with aperture a; zero when the face is closed or touches
 // no fluid cell (no cell would use it).
-template <int d>
-__device__ __forceinline__ void face_flux(const float *PL, const float *sL, int TL, const float *PR, const float *sR, int TR,
-                                          float a, float *F) {
+// kPin (operands read from LDS): every operand is pinned in a register before
+// the face is tested, so all of its LDS reads are issued together and waited
+// for once; otherwise the reads are sunk below each test, one LDS round trip
+// per test.
+template <int d, bool kPin = false>
+__device__ __forceinline__ void face_flux(float *PL, float *sL, int TL, float *PR, float *sR, int TR, float a, float *F) {
+  if constexpr (kPin) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) { keep_vgpr(PL[v]); keep_vgpr(sL[v]); keep_vgpr(PR[v]); keep_vgpr(sR[v]); }
+    keep_vgpr(TL); keep_vgpr(TR); keep_vgpr(a);
+  }
   if (!(a > 0.f) || (TL != 0 && TR != 0)) {
 #pragma unroll
     for (int v = 0; v < 5; v++) F[v] = 0.f;
@@ -157,6 +165,14 @@ __device__ __forceinline__ void face_flux(const float *PL, const float *sL, int 
 __device__ __forceinline__ void slope5(const float *Pm, const float *P0, const float *Pp, bool ok, float *s) {
 #pragma unroll
   for (int v = 0; v < 5; v++) s[v] = ok ? mmod3(P0[v] - Pm[v], Pp[v] - P0[v]) : 0.f;
+}
+
+// The slope test of a cell from LDS: flags of the cell and its two neighbours
+// and the apertures of its two faces. All five are read before any is tested
+// (a short-circuit test would read them one LDS round trip at a time).
+__device__ __forceinline__ bool slope_ok(bool o2, int tm, int t0, int tp, float am, float ap) {
+  keep_vgpr(tm); keep_vgpr(t0); keep_vgpr(tp); keep_vgpr(am); keep_vgpr(ap);
+  return o2 & (tm == 0) & (t0 == 0) & (tp == 0) & (am == 1.f) & (ap == 1.f);
 }
 
 template <int TX, int TY>
@@ -222,24 +238,6 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
   const float *__restrict__ ap0 = cv.ap, *__restrict__ ap1 = cv.ap + cv.N, *__restrict__ ap2 = cv.ap + 2 * (size_t)cv.N,
                             *__restrict__ ap3 = cv.ap + 3 * (size_t)cv.N;
   auto apz = [&](int k) -> float { return (k < 0 || k >= NZ || !own) ? 1.f : gld(ap2, (unsigned)(k * NXY) + qc); };
-
-  // ---- z queue preamble: planes k0-2 .. k0+1
-  float Pz[4][5];          // primitives of planes k-1, k, k+1, k+2 (own column)
-  int Tz[4];
-  float Pm2[5];
-  // columns outside the box (partial tiles) carry freestream ghosts
-  const int Tm2 = load_prim(cv, kc, qin, ns, own ? i : -1, j, k0 - 2, Pm2);
-#pragma unroll
-  for (int m = 0; m < 3; m++) Tz[m] = load_prim(cv, kc, qin, ns, own ? i : -1, j, k0 - 1 + m, Pz[m]);
-  float Am1 = apz(k0 - 2), A0z = apz(k0 - 1), A1z = apz(k0);   // faces k0-3/2, k0-1/2, k0+1/2
-  float sz0[5], sz1[5], Fzm[5];
-  slope5(Pm2, Pz[0], Pz[1], o2 && Tm2 == 0 && Tz[0] == 0 && Tz[1] == 0 && Am1 == 1.f && A0z == 1.f, sz0);   // s(k0-1)
-  slope5(Pz[0], Pz[1], Pz[2], o2 && Tz[0] == 0 && Tz[1] == 0 && Tz[2] == 0 && A0z == 1.f && A1z == 1.f, sz1); // s(k0)
-  face_flux<2>(Pz[0], sz0, Tz[0], Pz[1], sz1, Tz[1], A0z, Fzm);                                             // F(k0-1/2)
-#pragma unroll
-  for (int v = 0; v < 5; v++) sz0[v] = sz1[v];                                                              // sz0 = s(k)
-  // queue: Pz[0] = k-1, Pz[1] = k, Pz[2] = k+1; Pz[3] loaded each plane
-  float sUpPrev = (own && k0 > 0) ? gld(ap3, (unsigned)((k0 - 1) * NXY) + qc) : 0.f;
 
   // halo ring (no corners: 4 rows of TX, 4 columns of TY), NHP cells per
   // thread; each cell's raw state is loaded one plane ahead (ha / ht) so the
@@ -309,6 +307,55 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
     }
   };
   ap_load(k0);
+  // operands of the cell update (conserved input and base state, phi, baffle
+  // slope, dz), loaded one plane ahead so that their memory round trip
+  // overlaps a whole plane instead of one barrier
+  constexpr bool kS1 = kMode == 1;   // stage 1: the base state is the input state (one read)
+  float uQ[5] = {}, uB[5] = {}, uPh = 1.f, uSu = 0.f, uDz = 1.f;
+  auto upd_load = [&](int kk) {
+    if (own) {
+      const unsigned qq = (unsigned)(kk * NXY) + qc;
+#pragma unroll
+      for (int c = 0; c < 5; c++) uQ[c] = gld(qin + c * ns, qq);
+      if (!kS1 && !(kMode == 0 && qin == bse)) {
+#pragma unroll
+        for (int c = 0; c < 5; c++) uB[c] = gld(bse + c * ns, qq);
+      }
+      uPh = gld(cv.phi, qq); uSu = gld(ap3, qq); uDz = cv.dz[kk];
+    }
+  };
+  upd_load(k0);
+
+  // ---- z queue preamble: planes k0-2 .. k0+1 (after the first plane's halo,
+  // aperture and update loads are issued, so all go out in one round trip)
+  float Pz[4][5];          // primitives of planes k-1, k, k+1, k+2 (own column)
+  int Tz[4];
+  float Pm2[5];
+  // columns outside the box (partial tiles) carry freestream ghosts
+  const int Tm2 = load_prim(cv, kc, qin, ns, own ? i : -1, j, k0 - 2, Pm2);
+#pragma unroll
+  for (int m = 0; m < 3; m++) Tz[m] = load_prim(cv, kc, qin, ns, own ? i : -1, j, k0 - 1 + m, Pz[m]);
+  float Am1 = apz(k0 - 2), A0z = apz(k0 - 1), A1z = apz(k0);   // faces k0-3/2, k0-1/2, k0+1/2
+  float sz0[5], sz1[5], Fzm[5];
+  slope5(Pm2, Pz[0], Pz[1], o2 && Tm2 == 0 && Tz[0] == 0 && Tz[1] == 0 && Am1 == 1.f && A0z == 1.f, sz0);   // s(k0-1)
+  slope5(Pz[0], Pz[1], Pz[2], o2 && Tz[0] == 0 && Tz[1] == 0 && Tz[2] == 0 && A0z == 1.f && A1z == 1.f, sz1); // s(k0)
+  face_flux<2>(Pz[0], sz0, Tz[0], Pz[1], sz1, Tz[1], A0z, Fzm);                                             // F(k0-1/2)
+#pragma unroll
+  for (int v = 0; v < 5; v++) sz0[v] = sz1[v];                                                              // sz0 = s(k)
+  // queue: Pz[0] = k-1, Pz[1] = k, Pz[2] = k+1; Pz[3] from the raw state of
+  // plane k+2 (zr / zt), loaded at the end of the plane before
+  float sUpPrev = (own && k0 > 0) ? gld(ap3, (unsigned)((k0 - 1) * NXY) + qc) : 0.f;
+  float zr[5];
+  int zt;
+  bool zin;
+  auto zraw_load = [&](int kk) {
+    zin = own && kk < NZ;
+    const unsigned q = zin ? (unsigned)(kk * NXY) + qc : 0u;
+    zt = gld(cv.t, q);
+#pragma unroll
+    for (int v = 0; v < 5; v++) zr[v] = gld(qin + v * ns, q);
+  };
+  zraw_load(k0 + 2);
 
   for (int k = k0; k < k1; k++) {
     if constexpr (kPrio) progress_prio(k - k0, k1 - k0);
@@ -316,7 +363,13 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
 #pragma unroll
     for (int p = 0; p < NHP; p++) { keep_vgpr(hoffv[p]); keep_vgpr(hinv[p]); keep_vgpr(hmirv[p]); }
     // ---------------- load phase
-    Tz[3] = load_prim(cv, kc, qin, ns, own ? i : -1, j, k + 2, Pz[3]);
+    {   // plane k+2 of the own column (never a mirror row): load_prim on the raw state loaded a plane earlier
+      float w[5];
+      prim_fast(zr[0], zr[1], zr[2], zr[3], zr[4], w);
+#pragma unroll
+      for (int v = 0; v < 5; v++) Pz[3][v] = zin ? w[v] : kc.Fw[v];
+      Tz[3] = zin ? zt : -1;
+    }
     const float A2z = apz(k + 1);                   // face k+3/2
     // tile interior of plane k from the queue
 #pragma unroll
@@ -353,13 +406,14 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       float P0[5], Pm[5], Pp[5], s[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) { P0[v] = sm.P[v][yj][xi]; Pm[v] = sm.P[v][yj][xi - 1]; Pp[v] = sm.P[v][yj][xi + 1]; }
-      bool ok = o2 && sm.T[yj][xi] == 0 && sm.T[yj][xi - 1] == 0 && sm.T[yj][xi + 1] == 0 && sm.A0[ty][tx + 1] == 1.f && sm.A0[ty][tx + 2] == 1.f;
+      const int t0 = sm.T[yj][xi];
+      bool ok = slope_ok(o2, sm.T[yj][xi - 1], t0, sm.T[yj][xi + 1], sm.A0[ty][tx + 1], sm.A0[ty][tx + 2]);
       slope5(Pm, P0, Pp, ok, s);
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Sx[v][ty][tx + 1] = s[v];
 #pragma unroll
       for (int v = 0; v < 5; v++) { Pm[v] = sm.P[v][yj - 1][xi]; Pp[v] = sm.P[v][yj + 1][xi]; }
-      ok = o2 && sm.T[yj][xi] == 0 && sm.T[yj - 1][xi] == 0 && sm.T[yj + 1][xi] == 0 && sm.A1[ty + 1][tx] == 1.f && sm.A1[ty + 2][tx] == 1.f;
+      ok = slope_ok(o2, sm.T[yj - 1][xi], t0, sm.T[yj + 1][xi], sm.A1[ty + 1][tx], sm.A1[ty + 2][tx]);
       slope5(Pm, P0, Pp, ok, s);
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Sy[v][ty + 1][tx] = s[v];
@@ -376,7 +430,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         const float am = sm.A0[r][cx + 1];
         const float apf = (cx == -1) ? sm.A0[r][cx + 2] : aSk;
         (void)gi; (void)gj;
-        const bool ok = o2 && sm.T[yj][xi] == 0 && sm.T[yj][xi - 1] == 0 && sm.T[yj][xi + 1] == 0 && am == 1.f && apf == 1.f;
+        const bool ok = slope_ok(o2, sm.T[yj][xi - 1], sm.T[yj][xi], sm.T[yj][xi + 1], am, apf);
         slope5(Pm, P0, Pp, ok, s);
 #pragma unroll
         for (int v = 0; v < 5; v++) sm.Sx[v][r][cx + 1] = s[v];
@@ -388,7 +442,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         const float am = sm.A1[ry + 1][c];
         const float apf = (ry == -1) ? sm.A1[ry + 2][c] : aSk;
         (void)gi; (void)gj;
-        const bool ok = o2 && sm.T[yj][xi] == 0 && sm.T[yj - 1][xi] == 0 && sm.T[yj + 1][xi] == 0 && am == 1.f && apf == 1.f;
+        const bool ok = slope_ok(o2, sm.T[yj - 1][xi], sm.T[yj][xi], sm.T[yj + 1][xi], am, apf);
         slope5(Pm, P0, Pp, ok, s);
 #pragma unroll
         for (int v = 0; v < 5; v++) sm.Sy[v][ry + 1][c] = s[v];
@@ -405,12 +459,12 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       float PL[5], PR[5], sL[5], sR[5], F[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) { PL[v] = sm.P[v][yj][xi]; PR[v] = sm.P[v][yj][xi + 1]; sL[v] = sm.Sx[v][ty][tx + 1]; sR[v] = sm.Sx[v][ty][tx + 2]; }
-      face_flux<0>(PL, sL, sm.T[yj][xi], PR, sR, sm.T[yj][xi + 1], sm.A0[ty][tx + 2], F);
+      face_flux<0, true>(PL, sL, sm.T[yj][xi], PR, sR, sm.T[yj][xi + 1], sm.A0[ty][tx + 2], F);
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Fx[v][ty][tx + 1] = F[v];
 #pragma unroll
       for (int v = 0; v < 5; v++) { PR[v] = sm.P[v][yj + 1][xi]; sL[v] = sm.Sy[v][ty + 1][tx]; sR[v] = sm.Sy[v][ty + 2][tx]; }
-      face_flux<1>(PL, sL, sm.T[yj][xi], PR, sR, sm.T[yj + 1][xi], sm.A1[ty + 2][tx], F);
+      face_flux<1, true>(PL, sL, sm.T[yj][xi], PR, sR, sm.T[yj + 1][xi], sm.A1[ty + 2][tx], F);
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Fy[v][ty + 1][tx] = F[v];
     }
@@ -419,7 +473,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       float PL[5], PR[5], sL[5], sR[5], F[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) { PL[v] = sm.P[v][1][xi]; PR[v] = sm.P[v][2][xi]; sL[v] = sm.Sy[v][0][c]; sR[v] = sm.Sy[v][1][c]; }
-      face_flux<1>(PL, sL, sm.T[1][xi], PR, sR, sm.T[2][xi], sm.A1[1][c], F);
+      face_flux<1, true>(PL, sL, sm.T[1][xi], PR, sR, sm.T[2][xi], sm.A1[1][c], F);
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Fy[v][0][c] = F[v];
     } else if (tidv >= TX && tidv < TX + TY) {           // x face left of column 0, row r
@@ -427,7 +481,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       float PL[5], PR[5], sL[5], sR[5], F[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) { PL[v] = sm.P[v][yj][1]; PR[v] = sm.P[v][yj][2]; sL[v] = sm.Sx[v][r][0]; sR[v] = sm.Sx[v][r][1]; }
-      face_flux<0>(PL, sL, sm.T[yj][1], PR, sR, sm.T[yj][2], sm.A0[r][1], F);
+      face_flux<0, true>(PL, sL, sm.T[yj][1], PR, sR, sm.T[yj][2], sm.A0[r][1], F);
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Fx[v][r][0] = F[v];
     }
@@ -435,20 +489,12 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
     face_flux<2>(Pz[1], sz0, Tz[1], Pz[2], sz1n, Tz[2], A1z, Fzp);
     // own apertures into registers (LDS apertures are rewritten by the next plane's load)
     const float ax_m = sm.A0[ty][tx + 1], ax_p = sm.A0[ty][tx + 2], ay_m = sm.A1[ty + 1][tx], ay_p = sm.A1[ty + 2][tx];
-    // the cell update's loads go out before the barrier (their latency overlaps it)
-    float Qc[5], Bq[5], phq = 1.f, sUpq = 0.f, dzq = 1.f;
-    if (own) {
+    // this plane's update operands (loaded one plane earlier), then the next plane's loads
+    float Qc[5], Bq[5];
+    const float phq = uPh, sUpq = uSu, dzq = uDz;
 #pragma unroll
-      for (int c = 0; c < 5; c++) Qc[c] = gld(qin + c * ns, qk);
-      if (kMode == 1 || (kMode == 0 && qin == bse)) {   // stage 1: the base state is the input state (one read)
-#pragma unroll
-        for (int c = 0; c < 5; c++) Bq[c] = Qc[c];
-      } else {
-#pragma unroll
-        for (int c = 0; c < 5; c++) Bq[c] = gld(bse + c * ns, qk);
-      }
-      phq = gld(cv.phi, qk); sUpq = gld(ap3, qk); dzq = cv.dz[k];
-    }
+    for (int c = 0; c < 5; c++) { Qc[c] = uQ[c]; Bq[c] = (kS1 || (kMode == 0 && qin == bse)) ? uQ[c] : uB[c]; }
+    if (k + 1 < k1) upd_load(k + 1);
     __syncthreads();
 
     // ---------------- cell update
@@ -518,6 +564,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
     }
     Tz[0] = Tz[1]; Tz[1] = Tz[2]; Tz[2] = Tz[3];
     A0z = A1z; A1z = A2z;
+    if (k + 1 < k1) zraw_load(k + 3);
     (void)Am1;
   }
 }
