@@ -56,10 +56,11 @@ __device__ __forceinline__ void keep_vgpr(T &x) {
 // at half the occupancy. With the priority falling as a block advances, the
 // block that is behind is issued first and both finish together.
 __device__ __forceinline__ void progress_prio(int done, int len) {
-  const int q = (4 * done) / len;
-  if (q == 0) __builtin_amdgcn_s_setprio(3);
-  else if (q == 1) __builtin_amdgcn_s_setprio(2);
-  else if (q == 2) __builtin_amdgcn_s_setprio(1);
+  // quarter q = floor(4 done / len), tested as 4 done < (q + 1) len (no division)
+  const int d4 = 4 * done;
+  if (d4 < len) __builtin_amdgcn_s_setprio(3);
+  else if (d4 < 2 * len) __builtin_amdgcn_s_setprio(2);
+  else if (d4 < 3 * len) __builtin_amdgcn_s_setprio(1);
   else __builtin_amdgcn_s_setprio(0);
 }
 
@@ -138,9 +139,12 @@ __device__ __forceinline__ void hllc_fast(const float *L, const float *R, float 
 template <int d, bool kPin = false>
 __device__ __forceinline__ void face_flux(float *PL, float *sL, int TL, float *PR, float *sR, int TR, float a, float *F) {
   if constexpr (kPin) {
-#pragma unroll
-    for (int v = 0; v < 5; v++) { keep_vgpr(PL[v]); keep_vgpr(sL[v]); keep_vgpr(PR[v]); keep_vgpr(sR[v]); }
-    keep_vgpr(TL); keep_vgpr(TR); keep_vgpr(a);
+#ifdef __AMDGCN__
+    // one statement, so the reads are waited for once (one wait per pinned value otherwise)
+    asm volatile("" : "+v"(PL[0]), "+v"(PL[1]), "+v"(PL[2]), "+v"(PL[3]), "+v"(PL[4]), "+v"(sL[0]), "+v"(sL[1]), "+v"(sL[2]),
+                 "+v"(sL[3]), "+v"(sL[4]), "+v"(PR[0]), "+v"(PR[1]), "+v"(PR[2]), "+v"(PR[3]), "+v"(PR[4]), "+v"(sR[0]),
+                 "+v"(sR[1]), "+v"(sR[2]), "+v"(sR[3]), "+v"(sR[4]), "+v"(TL), "+v"(TR), "+v"(a));
+#endif
   }
   if (!(a > 0.f) || (TL != 0 && TR != 0)) {
 #pragma unroll
@@ -171,7 +175,9 @@ __device__ __forceinline__ void slope5(const float *Pm, const float *P0, const f
 // and the apertures of its two faces. All five are read before any is tested
 // (a short-circuit test would read them one LDS round trip at a time).
 __device__ __forceinline__ bool slope_ok(bool o2, int tm, int t0, int tp, float am, float ap) {
-  keep_vgpr(tm); keep_vgpr(t0); keep_vgpr(tp); keep_vgpr(am); keep_vgpr(ap);
+#ifdef __AMDGCN__
+  asm volatile("" : "+v"(tm), "+v"(t0), "+v"(tp), "+v"(am), "+v"(ap));
+#endif
   return o2 & (tm == 0) & (t0 == 0) & (tp == 0) & (am == 1.f) & (ap == 1.f);
 }
 
