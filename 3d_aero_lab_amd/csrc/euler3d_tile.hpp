@@ -545,7 +545,12 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         }
         const float ph = phq, iph = frcp(ph);
         float Bw[5];
-        prim_fast(Bq[0], Bq[1], Bq[2], Bq[3], Bq[4], Bw);
+        if constexpr (kS1) {      // stage 1: the base state is the input state, whose primitives are C
+#pragma unroll
+          for (int c = 0; c < 5; c++) Bw[c] = C[c];
+        } else {
+          prim_fast(Bq[0], Bq[1], Bq[2], Bq[3], Bq[4], Bw);
+        }
         const float a = fsqrt(kGamma * Bw[4] * frcp(Bw[0]));
         const float dt = kc.cfl * fmaxf(ph, 0.25f) *
                          frcp((fabsf(Bw[1]) + a) * ihx + (fabsf(Bw[2]) + a) * ihy + (fabsf(Bw[3]) + a) * ihz);
