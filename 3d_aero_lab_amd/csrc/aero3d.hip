@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4)
   const int l = L / nt, t = L % nt;
   const int k0 = (int)(((long long)l * cv.NZ) / c), k1 = (int)(((long long)(l + 1) * cv.NZ) / c);
   const int bx = t % ntx, by = t / ntx;
-  tile_stage<64, TY, kPrio, kMode>(cv, kc, qin, bse, out, dtout, ns, wa, wb, bx * 64, by * TY, k0, k1, sm);
+  tile_stage<64, TY, kPrio, kMode, (TY >= 8)>(cv, kc, qin, bse, out, dtout, ns, wa, wb, bx * 64, by * TY, k0, k1, sm);
 }
 
 // Reset: every plane of buffer `dst` := F.

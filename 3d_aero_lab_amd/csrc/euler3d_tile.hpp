@@ -225,11 +225,13 @@ __device__ __forceinline__ int load_prim(const CaseView &cv, const StageConsts &
 // kPrio: set the wave priority from the chunk's progress (for launches whose
 // blocks are all resident at once; where blocks are dispatched over many
 // rounds, oldest-first issue finishes blocks sooner and frees their slots).
+// kEarlyY: the y face's LDS operands are read before the z face is computed
+// (hides their latency; costs the registers of 15 values across the z face).
 // kMode: 0 decides at run time (stage 1 when qin == bse, dt written when
 // dtout is non-null); 1 stage 1 (bse == qin), 2 stage 2 without dt, 3 stage 2
 // with dt. The fixed modes hold fewer base pointers in scalar registers
 // (fewer of them spilled to vector-register lanes and reloaded per plane).
-template <int TX, int TY, bool kPrio = false, int kMode = 0>
+template <int TX, int TY, bool kPrio = false, int kMode = 0, bool kEarlyY = false>
 __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts &kc, const float *__restrict__ qin,
                                            const float *__restrict__ bse, float *__restrict__ out, float *__restrict__ dtout,
                                            size_t ns, float wa, float wb, int i0, int j0, int k0, int k1,
@@ -459,7 +461,8 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
     slope5(Pz[1], Pz[2], Pz[3], o2 && Tz[1] == 0 && Tz[2] == 0 && Tz[3] == 0 && A1z == 1.f && A2z == 1.f, sz1n);
     __syncthreads();
 
-    // ---------------- fluxes: own + x, + y faces, tile-edge - faces, own + z face
+    // ---------------- fluxes: own + x, + z, + y faces, tile-edge - faces
+    float Fzp[5];
     {
       const int xi = tx + 2, yj = ty + 2;
       float PL[5], PR[5], sL[5], sR[5], F[5];
@@ -468,9 +471,19 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       face_flux<0, true>(PL, sL, sm.T[yj][xi], PR, sR, sm.T[yj][xi + 1], sm.A0[ty][tx + 2], F);
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Fx[v][ty][tx + 1] = F[v];
+      if constexpr (kEarlyY) {   // the y face's LDS reads go out before the z face (registers only) is computed
+        float PRy[5], sLy[5], sRy[5];
 #pragma unroll
-      for (int v = 0; v < 5; v++) { PR[v] = sm.P[v][yj + 1][xi]; sL[v] = sm.Sy[v][ty + 1][tx]; sR[v] = sm.Sy[v][ty + 2][tx]; }
-      face_flux<1, true>(PL, sL, sm.T[yj][xi], PR, sR, sm.T[yj + 1][xi], sm.A1[ty + 2][tx], F);
+        for (int v = 0; v < 5; v++) { PRy[v] = sm.P[v][yj + 1][xi]; sLy[v] = sm.Sy[v][ty + 1][tx]; sRy[v] = sm.Sy[v][ty + 2][tx]; }
+        const int Ty0 = sm.T[yj][xi], Ty1 = sm.T[yj + 1][xi];
+        const float Ay = sm.A1[ty + 2][tx];
+        face_flux<2>(Pz[1], sz0, Tz[1], Pz[2], sz1n, Tz[2], A1z, Fzp);
+        face_flux<1, true>(PL, sLy, Ty0, PRy, sRy, Ty1, Ay, F);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; v++) { PR[v] = sm.P[v][yj + 1][xi]; sL[v] = sm.Sy[v][ty + 1][tx]; sR[v] = sm.Sy[v][ty + 2][tx]; }
+        face_flux<1, true>(PL, sL, sm.T[yj][xi], PR, sR, sm.T[yj + 1][xi], sm.A1[ty + 2][tx], F);
+      }
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Fy[v][ty + 1][tx] = F[v];
     }
@@ -491,8 +504,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Fx[v][r][0] = F[v];
     }
-    float Fzp[5];
-    face_flux<2>(Pz[1], sz0, Tz[1], Pz[2], sz1n, Tz[2], A1z, Fzp);
+    if constexpr (!kEarlyY) face_flux<2>(Pz[1], sz0, Tz[1], Pz[2], sz1n, Tz[2], A1z, Fzp);
     // own apertures into registers (LDS apertures are rewritten by the next plane's load)
     const float ax_m = sm.A0[ty][tx + 1], ax_p = sm.A0[ty][tx + 2], ay_m = sm.A1[ty + 1][tx], ay_p = sm.A1[ty + 2][tx];
     // this plane's update operands (loaded one plane earlier), then the next plane's loads
