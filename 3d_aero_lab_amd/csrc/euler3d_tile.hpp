@@ -242,6 +242,8 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
   const bool own = i < NX && j < NY;
   const unsigned qc = own ? (unsigned)(j * NX + i) : 0u;   // column base index (plane 0)
   const bool o2 = kc.order == 2;
+  // RK weights out = wA base + wB (in + dt R): stage 1 (0, 1), stage 2 (1/2, 1/2); run-time mode takes the arguments
+  const float wA = kMode == 1 ? 0.f : (kMode >= 2 ? 0.5f : wa), wB = kMode == 1 ? 1.f : (kMode >= 2 ? 0.5f : wb);
   const float ihx = own ? frcp(cv.dx[i]) : 0.f, ihy = own ? frcp(cv.dy[j]) : 0.f;
   const float *__restrict__ ap0 = cv.ap, *__restrict__ ap1 = cv.ap + cv.N, *__restrict__ ap2 = cv.ap + 2 * (size_t)cv.N,
                             *__restrict__ ap3 = cv.ap + 3 * (size_t)cv.N;
@@ -528,24 +530,21 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         const float *C = Pz[1];
         const float pC = C[4];
         float R[5];
-        // x: + face, - face, wall pressure
+        // each face's flux weighted by aperture / width (weights formed once per cell), + faces
+        // subtracted, - faces added; then the wall pressure p_C (A+ - A-) / h on each axis
+        const float wxp = ax_p * ihx, wxm = ax_m * ihx, wyp = ay_p * ihy, wym = ay_m * ihy, wzp = A1z * ihz, wzm = A0z * ihz;
 #pragma unroll
-        for (int c = 0; c < 5; c++) R[c] = -(ax_p * sm.Fx[c][ty][tx + 1]) * ihx;
-#pragma unroll
-        for (int c = 0; c < 5; c++) R[c] += (ax_m * sm.Fx[c][ty][tx]) * ihx;
-        R[1] += pC * (ax_p - ax_m) * ihx;
-        // y
-#pragma unroll
-        for (int c = 0; c < 5; c++) R[c] -= (ay_p * sm.Fy[c][ty + 1][tx]) * ihy;
-#pragma unroll
-        for (int c = 0; c < 5; c++) R[c] += (ay_m * sm.Fy[c][ty][tx]) * ihy;
-        R[2] += pC * (ay_p - ay_m) * ihy;
-        // z
-#pragma unroll
-        for (int c = 0; c < 5; c++) R[c] -= (A1z * Fzp[c]) * ihz;
-#pragma unroll
-        for (int c = 0; c < 5; c++) R[c] += (A0z * Fzm[c]) * ihz;
-        R[3] += pC * (A1z - A0z) * ihz;
+        for (int c = 0; c < 5; c++) {
+          float r = -(sm.Fx[c][ty][tx + 1] * wxp);
+          r = fmaf(sm.Fx[c][ty][tx], wxm, r);
+          r = fmaf(-sm.Fy[c][ty + 1][tx], wyp, r);
+          r = fmaf(sm.Fy[c][ty][tx], wym, r);
+          r = fmaf(-Fzp[c], wzp, r);
+          R[c] = fmaf(Fzm[c], wzm, r);
+        }
+        R[1] = fmaf(pC, wxp - wxm, R[1]);
+        R[2] = fmaf(pC, wyp - wym, R[2]);
+        R[3] = fmaf(pC, wzp - wzm, R[3]);
         // baffle transpiration (src/air_cfd.js:226-233)
         const float sUp = sUpq, sDn = sUpPrev;
         if (sUp != 0.f || sDn != 0.f) {
@@ -555,7 +554,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
           const float mh = m * ihz;
           R[0] += mh; R[1] += mh * u; R[2] += mh * v; R[3] += mh * w; R[4] += mh * Hh;
         }
-        const float ph = phq, iph = frcp(ph);
+        const float ph = phq;
         float Bw[5];
         if constexpr (kS1) {      // stage 1: the base state is the input state, whose primitives are C
 #pragma unroll
@@ -563,13 +562,17 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         } else {
           prim_fast(Bq[0], Bq[1], Bq[2], Bq[3], Bq[4], Bw);
         }
-        const float a = fsqrt(kGamma * Bw[4] * frcp(Bw[0]));
-        const float dt = kc.cfl * fmaxf(ph, 0.25f) *
-                         frcp((fabsf(Bw[1]) + a) * ihx + (fabsf(Bw[2]) + a) * ihy + (fabsf(Bw[3]) + a) * ihz);
-        const float dtp = dt * iph;
+        const float gpB = kGamma * Bw[4];
+        const float a = gpB * frsq(gpB * Bw[0]);      // sqrt(g p / r) with one transcendental, as in the HLLC
+        const float den = (fabsf(Bw[1]) + a) * ihx + (fabsf(Bw[2]) + a) * ihy + (fabsf(Bw[3]) + a) * ihz;
+        const float cph = kc.cfl * fmaxf(ph, 0.25f);
+        // dt = cfl max(phi, 1/4) / den; the update takes dt / phi, one reciprocal where dt itself is not written
+        constexpr bool kNoDt = kMode == 1 || kMode == 2;
+        const float dt = kNoDt ? 0.f : cph * frcp(den);
+        const float dtp = kNoDt ? cph * frcp(den * ph) : dt * frcp(ph);
         float n[5];
 #pragma unroll
-        for (int c = 0; c < 5; c++) n[c] = wa * Bq[c] + wb * (Qc[c] + dtp * R[c]);
+        for (int c = 0; c < 5; c++) n[c] = wA * Bq[c] + wB * (Qc[c] + dtp * R[c]);
         n[0] = fmaxf(n[0], 1e-4f);
         const float ke = 0.5f * (n[1] * n[1] + n[2] * n[2] + n[3] * n[3]) * frcp(n[0]);
         if (kGm1 * (n[4] - ke) < 1e-5f) n[4] = 2.5e-5f + ke;
