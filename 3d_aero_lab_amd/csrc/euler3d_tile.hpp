@@ -566,10 +566,11 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         const float a = gpB * frsq(gpB * Bw[0]);      // sqrt(g p / r) with one transcendental, as in the HLLC
         const float den = (fabsf(Bw[1]) + a) * ihx + (fabsf(Bw[2]) + a) * ihy + (fabsf(Bw[3]) + a) * ihz;
         const float cph = kc.cfl * fmaxf(ph, 0.25f);
-        // dt = cfl max(phi, 1/4) / den; the update takes dt / phi, one reciprocal where dt itself is not written
+        // dt = cfl max(phi, 1/4) / den; the update takes dt / phi = cfl max(phi, 1/4) / (den phi) in every stage
+        // mode (the same bits whether or not dt is written out)
         constexpr bool kNoDt = kMode == 1 || kMode == 2;
         const float dt = kNoDt ? 0.f : cph * frcp(den);
-        const float dtp = kNoDt ? cph * frcp(den * ph) : dt * frcp(ph);
+        const float dtp = cph * frcp(den * ph);
         float n[5];
 #pragma unroll
         for (int c = 0; c < 5; c++) n[c] = wA * Bq[c] + wB * (Qc[c] + dtp * R[c]);
