@@ -159,7 +159,7 @@ def batched_leg(res_drop=1e-4, max_iters=2000, check_every=50):
     best = None
     for pers in (0, 1):
         tb.set_persistent(pers)
-        for zc in (3, 4, 6, 8):
+        for zc in (0, 3, 4, 6, 8):          # 0: by grid size
             for gc in (1_000_000, 2_000_000, 4_000_000, 1 << 40):
                 tb.set_zchunk(zc)
                 tb.set_group_cells(gc)
@@ -215,7 +215,7 @@ def serve_leg(n_requests=200, res_drop=1e-4, max_iters=2000, check_every=50):
     reqs = serve_workload()[:n_requests]
     print(f"[bench] serve: {len(reqs)} requests over {reqs[-1][3]:.1f} s of arrivals", file=sys.stderr, flush=True)
     built = geometry.build_cases(SERVE_SPECS)
-    done, wall, gpu, cu = run_server(built, reqs, res_drop, max_iters, check_every, 3, arena_bytes=64 << 30)
+    done, wall, gpu, cu = run_server(built, reqs, res_drop, max_iters, check_every, 0, arena_bytes=64 << 30)
     lat = sorted(d["t_done"] - d["arrival"] for d in done)
     pct = lambda p: lat[min(len(lat) - 1, int(round(p / 100 * (len(lat) - 1))))]
     # the same requests one at a time on the plain path (arrival times ignored: its best case)
@@ -225,7 +225,7 @@ def serve_leg(n_requests=200, res_drop=1e-4, max_iters=2000, check_every=50):
     it_s = {d["tag"]: d["iters"] for d in done}
     # capacity: the same requests all arriving at once (the stream above is arrival-bound at ~20 req/s)
     print("[bench] serve: capacity run, all requests at t = 0", file=sys.stderr, flush=True)
-    done_c, wall_c, gpu_c, cu_c = run_server(built, [(r[0], r[1], r[2], 0.0) for r in reqs], res_drop, max_iters, check_every, 3,
+    done_c, wall_c, gpu_c, cu_c = run_server(built, [(r[0], r[1], r[2], 0.0) for r in reqs], res_drop, max_iters, check_every, 0,
                                              arena_bytes=64 << 30)
     # capacity per grid level: the three bodies at one level, several requests each, all at once, against the plain path
     print("[bench] serve: capacity per grid level", file=sys.stderr, flush=True)
@@ -235,7 +235,7 @@ def serve_leg(n_requests=200, res_drop=1e-4, max_iters=2000, check_every=50):
         specs = [f"sphere:{cells}", f"transport:{cells}", f"interceptor:{cells}"]
         lvl_built = [built[SERVE_SPECS.index(sp)] for sp in specs] if all(sp in SERVE_SPECS for sp in specs) else geometry.build_cases(specs)
         lreq = [(b, 0.3 + 0.55 * rnd(), -2 + 10 * rnd(), 0.0) for b in range(3) for _ in range(per_body)]
-        dl, wl, gl, cul = run_server(lvl_built, lreq, res_drop, max_iters, check_every, 3, arena_bytes=64 << 30)
+        dl, wl, gl, cul = run_server(lvl_built, lreq, res_drop, max_iters, check_every, 0, arena_bytes=64 << 30)
         wp, ip, _ = run_converged_plain([lvl_built[r[0]] for r in lreq], [(r[1], r[2]) for r in lreq], res_drop, max_iters,
                                         check_every, (8, 0))
         it_l = {d["tag"]: d["iters"] for d in dl}

@@ -219,7 +219,12 @@ int aero3d_batch_case_bytes(const aero3d_batch *b, int gid, int64_t *bytes);
  * arena room is left. */
 int aero3d_batch_add(aero3d_batch *b, int gid, const double F[5], const double fix[15], double cfl, int order, int *slot);
 int aero3d_batch_remove(aero3d_batch *b, int slot);
-/* Stage-kernel z chunk of every case (tiles are 64 x 8 columns). */
+/* Stage-kernel z chunk of every case (tiles are 64 x 8 columns): zchunk > 0
+ * planes per work item; zchunk 0 (the default) from the load, each time the
+ * case table changes: the longest of 8, 6, 4 and 3 planes that gives at
+ * least four work items per resident block (long chunks for throughput when
+ * many cases are in flight, short ones for latency when few are). Negative
+ * values are rejected with AERO3D_E_ARG. */
 int aero3d_batch_set_zchunk(aero3d_batch *b, int zchunk);
 /* Cases are stepped in groups of about `cells` cells (default: all cases in
  * one group): a group's state and masks stay in the Infinity Cache across its
