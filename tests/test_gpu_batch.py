@@ -192,3 +192,27 @@ def test_server_requests_that_cannot_fit_do_not_stall():
     with pytest.raises(aero3d.Aero3DError, match="more than the arena"):
         srv.submit(gid2, F, [F] * 3, 0.5, 2, ref, 1e-4, 20, srv.clock(), 2)
     srv.dispose()
+
+
+def test_zchunk_policies_bit_identical():
+    """The z chunk only changes which block computes which planes: fixed chunks
+    of 1, 3 and 8 planes and the chunk chosen from the load (0, the default)
+    give the same bits as the plain path; a negative chunk is an argument
+    error."""
+    import aero3d
+    cs = cases()
+    ref = {name: plain(g, F, fix, cfl, 12) for name, g, F, fix, cfl in cs}
+    for zc in (0, 1, 3, 8):
+        b = aero3d.Batch(arena_bytes=1 << 30, max_slots=8)
+        gids = {name: b.add_geometry(g.grid(), g.geo()) for name, g, *_ in cs}
+        slots = {name: b.add(gids[name], F, fix, cfl) for name, g, F, fix, cfl in cs}
+        b.set_zchunk(zc)
+        b.step(7); b.step(5)
+        for name in slots:
+            Q = b.read(slots[name])
+            assert np.array_equal(Q.view(np.uint32), ref[name].view(np.uint32)), (zc, name)
+        if zc == 0:
+            with pytest.raises(aero3d.Aero3DError):
+                b.set_zchunk(-1)
+        say("zchunk", zc, "bit-identical to the plain path")
+        b.dispose()
