@@ -52,9 +52,9 @@ struct aero3d_solver {
   float *state = nullptr;     // 3 * 5 * ns
   float *dt = nullptr, *rsnap = nullptr;
   int cur = 0;                // buffer index holding the current state Q
-  int tile_ty = 8, tile_zc = 0; // stage kernel: ty 0 = per-cell kernel, else 64 x ty tiles; zc > 0 fixed z chunks,
+  int tile_ty = 8, tile_zc = 0; // stage kernel: ty 0 = per-cell kernel, 4 / 8: 64 x ty tiles, 32: 32 x 8 tiles; zc > 0 fixed z chunks,
                                 //   zc <= 0 balanced launch of (max(1, -zc)) x resident blocks
-  int resident4 = 0, resident8 = 0; // resident blocks of the balanced kernels on this device
+  int resident4 = 0, resident8 = 0, resident32 = 0; // resident blocks of the balanced kernels on this device
   int it = 0, snap_it = -1;
   double F[5] = {1, 0, 0, 0, 1 / (1.4 * 0.4)};
   double fix[3][5];
@@ -104,16 +104,16 @@ __global__ void __launch_bounds__(kBlock) stage_kernel(CaseView cv, StageConsts 
   stage_cell(cv, kc, qin, bse, out, dtout, ns, wa, wb, q);
 }
 
-template <int TY>
-__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4))) tile_kernel(CaseView cv, StageConsts kc, const float *__restrict__ qin,
+template <int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4))) tile_kernel(CaseView cv, StageConsts kc, const float *__restrict__ qin,
                                                       const float *__restrict__ bse, float *__restrict__ out,
                                                       float *__restrict__ dtout, size_t ns, float wa, float wb, int ntx, int nty,
                                                       int zc) {
-  __shared__ TileSmem<64, TY> sm;
+  __shared__ TileSmem<TX, TY> sm;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int bx = lb % ntx, by = (lb / ntx) % nty, bz = lb / (ntx * nty);
   const int k0 = bz * zc, k1 = min(cv.NZ, k0 + zc);
-  tile_stage<64, TY>(cv, kc, qin, bse, out, dtout, ns, wa, wb, bx * 64, by * TY, k0, k1, sm);
+  tile_stage<TX, TY>(cv, kc, qin, bse, out, dtout, ns, wa, wb, bx * TX, by * TY, k0, k1, sm);
 }
 
 // Balanced launch: every tile gets the same number c of z chunks (c = G / nt
@@ -124,17 +124,17 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4)
 // chunk below (whose top planes start this chunk's z queue) share an L2.
 // kPrio (launches of at most one resident round): wave priority from chunk
 // progress, so the two blocks of a CU finish together. kMode: as tile_stage.
-template <int TY, bool kPrio, int kMode>
-__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4))) tile_kernel_bal(CaseView cv, StageConsts kc, const float *__restrict__ qin,
+template <int TX, int TY, bool kPrio, int kMode>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4))) tile_kernel_bal(CaseView cv, StageConsts kc, const float *__restrict__ qin,
                                                           const float *__restrict__ bse, float *__restrict__ out,
                                                           float *__restrict__ dtout, size_t ns, float wa, float wb, int ntx, int nty) {
-  __shared__ TileSmem<64, TY> sm;
+  __shared__ TileSmem<TX, TY> sm;
   const int G = gridDim.x, nt = ntx * nty, c = G / nt;
   const int L = xcd_remap(blockIdx.x, G);
   const int l = L / nt, t = L % nt;
   const int k0 = (int)(((long long)l * cv.NZ) / c), k1 = (int)(((long long)(l + 1) * cv.NZ) / c);
   const int bx = t % ntx, by = t / ntx;
-  tile_stage<64, TY, kPrio, kMode, (TY >= 8)>(cv, kc, qin, bse, out, dtout, ns, wa, wb, bx * 64, by * TY, k0, k1, sm);
+  tile_stage<TX, TY, kPrio, kMode, (TY >= 8)>(cv, kc, qin, bse, out, dtout, ns, wa, wb, bx * TX, by * TY, k0, k1, sm);
 }
 
 // Reset: every plane of buffer `dst` := F.
@@ -203,10 +203,16 @@ int apply_fix(aero3d_solver *s) {
 // The balanced kernels by [one resident round][stage mode - 1]; stage mode 1 =
 // stage 1 (qin == bse, which never writes dt), 2 = stage 2, 3 = stage 2 with dt.
 using BalKernel = void (*)(CaseView, StageConsts, const float *, const float *, float *, float *, size_t, float, float, int, int);
-const BalKernel kBal4[2][3] = {{tile_kernel_bal<4, false, 1>, tile_kernel_bal<4, false, 2>, tile_kernel_bal<4, false, 3>},
-                               {tile_kernel_bal<4, true, 1>, tile_kernel_bal<4, true, 2>, tile_kernel_bal<4, true, 3>}};
-const BalKernel kBal8[2][3] = {{tile_kernel_bal<8, false, 1>, tile_kernel_bal<8, false, 2>, tile_kernel_bal<8, false, 3>},
-                               {tile_kernel_bal<8, true, 1>, tile_kernel_bal<8, true, 2>, tile_kernel_bal<8, true, 3>}};
+const BalKernel kBal4[2][3] = {{tile_kernel_bal<64, 4, false, 1>, tile_kernel_bal<64, 4, false, 2>, tile_kernel_bal<64, 4, false, 3>},
+                               {tile_kernel_bal<64, 4, true, 1>, tile_kernel_bal<64, 4, true, 2>, tile_kernel_bal<64, 4, true, 3>}};
+const BalKernel kBal8[2][3] = {{tile_kernel_bal<64, 8, false, 1>, tile_kernel_bal<64, 8, false, 2>, tile_kernel_bal<64, 8, false, 3>},
+                               {tile_kernel_bal<64, 8, true, 1>, tile_kernel_bal<64, 8, true, 2>, tile_kernel_bal<64, 8, true, 3>}};
+const BalKernel kBal32[2][3] = {{tile_kernel_bal<32, 8, false, 1>, tile_kernel_bal<32, 8, false, 2>, tile_kernel_bal<32, 8, false, 3>},
+                                {tile_kernel_bal<32, 8, true, 1>, tile_kernel_bal<32, 8, true, 2>, tile_kernel_bal<32, 8, true, 3>}};
+
+// Tile shape of a shape code: 4 and 8 are 64 x ty tiles, 32 is 32 x 8 tiles.
+int tile_tx(int code) { return code == 32 ? 32 : 64; }
+int tile_th(int code) { return code == 32 ? 8 : code; }
 
 void launch_stage(aero3d_solver *s, const CaseView &cv, const StageConsts &kc, const float *qin, const float *bse, float *out,
                   float *dtout, float wa, float wb) {
@@ -214,25 +220,29 @@ void launch_stage(aero3d_solver *s, const CaseView &cv, const StageConsts &kc, c
     stage_kernel<<<(s->N + kBlock - 1) / kBlock, kBlock, 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb);
     return;
   }
-  const int ntx = (s->NX + 63) / 64, nty = (s->NY + s->tile_ty - 1) / s->tile_ty;
+  const int TX = tile_tx(s->tile_ty), TH = tile_th(s->tile_ty);
+  const int ntx = (s->NX + TX - 1) / TX, nty = (s->NY + TH - 1) / TH;
   if (s->tile_zc <= 0) {            // balanced launch, G = multiple of the resident block count
     const int mult = s->tile_zc == 0 ? 1 : -s->tile_zc;
     const long long nt = (long long)ntx * nty;
-    const long long res = (long long)mult * (s->tile_ty == 4 ? s->resident4 : s->resident8);
+    const int resident = s->tile_ty == 4 ? s->resident4 : (s->tile_ty == 32 ? s->resident32 : s->resident8);
+    const long long res = (long long)mult * resident;
     const long long c = std::max(1LL, std::min((long long)s->NZ, res / nt));   // z chunks per tile
     const long long G = nt * c;
-    const bool one_round = G <= (s->tile_ty == 4 ? s->resident4 : s->resident8);
+    const bool one_round = G <= resident;
     const int mode = qin == bse ? 1 : (dtout ? 3 : 2);
-    const BalKernel k = s->tile_ty == 4 ? kBal4[one_round][mode - 1] : kBal8[one_round][mode - 1];
-    k<<<(int)G, dim3(64, s->tile_ty), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
+    const BalKernel k = s->tile_ty == 4 ? kBal4[one_round][mode - 1] : (s->tile_ty == 32 ? kBal32[one_round][mode - 1] : kBal8[one_round][mode - 1]);
+    k<<<(int)G, dim3(TX, TH), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty);
     return;
   }
   const int nzc = (s->NZ + s->tile_zc - 1) / s->tile_zc;
   const int nb = ntx * nty * nzc;
   if (s->tile_ty == 4)
-    tile_kernel<4><<<nb, dim3(64, 4), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty, s->tile_zc);
+    tile_kernel<64, 4><<<nb, dim3(64, 4), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty, s->tile_zc);
+  else if (s->tile_ty == 32)
+    tile_kernel<32, 8><<<nb, dim3(32, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty, s->tile_zc);
   else
-    tile_kernel<8><<<nb, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty, s->tile_zc);
+    tile_kernel<64, 8><<<nb, dim3(64, 8), 0, s->stream>>>(cv, kc, qin, bse, out, dtout, s->ns, wa, wb, ntx, nty, s->tile_zc);
 }
 
 int enqueue_steps(aero3d_solver *s, int n) {
@@ -363,13 +373,15 @@ int aero3d_create(const aero3d_grid *g, const uint8_t *t, const float *phi, cons
   if (hipGetDevice(&s->device) != hipSuccess) return bail(fail(s, AERO3D_E_NODEV, "hipGetDevice failed"));
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(s, AERO3D_E_NODEV, "hipStreamCreate failed"));
   {
-    int ncu = 0, o4 = 0, o8 = 0;
+    int ncu = 0, o4 = 0, o8 = 0, o32 = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || ncu <= 0)
       return bail(fail(s, AERO3D_E_NODEV, "hipDeviceGetAttribute(MultiprocessorCount) failed"));
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o4, tile_kernel_bal<4, true, 2>, 256, 0) != hipSuccess) o4 = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o8, tile_kernel_bal<8, true, 2>, 512, 0) != hipSuccess) o8 = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o4, tile_kernel_bal<64, 4, true, 2>, 256, 0) != hipSuccess) o4 = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o8, tile_kernel_bal<64, 8, true, 2>, 512, 0) != hipSuccess) o8 = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o32, tile_kernel_bal<32, 8, true, 2>, 256, 0) != hipSuccess) o32 = 1;
     s->resident4 = std::max(1, ncu * std::max(1, o4));
     s->resident8 = std::max(1, ncu * std::max(1, o8));
+    s->resident32 = std::max(1, ncu * std::max(1, o32));
   }
   // grid: FP32 widths and centres from FP64 faces, as the reference stores them
   std::vector<float> hdx(s->NX), hdy(s->NY), hdz(s->NZ), hcen;
@@ -679,8 +691,8 @@ int aero3d_time_steps(aero3d_solver *s, int n, float *ms) {
 
 int aero3d_set_tile(aero3d_solver *s, int ty, int zchunk) {
   if (!s) return AERO3D_E_ARG;
-  if (!(ty == 0 || ty == 4 || ty == 8) || zchunk < -8)
-    return fail(s, AERO3D_E_ARG, "set_tile: ty must be 0, 4 or 8 and zchunk >= -8");
+  if (!(ty == 0 || ty == 4 || ty == 8 || ty == 32) || zchunk < -8)
+    return fail(s, AERO3D_E_ARG, "set_tile: ty must be 0, 4, 8 or 32 and zchunk >= -8");
   s->tile_ty = ty;
   s->tile_zc = zchunk;
   return 0;
@@ -694,7 +706,7 @@ int aero3d_get_tile(const aero3d_solver *s, int *ty, int *zchunk) {
 int aero3d_autotune(aero3d_solver *s, int iters, int *ty, int *zchunk, float *ms_best) {
   if (!s || iters < 1) return AERO3D_E_ARG;
   // candidates: both tile heights, z chunks from short (many blocks) to long (little preamble)
-  const int tys[] = {4, 8};
+  const int tys[] = {4, 8, 32};
   const int zcs[] = {0, -2, -3, 2, 3, 4, 6, 8};
   // time on a scratch copy of the state: save the current state and the
   // local dt of the last iteration, restore both after

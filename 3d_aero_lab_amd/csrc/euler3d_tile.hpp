@@ -1,6 +1,6 @@
 // ============================================================================
 // euler3d_tile.hpp — tiled, face-once SSP-RK2 stage of the 3D Euler cut-cell
-// scheme for gfx950 (one wavefront = one 64-cell x row of a tile).
+// scheme for gfx950 (a wavefront = one 64-cell x row of a tile, or two 32-cell rows).
 //
 // Same arithmetic as the reference's per-cell residual (CfdCPU.cellResidual,
 // src/air_cfd.js:198-236) but every face flux is evaluated once and shared by
@@ -10,8 +10,12 @@
 // i+1/2 seen from cell i (sg = +1) and from cell i+1 (sg = -1) is the same
 // number (src/air_cfd.js:213-223).
 //
-// Work decomposition of one block (TX = 64 x TY threads):
+// Work decomposition of one block (TX x TY threads, TX = 64 or 32; a
+// wavefront covers 64 / TX rows of the tile):
 //   x-y tile of TX x TY columns, marching planes k0..k1-1 of a z chunk.
+//   Tile-edge work: the y faces below row 0 on threads 0..TX-1 (wave 0), the x
+//   faces left of column 0 on threads 64..64+TY-1 (wave 1), the tile-edge
+//   slopes on threads 128.. (waves 2, 3, ...).
 //   LDS per plane: primitives and flags of the tile plus a 2-cell halo ring,
 //   x / y apertures, x / y slopes, x / y face fluxes.
 //   Registers per column: primitives of planes k-1..k+2, z slopes of k and
@@ -289,7 +293,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
   // apertures written to LDS each plane (own x / y faces, the tile-edge face
   // of lanes 0..2TY-1 and 64..64+2TX-1) and the far-face aperture of this
   // lane's tile-edge slope, loaded one plane ahead
-  const int et = (tid + TX * TY - 2 * TX) % (TX * TY);
+  const int et = (tid + TX * TY - 128) % (TX * TY);     // edge slope e on thread 128 + e (waves 2, 3, ...)
   int etv = et;
   float aX = 1.f, aY = 1.f, aE = 1.f, aS = 1.f;
   auto ap_load = [&](int kk) {
@@ -497,8 +501,8 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       face_flux<1, true>(PL, sL, sm.T[1][xi], PR, sR, sm.T[2][xi], sm.A1[1][c], F);
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Fy[v][0][c] = F[v];
-    } else if (tidv >= TX && tidv < TX + TY) {           // x face left of column 0, row r
-      const int r = tidv - TX, yj = r + 2;
+    } else if (tidv >= 64 && tidv < 64 + TY) {           // x face left of column 0, row r (wave 1)
+      const int r = tidv - 64, yj = r + 2;
       float PL[5], PR[5], sL[5], sR[5], F[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) { PL[v] = sm.P[v][yj][1]; PR[v] = sm.P[v][yj][2]; sL[v] = sm.Sx[v][r][0]; sR[v] = sm.Sx[v][r][1]; }

@@ -177,14 +177,16 @@ int aero3d_time_steps(aero3d_solver *s, int n, float *ms);
 /* Stage-kernel shape (the tuning knob of the stage kernel, no reference
  * counterpart: the reference's shader has one fragment per cell,
  * src/air_cfd.js:405-413). ty = 4 or 8: tiles of 64 x ty columns marching z;
- * zchunk > 0: fixed chunks of zchunk planes, one block each; zchunk <= 0:
+ * ty = 32: tiles of 32 x 8 columns (a wavefront covers two 32-column rows),
+ * for grids whose x extent fills 32-column tiles better (same states bit for
+ * bit as the other tile shapes); zchunk > 0: fixed chunks of zchunk planes, one block each; zchunk <= 0:
  * balanced launch of max(1, -zchunk) x (resident blocks of the device), each
  * block marching an equal share of one tile's planes. ty = 0: one thread per
  * cell, each face flux evaluated by both of its cells as the reference does.
  * Default ty 8, zchunk 0. */
 int aero3d_set_tile(aero3d_solver *s, int ty, int zchunk);
 int aero3d_get_tile(const aero3d_solver *s, int *ty, int *zchunk);
-/* Autotuner of the stage-kernel shape: times every candidate (ty in {4, 8},
+/* Autotuner of the stage-kernel shape: times every candidate (ty in {4, 8, 32},
  * zchunk in {0, -2, -3, 2, 3, 4, 6, 8}) over `iters` iterations on this handle's grid, keeps
  * the fastest, and restores the state, the local dt that read() returns and
  * the iteration count. Outputs may be

@@ -239,9 +239,10 @@ def test_bad_option_is_an_error():
         s.step(1)
 
 
-@pytest.mark.parametrize("ty,zc", [(0, 1), (4, 3), (4, 75), (8, 1), (8, 8), (8, 80), (8, 0), (4, 0), (8, -3), (4, -8)])
+@pytest.mark.parametrize("ty,zc", [(0, 1), (4, 3), (4, 75), (8, 1), (8, 8), (8, 80), (8, 0), (4, 0), (8, -3), (4, -8),
+                                   (32, 0), (32, 3), (32, -2)])
 def test_kernel_shapes_agree(ty, zc):
-    """Every stage-kernel shape (per-cell kernel, tiles of 64 x 4 / 64 x 8 with
+    """Every stage-kernel shape (per-cell kernel, tiles of 64 x 4 / 64 x 8 / 32 x 8 with
     short to whole-domain z chunks, balanced launches) reproduces the oracle."""
     g = Golden("transport60k")
     F, fix = g.s["F"], g.s["fix"]
@@ -262,7 +263,7 @@ def test_autotune_keeps_state():
     Q = s.read(with_dt=True)
     ty, zc, ms = s.autotune(3)
     say("autotune sphere", ty, zc, ms)
-    assert ty in (4, 8) and zc >= -8 and ms > 0
+    assert ty in (4, 8, 32) and zc >= -8 and ms > 0
     assert s.it == 10 and np.array_equal(s.read(with_dt=True), Q)      # state and the local dt restored
     s.step(190)
     d = rel_diff(s.read(), g.state("it200"), g.geo()["t"])

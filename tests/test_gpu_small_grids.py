@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
 ITERS = 10
-SHAPES = [(8, 0), (4, 0), (8, 1), (4, 3), (8, -3), (0, 1)]
+SHAPES = [(8, 0), (4, 0), (8, 1), (4, 3), (8, -3), (0, 1), (32, 0), (32, 2)]   # ty 32: tiles of 32 x 8
 
 
 def say(*a):
@@ -83,11 +83,12 @@ def test_stage_modes_bit_identical():
     """The balanced kernels are compiled once per stage (stage 1, stage 2, the
     last stage 2 writing dt); the fixed-chunk tile kernel decides the stage at
     run time. The same arithmetic either way: after 20 iterations the states
-    and the local dt of both agree bit for bit, for 64 x 8 and 64 x 4 tiles."""
+    and the local dt of both agree bit for bit, for 64 x 8, 64 x 4 and 32 x 8
+    tiles (which also agree with each other)."""
     import aero3d
     grid, geo, F, fix = synth_case(174, 77, 40, seed=5)
     out = {}
-    for ty, zc in ((8, 0), (8, 8), (4, 0), (4, 3)):
+    for ty, zc in ((8, 0), (8, 8), (4, 0), (4, 3), (32, 0), (32, 5)):
         say("stage modes", ty, zc)
         s = aero3d.CfdHIP(grid, geo)
         s.opt["cfl"] = 0.5

@@ -20,7 +20,8 @@ def emul_lib(tmp_path_factory):
     return out
 
 
-@pytest.mark.parametrize("case,iters,ty,zc", [("sphere", 2, 4, 5), ("transport60k", 2, 8, 3), ("interceptor", 1, 8, 40)])
+@pytest.mark.parametrize("case,iters,ty,zc", [("sphere", 2, 4, 5), ("transport60k", 2, 8, 3), ("interceptor", 1, 8, 40),
+                                               ("sphere", 2, 32, 4)])
 def test_tile_kernel_cpu_build_matches_oracle(emul_lib, case, iters, ty, zc):
     asan = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True, text=True).stdout.strip()
     env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0")
@@ -36,7 +37,7 @@ def test_tile_kernel_cpu_build_matches_oracle(emul_lib, case, iters, ty, zc):
 # tests/synth_cases.py SIZES (single cells, sub-tile, just past a tile, tall and
 # thin), each with a z chunk shorter and one longer than the grid
 SYNTH = [("synth:1:2:1", 8, 3), ("synth:2:2:3", 4, 1), ("synth:3:5:2", 8, 1), ("synth:65:9:3", 8, 2),
-         ("synth:7:40:2", 4, 3), ("synth:200:2:1", 8, 3)]
+         ("synth:7:40:2", 4, 3), ("synth:200:2:1", 8, 3), ("synth:33:9:3", 32, 2), ("synth:1:2:1", 32, 3)]
 
 
 @pytest.mark.parametrize("case,ty,zc", SYNTH)
