@@ -336,7 +336,7 @@ class Result(C.Structure):
 
 BATCH_EXPORTS = ["aero3d_batch_create", "aero3d_batch_destroy", "aero3d_batch_last_error", "aero3d_batch_add_geometry",
                  "aero3d_batch_remove_geometry", "aero3d_batch_case_bytes", "aero3d_batch_add", "aero3d_batch_remove",
-                 "aero3d_batch_set_zchunk", "aero3d_batch_set_group_cells", "aero3d_batch_set_persistent", "aero3d_batch_step", "aero3d_batch_sync", "aero3d_batch_iter", "aero3d_batch_read",
+                 "aero3d_batch_set_zchunk", "aero3d_batch_set_tile_width", "aero3d_batch_set_group_cells", "aero3d_batch_set_persistent", "aero3d_batch_step", "aero3d_batch_sync", "aero3d_batch_iter", "aero3d_batch_read",
                  "aero3d_batch_residuals", "aero3d_batch_forces", "aero3d_batch_footprint", "aero3d_batch_time_steps",
                  "aero3d_server_create", "aero3d_server_destroy", "aero3d_server_batch", "aero3d_server_clock",
                  "aero3d_server_submit", "aero3d_server_tick", "aero3d_server_poll", "aero3d_server_stats"]
@@ -361,6 +361,7 @@ def blib():
         L.aero3d_batch_add.argtypes = [vp, ci, f64p, f64p, cd, ci, pi]
         L.aero3d_batch_remove.argtypes = [vp, ci]
         L.aero3d_batch_set_zchunk.argtypes = [vp, ci]
+        L.aero3d_batch_set_tile_width.argtypes = [vp, ci]
         L.aero3d_batch_set_group_cells.argtypes = [vp, C.c_int64]
         L.aero3d_batch_set_persistent.argtypes = [vp, C.c_int]
         L.aero3d_batch_step.argtypes = [vp, ci]
@@ -449,6 +450,10 @@ class Batch:
 
     def set_zchunk(self, zc):
         self._chk(blib().aero3d_batch_set_zchunk(self.h, zc))
+
+    def set_tile_width(self, width):
+        """64 or 32 columns for every case, 0 (default): per case by its x extent."""
+        self._chk(blib().aero3d_batch_set_tile_width(self.h, width))
 
     def set_persistent(self, on):
         self._chk(blib().aero3d_batch_set_persistent(self.h, int(bool(on))))

@@ -366,6 +366,9 @@ int aero3d_create(const aero3d_grid *g, const uint8_t *t, const float *phi, cons
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, AERO3D_E_NODEV, "aero3d_create: no HIP device");
   aero3d_solver *s = new aero3d_solver();
   s->NX = g->NX; s->NY = g->NY; s->NZ = g->NZ; s->N = g->NX * g->NY * g->NZ;
+  // default stage-kernel shape: 32 x 8 tiles where 32-column tiles cover NX with
+  // fewer padded columns than 64-column ones (the same bits either way), else 64 x 8
+  s->tile_ty = (g->NX + 31) / 32 * 32 < (g->NX + 63) / 64 * 64 ? 32 : 8;
   s->ns = ((size_t)s->N + 63) / 64 * 64;
   for (int k = 0; k < 3; k++) std::memcpy(s->fix[k], s->F, sizeof s->F);
   auto bail = [&](int rc) { g_create_error = s->err; aero3d_destroy(s); return rc; };

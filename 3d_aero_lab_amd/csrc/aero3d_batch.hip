@@ -81,10 +81,12 @@ struct SlotStat {
 // phase: iterations since the table's buffer indices were written (mod 3);
 // stage: 0 or 1 of SSP-RK2. A frozen case's items return at once. At least
 // four waves per SIMD (<= 128 VGPRs).
-template <int stage, int persist>
-__global__ void __launch_bounds__(64 * kTY) __attribute__((amdgpu_waves_per_eu(4)))
+// TX: tile width of every case of the group (64, or 32 for cases whose x
+// extent fills 32-column tiles better).
+template <int stage, int persist, int TX>
+__global__ void __launch_bounds__(TX * kTY) __attribute__((amdgpu_waves_per_eu(4)))
 batch_kernel(const TableHdr *__restrict__ hdr, const SlotDev *__restrict__ table, const SlotStat *__restrict__ stat, int phase) {
-  __shared__ TileSmem<64, kTY> sm;
+  __shared__ TileSmem<TX, kTY> sm;
   const int total = hdr->total, nsl = hdr->nslots;
   const SlotDev *__restrict__ slots = table + hdr->base;
   const int G = gridDim.x, r = xcd_remap(blockIdx.x, G);
@@ -101,8 +103,8 @@ batch_kernel(const TableHdr *__restrict__ hdr, const SlotDev *__restrict__ table
     const size_t plane5 = 5 * (size_t)S.ns;
     float *Q = S.state + (size_t)q * plane5, *S1 = S.state + (size_t)((q + 1) % 3) * plane5,
           *Bb = S.state + (size_t)((q + 2) % 3) * plane5;
-    if constexpr (stage == 0) tile_stage<64, kTY, false, 1>(S.cv, S.kc, Q, Q, S1, nullptr, S.ns, 0.f, 1.f, bx * 64, by * kTY, k0, k1, sm);
-    else tile_stage<64, kTY, false, 2>(S.cv, S.kc, S1, Q, Bb, nullptr, S.ns, 0.5f, 0.5f, bx * 64, by * kTY, k0, k1, sm);
+    if constexpr (stage == 0) tile_stage<TX, kTY, false, 1>(S.cv, S.kc, Q, Q, S1, nullptr, S.ns, 0.f, 1.f, bx * TX, by * kTY, k0, k1, sm);
+    else tile_stage<TX, kTY, false, 2>(S.cv, S.kc, S1, Q, Bb, nullptr, S.ns, 0.5f, 0.5f, bx * TX, by * kTY, k0, k1, sm);
   };
   if constexpr (persist) {
     for (int base = 0; base < total; base += G) {
@@ -240,11 +242,13 @@ struct aero3d_batch {
   SlotStat *d_stat = nullptr;        // max_slots status records
   RbSeg *d_seg = nullptr;            // 2 x max_slots readback segments (admission, check)
   int max_slots = 0;
-  int grid = 0;                      // resident blocks of the persistent form
+  int grid = 0, grid32 = 0;          // resident blocks of the persistent form (64- / 32-column tiles)
   std::vector<hipGraph_t> graphs;    // per case group: three iterations
   std::vector<hipGraphExec_t> execs;
   std::vector<int> totals;           // per case group: work items (the launch grid)
-  std::vector<int> graph_total;      // per case group: the launch grid its graph was captured with
+  std::vector<int> gtx;              // per case group: tile width of its cases
+  int tile_width = 0;                // 64 or 32 for every case; 0: per case (case_tx)
+  std::vector<int> graph_total;      // per case group: the launch grid and tile width its graph was captured with
   int persistent = 0;                // 1: resident blocks walk the items (batch_kernel<., 1>)
   bool table_dirty = true;
   std::vector<int> order;            // slot ids in table order
@@ -323,6 +327,17 @@ struct StageLayout {
 // solves/s against 48.5 at 3; the arrival-bound stream (a few requests in
 // flight) runs at p50 / p99 0.13 / 0.37 s with 3 planes against 0.20 / 0.41 s
 // with 8.
+// Tile width of a case: 32 when 32-column tiles cover its x extent with fewer
+// padded columns than 64-column ones (e.g. NX 72, 95, 160), else 64; or the
+// batch's fixed width. The state is the same bit for bit either way. Measured
+// on the plain path (one case, us per iteration, 64 / 32 wide): NX 72 22.1 /
+// 21.5, 95 22.2 / 21.0, 160 42.2 / 36.5; where the padded widths are equal,
+// 64 wins (NX 115: 40.7 / 43.2, NX 174: 60.8 / 62.9).
+int case_tx(const aero3d_batch *b, int NX) {
+  if (b->tile_width) return b->tile_width;
+  return (NX + 31) / 32 * 32 < (NX + 63) / 64 * 64 ? 32 : 64;
+}
+
 int load_zc(const aero3d_batch *b) {
   const int cand[4] = {8, 6, 4, 3};
   for (int c = 0; c < 4; c++) {
@@ -330,7 +345,8 @@ int load_zc(const aero3d_batch *b) {
     for (const Slot &S : b->slots) {
       if (!S.used) continue;
       const Geometry &g = b->geos[S.geo];
-      items += (int64_t)((g.NX + 63) / 64) * ((g.NY + kTY - 1) / kTY) * ((g.NZ + cand[c] - 1) / cand[c]);
+      const int tx = case_tx(b, g.NX);
+      items += (int64_t)((g.NX + tx - 1) / tx) * ((g.NY + kTY - 1) / kTY) * ((g.NZ + cand[c] - 1) / cand[c]);
     }
     if (items >= 4 * (int64_t)b->grid) return cand[c];
   }
@@ -343,12 +359,18 @@ int upload_table(aero3d_batch *b) {
   std::vector<SlotDev> rows;
   std::vector<TableHdr> hdrs;
   int64_t cells = 0;
+  b->gtx.clear();
+  // cases of 64-column tiles first, then those of 32-column tiles: a group holds one tile width
+  for (int pass = 0; pass < 2; pass++)
   for (int s = 0; s < (int)b->slots.size(); s++) {
     Slot &S = b->slots[s];
     if (!S.used) continue;
     const Geometry &g = b->geos[S.geo];
-    if (hdrs.empty() || (hdrs.back().nslots > 0 && cells + g.N > b->group_cells)) {
+    const int tx = case_tx(b, g.NX);
+    if ((tx == 32) != (pass == 1)) continue;
+    if (hdrs.empty() || b->gtx.back() != tx || (hdrs.back().nslots > 0 && cells + g.N > b->group_cells)) {
       hdrs.push_back(TableHdr{0, 0, (int)rows.size()});
+      b->gtx.push_back(tx);
       cells = 0;
     }
     cells += g.N;
@@ -358,7 +380,7 @@ int upload_table(aero3d_batch *b) {
     d.state = S.state; d.ns = S.ns; d.cur = S.cur;
     S.zc = zc_load;
     d.zc = S.zc;
-    d.ntx = (g.NX + 63) / 64; d.nty = (g.NY + kTY - 1) / kTY;
+    d.ntx = (g.NX + tx - 1) / tx; d.nty = (g.NY + kTY - 1) / kTY;
     d.nitems = d.ntx * d.nty * ((g.NZ + S.zc - 1) / S.zc);
     d.item0 = hdrs.back().total;
     d.slot = s;
@@ -386,20 +408,29 @@ int upload_table(aero3d_batch *b) {
 }
 
 // Both stages of one SSP-RK2 iteration of case group gi at rotation phase ph.
-void launch_iteration(aero3d_batch *b, int gi, int ph) {
+template <int TX>
+void launch_iteration_tx(aero3d_batch *b, int gi, int ph) {
   const TableHdr *h = b->d_hdr + gi;
   if (b->persistent) {
-    batch_kernel<0, 1><<<b->grid, dim3(64, kTY), 0, b->stream>>>(h, b->d_table, b->d_stat, ph);
-    batch_kernel<1, 1><<<b->grid, dim3(64, kTY), 0, b->stream>>>(h, b->d_table, b->d_stat, ph);
+    const int G = TX == 32 ? b->grid32 : b->grid;
+    batch_kernel<0, 1, TX><<<G, dim3(TX, kTY), 0, b->stream>>>(h, b->d_table, b->d_stat, ph);
+    batch_kernel<1, 1, TX><<<G, dim3(TX, kTY), 0, b->stream>>>(h, b->d_table, b->d_stat, ph);
   } else {
     const int G = std::max(1, b->totals[gi]);
-    batch_kernel<0, 0><<<G, dim3(64, kTY), 0, b->stream>>>(h, b->d_table, b->d_stat, ph);
-    batch_kernel<1, 0><<<G, dim3(64, kTY), 0, b->stream>>>(h, b->d_table, b->d_stat, ph);
+    batch_kernel<0, 0, TX><<<G, dim3(TX, kTY), 0, b->stream>>>(h, b->d_table, b->d_stat, ph);
+    batch_kernel<1, 0, TX><<<G, dim3(TX, kTY), 0, b->stream>>>(h, b->d_table, b->d_stat, ph);
   }
 }
+void launch_iteration(aero3d_batch *b, int gi, int ph) {
+  if (b->gtx[gi] == 32) launch_iteration_tx<32>(b, gi, ph);
+  else launch_iteration_tx<64>(b, gi, ph);
+}
 
-// the launch grid a group's iteration uses (what its captured graph holds)
-int group_grid(const aero3d_batch *b, int gi) { return b->persistent ? -b->grid : std::max(1, b->totals[gi]); }
+// the launch grid and tile width a group's iteration uses (what its captured graph holds)
+int group_grid(const aero3d_batch *b, int gi) {
+  const int g = b->persistent ? -(b->gtx[gi] == 32 ? b->grid32 : b->grid) : std::max(1, b->totals[gi]);
+  return 2 * g + (b->gtx[gi] == 32 ? 1 : 0);
+}
 
 int build_graph(aero3d_batch *b, int gi) {
   hipGraph_t g;
@@ -507,13 +538,15 @@ int aero3d_batch_create(int64_t arena_bytes, int max_slots, aero3d_batch **out) 
     aero3d_batch_destroy(b);
     return AERO3D_E_HIP_BASE;
   }
-  int dev = 0, ncu = 0, occ = 0;
+  int dev = 0, ncu = 0, occ = 0, occ32 = 0;
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_kernel<1, 1>, 64 * kTY, 0) != hipSuccess) {
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_kernel<1, 1, 64>, 64 * kTY, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ32, batch_kernel<1, 1, 32>, 32 * kTY, 0) != hipSuccess) {
     aero3d_batch_destroy(b);
     return AERO3D_E_HIP_BASE;
   }
   b->grid = std::max(1, ncu * std::max(1, occ));       // resident blocks: the persistent form's grid
+  b->grid32 = std::max(1, ncu * std::max(1, occ32));
   *out = b;
   return 0;
 }
@@ -642,6 +675,13 @@ int aero3d_batch_add(aero3d_batch *b, int gid, const double F[5], const double f
 int aero3d_batch_remove(aero3d_batch *b, int slot) {
   if (!b || slot < 0 || slot >= b->max_slots || !b->slots[slot].used) return bfail(b, AERO3D_E_ARG, "remove: bad slot");
   free_slot(b, slot);
+  return 0;
+}
+
+int aero3d_batch_set_tile_width(aero3d_batch *b, int width) {
+  if (!b || !(width == 0 || width == 32 || width == 64)) return bfail(b, AERO3D_E_ARG, "set_tile_width: 0, 32 or 64");
+  b->tile_width = width;
+  b->table_dirty = true;
   return 0;
 }
 

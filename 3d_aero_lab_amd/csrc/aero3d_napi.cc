@@ -715,7 +715,7 @@ napi_value BatchForces(napi_env env, napi_callback_info info) {
   set_num(env, o, "CL", out[0]); set_num(env, o, "CD", out[1]); set_num(env, o, "Cm", out[2]);
   return o;
 }
-// batchSetOption(h, name, value): zchunk | persistent | groupCells
+// batchSetOption(h, name, value): zchunk | tileWidth | persistent | groupCells
 napi_value BatchSetOption(napi_env env, napi_callback_info info) {
   napi_value a[3];
   if (!args(env, info, 3, a)) return nullptr;
@@ -727,9 +727,10 @@ napi_value BatchSetOption(napi_env env, napi_callback_info info) {
   const std::string k(key, len);
   int rc;
   if (k == "zchunk") rc = aero3d_batch_set_zchunk(r->b, (int)num(env, a[2]));
+  else if (k == "tileWidth") rc = aero3d_batch_set_tile_width(r->b, (int)num(env, a[2]));
   else if (k == "persistent") rc = aero3d_batch_set_persistent(r->b, (int)num(env, a[2]));
   else if (k == "groupCells") rc = aero3d_batch_set_group_cells(r->b, i64(env, a[2]));
-  else { napi_throw_range_error(env, nullptr, "batchSetOption: zchunk, persistent or groupCells"); return nullptr; }
+  else { napi_throw_range_error(env, nullptr, "batchSetOption: zchunk, tileWidth, persistent or groupCells"); return nullptr; }
   return rc ? batch_throw(env, r->b, rc) : undef(env);
 }
 napi_value BatchFootprint(napi_env env, napi_callback_info info) {

@@ -80,8 +80,9 @@ def batched_workload(seed=20261203, n=64):
     return out
 
 
-def run_converged_plain(built, conds, res_drop, max_iters, check_every, tile):
-    """Each case alone on the plain path, one after another, read back every check_every iterations
+def run_converged_plain(built, conds, res_drop, max_iters, check_every, tile=None):
+    """Each case alone on the plain path (tile: stage-kernel shape, None: the solver's default for the grid, the
+    width rule the batched form applies per case), one after another, read back every check_every iterations
     (aero3d_readback) under the residual-drop rule (res/res0 <= res_drop, or max_iters), the server's
     rule. Returns the wall time, the iterations and the final (CL, CD, Cm) of each case."""
     import aero3d
@@ -91,7 +92,8 @@ def run_converged_plain(built, conds, res_drop, max_iters, check_every, tile):
         F, fix, fref = case_setup(meta, M, alpha)
         s = aero3d.CfdHIP(grid, geo)
         s.opt["cfl"] = 0.5
-        s.set_tile(*tile)
+        if tile is not None:
+            s.set_tile(*tile)
         s.set_free(F); s.set_fix(fix); s.reset()
         s.set_force_list(fl)
         hist = s.solve(fref, aero3d.ConvRule(aero3d.RULE_RES_DROP, res_drop, max_iters), check_every)
@@ -174,7 +176,7 @@ def batched_leg(res_drop=1e-4, max_iters=2000, check_every=50):
     done, wall_b, gpu_b, cu = run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes=int(cells * 110 + (2 << 30)),
                                          group_cells=group, persistent=pers)
     print("[bench] batched: plain path, one case at a time", file=sys.stderr, flush=True)
-    wall_p, iters_p, coefs_p = run_converged_plain(built, conds, res_drop, max_iters, check_every, (8, 0))
+    wall_p, iters_p, coefs_p = run_converged_plain(built, conds, res_drop, max_iters, check_every)
     it_b = {d["tag"]: d["iters"] for d in done}
     same = all(it_b[k] == iters_p[k] for k in range(len(conds)))
     dcoef = coef_rel_diff(done, coefs_p)
@@ -221,7 +223,7 @@ def serve_leg(n_requests=200, res_drop=1e-4, max_iters=2000, check_every=50):
     # the same requests one at a time on the plain path (arrival times ignored: its best case)
     print("[bench] serve: plain path, one request at a time", file=sys.stderr, flush=True)
     wall_p, iters_p, coefs_p = run_converged_plain([built[r[0]] for r in reqs], [(r[1], r[2]) for r in reqs], res_drop, max_iters,
-                                                   check_every, (8, 0))
+                                                   check_every)
     it_s = {d["tag"]: d["iters"] for d in done}
     # capacity: the same requests all arriving at once (the stream above is arrival-bound at ~20 req/s)
     print("[bench] serve: capacity run, all requests at t = 0", file=sys.stderr, flush=True)
@@ -237,7 +239,7 @@ def serve_leg(n_requests=200, res_drop=1e-4, max_iters=2000, check_every=50):
         lreq = [(b, 0.3 + 0.55 * rnd(), -2 + 10 * rnd(), 0.0) for b in range(3) for _ in range(per_body)]
         dl, wl, gl, cul = run_server(lvl_built, lreq, res_drop, max_iters, check_every, 0, arena_bytes=64 << 30)
         wp, ip, _ = run_converged_plain([lvl_built[r[0]] for r in lreq], [(r[1], r[2]) for r in lreq], res_drop, max_iters,
-                                        check_every, (8, 0))
+                                        check_every)
         it_l = {d["tag"]: d["iters"] for d in dl}
         levels[str(cells)] = {"requests": len(lreq), "served_solves_per_s": len(lreq) / wl, "plain_solves_per_s": len(lreq) / wp,
                               "speedup_vs_plain": wp / wl, "gpu_busy_frac": gl / wl, "served_cell_updates_per_s": cul / wl,

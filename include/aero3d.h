@@ -183,7 +183,8 @@ int aero3d_time_steps(aero3d_solver *s, int n, float *ms);
  * balanced launch of max(1, -zchunk) x (resident blocks of the device), each
  * block marching an equal share of one tile's planes. ty = 0: one thread per
  * cell, each face flux evaluated by both of its cells as the reference does.
- * Default ty 8, zchunk 0. */
+ * Default zchunk 0 and ty 32 where 32-column tiles cover NX with fewer padded
+ * columns than 64-column ones (NX 72, 95, 160, ...), else ty 8. */
 int aero3d_set_tile(aero3d_solver *s, int ty, int zchunk);
 int aero3d_get_tile(const aero3d_solver *s, int *ty, int *zchunk);
 /* Autotuner of the stage-kernel shape: times every candidate (ty in {4, 8, 32},
@@ -221,13 +222,18 @@ int aero3d_batch_case_bytes(const aero3d_batch *b, int gid, int64_t *bytes);
  * arena room is left. */
 int aero3d_batch_add(aero3d_batch *b, int gid, const double F[5], const double fix[15], double cfl, int order, int *slot);
 int aero3d_batch_remove(aero3d_batch *b, int slot);
-/* Stage-kernel z chunk of every case (tiles are 64 x 8 columns): zchunk > 0
+/* Stage-kernel z chunk of every case (tiles of 64 x 8 or 32 x 8 columns): zchunk > 0
  * planes per work item; zchunk 0 (the default) from the load, each time the
  * case table changes: the longest of 8, 6, 4 and 3 planes that gives at
  * least four work items per resident block (long chunks for throughput when
  * many cases are in flight, short ones for latency when few are). Negative
  * values are rejected with AERO3D_E_ARG. */
 int aero3d_batch_set_zchunk(aero3d_batch *b, int zchunk);
+/* Tile width of the stage kernel: 64 or 32 columns for every case, or 0 (the
+ * default) per case: 32 when 32-column tiles cover the case's x extent with
+ * fewer padded columns than 64-column ones, else 64. States are the same bit
+ * for bit either way. Other values are rejected with AERO3D_E_ARG. */
+int aero3d_batch_set_tile_width(aero3d_batch *b, int width);
 /* Cases are stepped in groups of about `cells` cells (default: all cases in
  * one group): a group's state and masks stay in the Infinity Cache across its
  * stages and iterations before the next group runs. */
