@@ -231,8 +231,10 @@ int aero3d_batch_remove(aero3d_batch *b, int slot);
 int aero3d_batch_set_zchunk(aero3d_batch *b, int zchunk);
 /* Tile width of the stage kernel: 64 or 32 columns for every case, or 0 (the
  * default) per case: 32 when 32-column tiles cover the case's x extent with
- * fewer padded columns than 64-column ones, else 64. States are the same bit
- * for bit either way. Other values are rejected with AERO3D_E_ARG. */
+ * fewer padded columns than 64-column ones, else 64; cases of each width are
+ * launched apart, so when either width would have fewer than two work items
+ * per resident block every case takes 64 columns (one launch). States are the
+ * same bit for bit either way. Other values are rejected with AERO3D_E_ARG. */
 int aero3d_batch_set_tile_width(aero3d_batch *b, int width);
 /* Cases are stepped in groups of about `cells` cells (default: all cases in
  * one group): a group's state and masks stay in the Infinity Cache across its
