@@ -143,9 +143,9 @@ def run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes, m
     return done, wall, gpu, cu
 
 
-def batched_leg(res_drop=1e-4, max_iters=2000, check_every=50):
-    """Converged solves/s of the 64-case batch (one persistent launch per stage, graph-replayed) against
-    the same cases solved one at a time on the plain path."""
+def batched_leg(res_drop=1e-4, max_iters=2000, check_every=50, config=None):
+    """Converged solves/s of the 64-case batch (one launch per stage, graph-replayed) against the same cases
+    solved one at a time on the plain path. config: (zchunk, group_cells, persistent), or None to tune them."""
     import aero3d
     import geometry
     wl = batched_workload()
@@ -153,24 +153,27 @@ def batched_leg(res_drop=1e-4, max_iters=2000, check_every=50):
     built = geometry.build_cases([w[0] for w in wl])
     conds = [(w[1], w[2]) for w in wl]
     cells = sum(b[3]["N"] for b in built)
-    # autotune the z chunk and the case-group size on a throwaway batch holding every case
-    tb = aero3d.Batch(arena_bytes=int(cells * 110 + (2 << 30)), max_slots=128)
-    for (g, geo, fl, meta), (M, alpha) in zip(built, conds):
-        F, fix, _ = case_setup(meta, M, alpha)
-        tb.add(tb.add_geometry(g, geo), F, fix, 0.5)
-    best = None
-    for pers in (0, 1):
-        tb.set_persistent(pers)
-        for zc in (0, 3, 4, 6, 8):          # 0: by grid size
-            for gc in (1_000_000, 2_000_000, 4_000_000, 1 << 40):
-                tb.set_zchunk(zc)
-                tb.set_group_cells(gc)
-                tb.time_steps(3)
-                ms = tb.time_steps(3)
-                if best is None or ms < best[1]:
-                    best = (zc, ms, gc, pers)
-    tb.dispose()
-    zc, group, pers = best[0], best[2], best[3]
+    if config is None:
+        # autotune the z chunk and the case-group size on a throwaway batch holding every case
+        tb = aero3d.Batch(arena_bytes=int(cells * 110 + (2 << 30)), max_slots=128)
+        for (g, geo, fl, meta), (M, alpha) in zip(built, conds):
+            F, fix, _ = case_setup(meta, M, alpha)
+            tb.add(tb.add_geometry(g, geo), F, fix, 0.5)
+        best = None
+        for pers in (0, 1):
+            tb.set_persistent(pers)
+            for zc in (0, 3, 4, 6, 8):          # 0: from the load
+                for gc in (1_000_000, 2_000_000, 4_000_000, 1 << 40):
+                    tb.set_zchunk(zc)
+                    tb.set_group_cells(gc)
+                    tb.time_steps(3)
+                    ms = tb.time_steps(3)
+                    if best is None or ms < best[1]:
+                        best = (zc, ms, gc, pers)
+        tb.dispose()
+        zc, group, pers = best[0], best[2], best[3]
+    else:
+        zc, group, pers = config
     print(f"[bench] batched: {cells} cells, zchunk {zc}, group {group}, persistent {pers}; server run", file=sys.stderr, flush=True)
     reqs = [(k, M, alpha, 0.0) for k, (M, alpha) in enumerate(conds)]
     done, wall_b, gpu_b, cu = run_server(built, reqs, res_drop, max_iters, check_every, zc, arena_bytes=int(cells * 110 + (2 << 30)),
@@ -336,6 +339,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--legs", default="plain,sphere,batched,serve,axi", help="comma list of plain, sphere, batched, serve, axi")
     ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--batch-config", default="auto",
+                    help="batched leg: 'zchunk,group_cells,persistent' (0 = from the load, group cells, 0 / 1) or 'auto' (tuner)")
     ap.add_argument("--tile", default="auto", help="stage-kernel shape 'ty,zchunk' or 'auto' (autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of known size (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -406,7 +411,7 @@ def main():
     if "sphere" in legs:
         out["sphere"] = sphere_leg()
     if "batched" in legs:
-        out["batched"] = batched_leg()
+        out["batched"] = batched_leg(config=None if a.batch_config == "auto" else tuple(int(x) for x in a.batch_config.split(",")))
     if "serve" in legs:
         out["served"] = serve_leg(a.serve_requests)
     if "axi" in legs:
