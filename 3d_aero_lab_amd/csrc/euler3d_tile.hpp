@@ -136,21 +136,23 @@ __device__ __forceinline__ void hllc_fast(const float *L, const float *R, float 
 // Face flux of axis d between left cell (PL, TL, slope sL) and right cell
 // (PR, TR, slope sR) with aperture a; zero when the face is closed or touches
 // no fluid cell (no cell would use it).
-// kPin (operands read from LDS): every operand is pinned in a register before
-// the face is tested, so all of its LDS reads are issued together and waited
-// for once; otherwise the reads are sunk below each test, one LDS round trip
-// per test.
+// kPin (operands read from LDS): every operand is an input of one empty asm
+// statement before the face is tested, so all of its LDS reads are issued
+// together and waited for once; otherwise the reads are sunk below the test,
+// a second LDS round trip. Input operands only: read-write ones made the
+// compiler copy the values out of the paired LDS reads' registers.
 template <int d, bool kPin = false>
 __device__ __forceinline__ void face_flux(float *PL, float *sL, int TL, float *PR, float *sR, int TR, float a, float *F) {
   if constexpr (kPin) {
 #ifdef __AMDGCN__
     // one statement, so the reads are waited for once (one wait per pinned value otherwise)
-    asm volatile("" : "+v"(PL[0]), "+v"(PL[1]), "+v"(PL[2]), "+v"(PL[3]), "+v"(PL[4]), "+v"(sL[0]), "+v"(sL[1]), "+v"(sL[2]),
-                 "+v"(sL[3]), "+v"(sL[4]), "+v"(PR[0]), "+v"(PR[1]), "+v"(PR[2]), "+v"(PR[3]), "+v"(PR[4]), "+v"(sR[0]),
-                 "+v"(sR[1]), "+v"(sR[2]), "+v"(sR[3]), "+v"(sR[4]), "+v"(TL), "+v"(TR), "+v"(a));
+    asm volatile("" ::"v"(PL[0]), "v"(PL[1]), "v"(PL[2]), "v"(PL[3]), "v"(PL[4]), "v"(sL[0]), "v"(sL[1]), "v"(sL[2]),
+                 "v"(sL[3]), "v"(sL[4]), "v"(PR[0]), "v"(PR[1]), "v"(PR[2]), "v"(PR[3]), "v"(PR[4]), "v"(sR[0]),
+                 "v"(sR[1]), "v"(sR[2]), "v"(sR[3]), "v"(sR[4]), "v"(TL), "v"(TR), "v"(a));
 #endif
   }
-  if (!(a > 0.f) || (TL != 0 && TR != 0)) {
+  // one test without short-circuit: a single branch, so the zero flux is set once
+  if ((!(a > 0.f)) | ((TL != 0) & (TR != 0))) {
 #pragma unroll
     for (int v = 0; v < 5; v++) F[v] = 0.f;
     return;
