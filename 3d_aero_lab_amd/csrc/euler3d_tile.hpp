@@ -92,6 +92,11 @@ __device__ __forceinline__ void prim_fast(float a0, float a1, float a2, float a3
   o[4] = fmaxf(kGm1 * (a4 - 0.5f * r * (u * u + v * v + w * w)), 1e-7f);
 }
 
+// Total energy of a side of a face from its primitives (rho, p, velocity).
+__device__ __forceinline__ float hllc_energy(float r, float p, float v0, float v1, float v2) {
+  return p * 2.5f + 0.5f * r * (v0 * v0 + v1 * v1 + v2 * v2);
+}
+
 // HLLC flux along axis d (src/air_cfd.js:179-195) with four hardware
 // transcendentals per face: two reciprocal square roots for the sound speeds,
 // the reciprocals of the contact-speed denominator and of S - SM (the star
@@ -104,30 +109,36 @@ __device__ __forceinline__ void hllc_fast(const float *L, const float *R, float 
   const float rL = L[0], rR = R[0], pL = L[4], pR = R[4], uL = L[1 + d], uR = R[1 + d];
   const float gpL = kGamma * pL, gpR = kGamma * pR;   // a = sqrt(g p / r) = g p / sqrt(g p r)
   const float aL = gpL * frsq(gpL * rL), aR = gpR * frsq(gpR * rR);
-  const float EL = pL * 2.5f + 0.5f * rL * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]);
-  const float ER = pR * 2.5f + 0.5f * rR * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]);
   const float SL = fminf(uL - aL, uR - aR), SR = fmaxf(uL + aL, uR + aR);
+  // total energy of the chosen side only (hllc_energy), from that side's
+  // selected values: the same operations as for either side, so the same bits
   if (SL >= 0.f || SR <= 0.f) {              // supersonic: the upwind side's flux
     const bool left = SL >= 0.f;
-    const float r = left ? rL : rR, u = left ? uL : uR, p = left ? pL : pR, E = left ? EL : ER;
+    const float r = left ? rL : rR, u = left ? uL : uR, p = left ? pL : pR;
+    const float v0 = left ? L[1] : R[1], v1 = left ? L[2] : R[2], v2 = left ? L[3] : R[3];
+    const float E = hllc_energy(r, p, v0, v1, v2);
     const float m = r * u;
+    // each component stored once (no update of an element picked by d)
     out[0] = m;
-#pragma unroll
-    for (int c = 1; c < 4; c++) out[c] = m * (left ? L[c] : R[c]);
-    out[1 + d] += p;
+    out[1] = d == 0 ? m * v0 + p : m * v0;
+    out[2] = d == 1 ? m * v1 + p : m * v1;
+    out[3] = d == 2 ? m * v2 + p : m * v2;
     out[4] = u * (E + p);
     return;
   }
   const float SM = (pR - pL + rL * uL * (SL - uL) - rR * uR * (SR - uR)) * frcp(rL * (SL - uL) - rR * (SR - uR));
   const bool left = SM >= 0.f;
-  const float S = left ? SL : SR, uK = left ? uL : uR, rK = left ? rL : rR, pK = left ? pL : pR, EK = left ? EL : ER;
+  const float S = left ? SL : SR, uK = left ? uL : uR, rK = left ? rL : rR, pK = left ? pL : pR;
+  const float v0 = left ? L[1] : R[1], v1 = left ? L[2] : R[2], v2 = left ? L[3] : R[3];
+  const float EK = hllc_energy(rK, pK, v0, v1, v2);
   const float SmU = S - uK, iSS = frcp(S - SM);
   const float f = rK * SmU * iSS, ru = rK * uK;
   const float m = ru + S * (f - rK);         // mass flux rK uK + S (f - rK)
+  const float mn = ru * uK + pK + S * (f * SM - ru);   // normal momentum
   out[0] = m;
-#pragma unroll
-  for (int c = 1; c < 4; c++) out[c] = m * (left ? L[c] : R[c]);
-  out[1 + d] = ru * uK + pK + S * (f * SM - ru);
+  out[1] = d == 0 ? mn : m * v0;
+  out[2] = d == 1 ? mn : m * v1;
+  out[3] = d == 2 ? mn : m * v2;
   // f (E/r + (SM - u)(SM + p / (r (S - u)))) with f = r (S - u) / (S - SM)
   const float Es = EK * SmU * iSS + (SM - uK) * (f * SM + pK * iSS);
   out[4] = uK * (EK + pK) + S * (Es - EK);
