@@ -193,7 +193,7 @@ __device__ __forceinline__ void slope5(const float *Pm, const float *P0, const f
 // (a short-circuit test would read them one LDS round trip at a time).
 __device__ __forceinline__ bool slope_ok(bool o2, int tm, int t0, int tp, float am, float ap) {
 #ifdef __AMDGCN__
-  asm volatile("" : "+v"(tm), "+v"(t0), "+v"(tp), "+v"(am), "+v"(ap));
+  asm volatile("" ::"v"(tm), "v"(t0), "v"(tp), "v"(am), "v"(ap));
 #endif
   return o2 & (tm == 0) & (t0 == 0) & (tp == 0) & (am == 1.f) & (ap == 1.f);
 }
@@ -201,14 +201,18 @@ __device__ __forceinline__ bool slope_ok(bool o2, int tm, int t0, int tp, float 
 template <int TX, int TY>
 struct TileSmem {
   static constexpr int PW = TX + 4, PH = TY + 4;
-  float P[5][PH][PW];        // primitives, tile + 2-cell halo (corners unused)
+  // components innermost: neighbouring cells' values lie within one paired
+  // read's offset range of one base address (fewer address adds and LDS
+  // instructions than one plane per component); cells are 20 bytes apart,
+  // so 32 consecutive lanes reading one component hit 32 different banks
+  float P[PH][PW][5];        // primitives, tile + 2-cell halo (corners unused)
   int8_t T[PH][PW];          // flags: 0..5 cells, -1 freestream ghost, -2 mirror ghost
   float A0[TY][TX + 3];      // ap0 of cells x = -2..TX (x faces)
   float A1[TY + 3][TX];      // ap1 of rows y = -2..TY (y faces)
-  float Sx[5][TY][TX + 2];   // x slopes of cells -1..TX
-  float Sy[5][TY + 2][TX];   // y slopes of rows -1..TY
-  float Fx[5][TY][TX + 1];   // x fluxes of faces -1/2..TX-1/2
-  float Fy[5][TY + 1][TX];   // y fluxes of faces -1/2..TY-1/2
+  float Sx[TY][TX + 2][5];   // x slopes of cells -1..TX
+  float Sy[TY + 2][TX][5];   // y slopes of rows -1..TY
+  float Fx[TY][TX + 1][5];   // x fluxes of faces -1/2..TX-1/2
+  float Fy[TY + 1][TX][5];   // y fluxes of faces -1/2..TY-1/2
 };
 
 // Classify and load the primitives of cell (i, j, k) of one case: freestream
@@ -270,7 +274,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
   // thread; each cell's raw state is loaded one plane ahead (ha / ht) so the
   // loads of plane k + 1 are in flight while plane k is computed
   constexpr int NT = TX * TY, NH = 4 * TX + 4 * TY, NHP = (NH + NT - 1) / NT, PW = TileSmem<TX, TY>::PW;
-  int hoff[NHP];             // LDS offset (hy + 2) * PW + hx + 2 in a plane of sm.P, -1: no cell
+  int hoff[NHP];             // cell index (hy + 2) * PW + hx + 2 in sm.P / sm.T, -1: no cell
   unsigned hq[NHP];          // plane-0 index of the cell (mirrored row below y = 0)
   bool hin[NHP], hmir[NHP];  // inside the box in x / y; mirror ghost
   float ha[NHP][5];
@@ -400,7 +404,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
     const float A2z = apz(k + 1);                   // face k+3/2
     // tile interior of plane k from the queue
 #pragma unroll
-    for (int v = 0; v < 5; v++) sm.P[v][ty + 2][tx + 2] = Pz[1][v];
+    for (int v = 0; v < 5; v++) sm.P[ty + 2][tx + 2][v] = Pz[1][v];
     sm.T[ty + 2][tx + 2] = (int8_t)Tz[1];
     // halo ring of plane k from the prefetched raw state (primitives as
     // load_prim), then the prefetch of plane k + 1
@@ -412,7 +416,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
 #pragma unroll
       for (int v = 0; v < 5; v++) {
         const float x = !hinv[p] ? kc.Fw[v] : w[v];
-        (&sm.P[v][0][0])[hoffv[p]] = (v == 2 && hinv[p] && hmirv[p]) ? -x : x;
+        (&sm.P[0][0][0])[hoffv[p] * 5 + v] = (v == 2 && hinv[p] && hmirv[p]) ? -x : x;
       }
       (&sm.T[0][0])[hoffv[p]] = (int8_t)(!hinv[p] ? -1 : (hmirv[p] ? -2 : ht[p]));
     }
@@ -432,18 +436,18 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       const int xi = tx + 2, yj = ty + 2;
       float P0[5], Pm[5], Pp[5], s[5];
 #pragma unroll
-      for (int v = 0; v < 5; v++) { P0[v] = sm.P[v][yj][xi]; Pm[v] = sm.P[v][yj][xi - 1]; Pp[v] = sm.P[v][yj][xi + 1]; }
+      for (int v = 0; v < 5; v++) { P0[v] = sm.P[yj][xi][v]; Pm[v] = sm.P[yj][xi - 1][v]; Pp[v] = sm.P[yj][xi + 1][v]; }
       const int t0 = sm.T[yj][xi];
       bool ok = slope_ok(o2, sm.T[yj][xi - 1], t0, sm.T[yj][xi + 1], sm.A0[ty][tx + 1], sm.A0[ty][tx + 2]);
       slope5(Pm, P0, Pp, ok, s);
 #pragma unroll
-      for (int v = 0; v < 5; v++) sm.Sx[v][ty][tx + 1] = s[v];
+      for (int v = 0; v < 5; v++) sm.Sx[ty][tx + 1][v] = s[v];
 #pragma unroll
-      for (int v = 0; v < 5; v++) { Pm[v] = sm.P[v][yj - 1][xi]; Pp[v] = sm.P[v][yj + 1][xi]; }
+      for (int v = 0; v < 5; v++) { Pm[v] = sm.P[yj - 1][xi][v]; Pp[v] = sm.P[yj + 1][xi][v]; }
       ok = slope_ok(o2, sm.T[yj - 1][xi], t0, sm.T[yj + 1][xi], sm.A1[ty + 1][tx], sm.A1[ty + 2][tx]);
       slope5(Pm, P0, Pp, ok, s);
 #pragma unroll
-      for (int v = 0; v < 5; v++) sm.Sy[v][ty + 1][tx] = s[v];
+      for (int v = 0; v < 5; v++) sm.Sy[ty + 1][tx][v] = s[v];
     }
     // the tile-edge slopes go to waves 2.. (waves 0 and 1 take the tile-edge fluxes)
     for (int e = etv; e < 2 * TY + 2 * TX; e += TX * TY) {
@@ -451,7 +455,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       if (e < 2 * TY) {                                 // x slope of cell -1 or TX in row r
         const int r = e >> 1, cx = (e & 1) ? TX : -1, xi = cx + 2, yj = r + 2;
 #pragma unroll
-        for (int v = 0; v < 5; v++) { P0[v] = sm.P[v][yj][xi]; Pm[v] = sm.P[v][yj][xi - 1]; Pp[v] = sm.P[v][yj][xi + 1]; }
+        for (int v = 0; v < 5; v++) { P0[v] = sm.P[yj][xi][v]; Pm[v] = sm.P[yj][xi - 1][v]; Pp[v] = sm.P[yj][xi + 1][v]; }
         // faces of cell cx: ap0 of cells cx-1 and cx; cell TX's + face lies outside the tile
         const int gi = i0 + cx, gj = j0 + r;
         const float am = sm.A0[r][cx + 1];
@@ -460,11 +464,11 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         const bool ok = slope_ok(o2, sm.T[yj][xi - 1], sm.T[yj][xi], sm.T[yj][xi + 1], am, apf);
         slope5(Pm, P0, Pp, ok, s);
 #pragma unroll
-        for (int v = 0; v < 5; v++) sm.Sx[v][r][cx + 1] = s[v];
+        for (int v = 0; v < 5; v++) sm.Sx[r][cx + 1][v] = s[v];
       } else {                                          // y slope of row -1 or TY in column c
         const int u = e - 2 * TY, c = u >> 1, ry = (u & 1) ? TY : -1, xi = c + 2, yj = ry + 2;
 #pragma unroll
-        for (int v = 0; v < 5; v++) { P0[v] = sm.P[v][yj][xi]; Pm[v] = sm.P[v][yj - 1][xi]; Pp[v] = sm.P[v][yj + 1][xi]; }
+        for (int v = 0; v < 5; v++) { P0[v] = sm.P[yj][xi][v]; Pm[v] = sm.P[yj - 1][xi][v]; Pp[v] = sm.P[yj + 1][xi][v]; }
         const int gi = i0 + c, gj = j0 + ry;
         const float am = sm.A1[ry + 1][c];
         const float apf = (ry == -1) ? sm.A1[ry + 2][c] : aSk;
@@ -472,7 +476,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         const bool ok = slope_ok(o2, sm.T[yj - 1][xi], sm.T[yj][xi], sm.T[yj + 1][xi], am, apf);
         slope5(Pm, P0, Pp, ok, s);
 #pragma unroll
-        for (int v = 0; v < 5; v++) sm.Sy[v][ry + 1][c] = s[v];
+        for (int v = 0; v < 5; v++) sm.Sy[ry + 1][c][v] = s[v];
       }
     }
     // z slope of plane k+1 (own column): cells k, k+1, k+2, faces k+1/2, k+3/2
@@ -486,42 +490,42 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       const int xi = tx + 2, yj = ty + 2;
       float PL[5], PR[5], sL[5], sR[5], F[5];
 #pragma unroll
-      for (int v = 0; v < 5; v++) { PL[v] = sm.P[v][yj][xi]; PR[v] = sm.P[v][yj][xi + 1]; sL[v] = sm.Sx[v][ty][tx + 1]; sR[v] = sm.Sx[v][ty][tx + 2]; }
+      for (int v = 0; v < 5; v++) { PL[v] = sm.P[yj][xi][v]; PR[v] = sm.P[yj][xi + 1][v]; sL[v] = sm.Sx[ty][tx + 1][v]; sR[v] = sm.Sx[ty][tx + 2][v]; }
       face_flux<0, true>(PL, sL, sm.T[yj][xi], PR, sR, sm.T[yj][xi + 1], sm.A0[ty][tx + 2], F);
 #pragma unroll
-      for (int v = 0; v < 5; v++) sm.Fx[v][ty][tx + 1] = F[v];
+      for (int v = 0; v < 5; v++) sm.Fx[ty][tx + 1][v] = F[v];
       if constexpr (kEarlyY) {   // the y face's LDS reads go out before the z face (registers only) is computed
         float PRy[5], sLy[5], sRy[5];
 #pragma unroll
-        for (int v = 0; v < 5; v++) { PRy[v] = sm.P[v][yj + 1][xi]; sLy[v] = sm.Sy[v][ty + 1][tx]; sRy[v] = sm.Sy[v][ty + 2][tx]; }
+        for (int v = 0; v < 5; v++) { PRy[v] = sm.P[yj + 1][xi][v]; sLy[v] = sm.Sy[ty + 1][tx][v]; sRy[v] = sm.Sy[ty + 2][tx][v]; }
         const int Ty0 = sm.T[yj][xi], Ty1 = sm.T[yj + 1][xi];
         const float Ay = sm.A1[ty + 2][tx];
         face_flux<2>(Pz[1], sz0, Tz[1], Pz[2], sz1n, Tz[2], A1z, Fzp);
         face_flux<1, true>(PL, sLy, Ty0, PRy, sRy, Ty1, Ay, F);
       } else {
 #pragma unroll
-        for (int v = 0; v < 5; v++) { PR[v] = sm.P[v][yj + 1][xi]; sL[v] = sm.Sy[v][ty + 1][tx]; sR[v] = sm.Sy[v][ty + 2][tx]; }
+        for (int v = 0; v < 5; v++) { PR[v] = sm.P[yj + 1][xi][v]; sL[v] = sm.Sy[ty + 1][tx][v]; sR[v] = sm.Sy[ty + 2][tx][v]; }
         face_flux<1, true>(PL, sL, sm.T[yj][xi], PR, sR, sm.T[yj + 1][xi], sm.A1[ty + 2][tx], F);
       }
 #pragma unroll
-      for (int v = 0; v < 5; v++) sm.Fy[v][ty + 1][tx] = F[v];
+      for (int v = 0; v < 5; v++) sm.Fy[ty + 1][tx][v] = F[v];
     }
     if (tidv < TX) {                                    // y face below row 0, column tid
       const int c = tidv, xi = c + 2;
       float PL[5], PR[5], sL[5], sR[5], F[5];
 #pragma unroll
-      for (int v = 0; v < 5; v++) { PL[v] = sm.P[v][1][xi]; PR[v] = sm.P[v][2][xi]; sL[v] = sm.Sy[v][0][c]; sR[v] = sm.Sy[v][1][c]; }
+      for (int v = 0; v < 5; v++) { PL[v] = sm.P[1][xi][v]; PR[v] = sm.P[2][xi][v]; sL[v] = sm.Sy[0][c][v]; sR[v] = sm.Sy[1][c][v]; }
       face_flux<1, true>(PL, sL, sm.T[1][xi], PR, sR, sm.T[2][xi], sm.A1[1][c], F);
 #pragma unroll
-      for (int v = 0; v < 5; v++) sm.Fy[v][0][c] = F[v];
+      for (int v = 0; v < 5; v++) sm.Fy[0][c][v] = F[v];
     } else if (tidv >= 64 && tidv < 64 + TY) {           // x face left of column 0, row r (wave 1)
       const int r = tidv - 64, yj = r + 2;
       float PL[5], PR[5], sL[5], sR[5], F[5];
 #pragma unroll
-      for (int v = 0; v < 5; v++) { PL[v] = sm.P[v][yj][1]; PR[v] = sm.P[v][yj][2]; sL[v] = sm.Sx[v][r][0]; sR[v] = sm.Sx[v][r][1]; }
+      for (int v = 0; v < 5; v++) { PL[v] = sm.P[yj][1][v]; PR[v] = sm.P[yj][2][v]; sL[v] = sm.Sx[r][0][v]; sR[v] = sm.Sx[r][1][v]; }
       face_flux<0, true>(PL, sL, sm.T[yj][1], PR, sR, sm.T[yj][2], sm.A0[r][1], F);
 #pragma unroll
-      for (int v = 0; v < 5; v++) sm.Fx[v][r][0] = F[v];
+      for (int v = 0; v < 5; v++) sm.Fx[r][0][v] = F[v];
     }
     if constexpr (!kEarlyY) face_flux<2>(Pz[1], sz0, Tz[1], Pz[2], sz1n, Tz[2], A1z, Fzp);
     // own apertures into registers (LDS apertures are rewritten by the next plane's load)
@@ -552,10 +556,10 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         const float wxp = ax_p * ihx, wxm = ax_m * ihx, wyp = ay_p * ihy, wym = ay_m * ihy, wzp = A1z * ihz, wzm = A0z * ihz;
 #pragma unroll
         for (int c = 0; c < 5; c++) {
-          float r = -(sm.Fx[c][ty][tx + 1] * wxp);
-          r = fmaf(sm.Fx[c][ty][tx], wxm, r);
-          r = fmaf(-sm.Fy[c][ty + 1][tx], wyp, r);
-          r = fmaf(sm.Fy[c][ty][tx], wym, r);
+          float r = -(sm.Fx[ty][tx + 1][c] * wxp);
+          r = fmaf(sm.Fx[ty][tx][c], wxm, r);
+          r = fmaf(-sm.Fy[ty + 1][tx][c], wyp, r);
+          r = fmaf(sm.Fy[ty][tx][c], wym, r);
           r = fmaf(-Fzp[c], wzp, r);
           R[c] = fmaf(Fzm[c], wzm, r);
         }
