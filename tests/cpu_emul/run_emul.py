@@ -30,8 +30,10 @@ def main():
     lib, case, iters, ty, zc = sys.argv[1], sys.argv[2], int(sys.argv[3]), int(sys.argv[4]), int(sys.argv[5])
     L = C.CDLL(lib)
     if case.startswith("synth:"):
-        n = [int(x) for x in case.split(":")[1:]]
-        grid, geo, F, fix = synth_case(*n, seed=sum(n))
+        # synth:NX:NY:NZ[@mach]
+        dims, _, mach = case.partition("@")
+        n = [int(x) for x in dims.split(":")[1:]]
+        grid, geo, F, fix = synth_case(*n, seed=sum(n), mach=float(mach) if mach else 0.6)
         cfl = 0.5
     else:
         g = Golden(case)

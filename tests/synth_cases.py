@@ -20,10 +20,12 @@ def _faces(rng, n, lo):
     return np.concatenate([[0.0], np.cumsum(1.0 + 0.15 * rng.random(n))]) + lo
 
 
-def synth_case(NX, NY, NZ, seed=0, body=True, quiescent=False):
+def synth_case(NX, NY, NZ, seed=0, body=True, quiescent=False, mach=0.6):
     """-> (grid, geo, F, fix) in the layouts the solver, the oracle and the batch take.
     body=False: no body, no baffles, no fixed cells (every cell fluid, every face
-    open). quiescent=True: gas at rest (M = 0) with fixed states equal to it."""
+    open). quiescent=True: gas at rest (M = 0) with fixed states equal to it.
+    mach: freestream Mach number along +x (negative: towards -x); |mach| > 1
+    makes most faces supersonic (the HLLC's upwind-side branches)."""
     rng = np.random.default_rng(seed)
     fx, fy, fz = _faces(rng, NX, -0.5 * NX), _faces(rng, NY, 0.0), _faces(rng, NZ, -0.5 * NZ)
     cx, cz = 0.5 * (fx[0] + fx[-1]), 0.5 * (fz[0] + fz[-1])
@@ -78,7 +80,7 @@ def synth_case(NX, NY, NZ, seed=0, body=True, quiescent=False):
         grid[a + "c"] = (0.5 * (f[:-1] + f[1:])).astype(np.float32)
     geo = {"t": t.reshape(-1), "phi": phi.reshape(-1).astype(np.float32),
            "ap": [a.reshape(-1).astype(np.float32) for a in ap] + [ap3.reshape(-1).astype(np.float32)]}
-    M, al = (0.0 if quiescent else 0.6), math.radians(4.0)
+    M, al = (0.0 if quiescent else mach), math.radians(4.0)
     p0 = 1 / 1.4
     F = [1.0, M * math.cos(al), 0.0, M * math.sin(al), p0 / 0.4 + 0.5 * M * M]
 

@@ -56,6 +56,28 @@ def test_small_grid_every_shape_matches_oracle(n):
     s.dispose()
 
 
+@pytest.mark.parametrize("mach", [2.0, -2.0])
+def test_supersonic_freestream_matches_oracle(mach):
+    """Mach 2 along +x and along -x: most x faces take the HLLC's upwind-side
+    (supersonic) branches, left and right respectively, on every shape."""
+    import aero3d
+    n = (65, 9, 5)
+    grid, geo, F, fix = synth_case(*n, seed=sum(n), mach=mach)
+    ref = oracle_state(grid, geo, F, fix)
+    floor = 1e-3 * max(abs(f) for f in F)
+    s = aero3d.CfdHIP(grid, geo)
+    s.opt["cfl"] = 0.5
+    s.set_free(F); s.set_fix(fix)
+    for ty, zc in [(8, 0), (4, 2), (32, 0)]:
+        say("mach", mach, "shape", ty, zc)
+        s.set_tile(ty, zc)
+        s.reset(); s.step(ITERS)
+        d = rel_diff(s.read(), ref, geo["t"], floor=floor)
+        say("mach", mach, "shape", ty, zc, d)
+        assert max(d.values()) < TOL, (mach, ty, zc, d)
+    s.dispose()
+
+
 @pytest.mark.parametrize("ty", [8, 4])
 def test_balanced_launch_priority_bit_identical(ty):
     """A balanced launch that fits one resident round sets each wave's priority

@@ -35,9 +35,11 @@ def test_tile_kernel_cpu_build_matches_oracle(emul_lib, case, iters, ty, zc):
 
 # one OS thread per emulated GPU thread makes each run cost seconds: a subset of
 # tests/synth_cases.py SIZES (single cells, sub-tile, just past a tile, tall and
-# thin), each with a z chunk shorter and one longer than the grid
+# thin), each with a z chunk shorter and one longer than the grid; two at Mach 2
+# along +x and -x (the HLLC's supersonic branches on most faces)
 SYNTH = [("synth:1:2:1", 8, 3), ("synth:2:2:3", 4, 1), ("synth:3:5:2", 8, 1), ("synth:65:9:3", 8, 2),
-         ("synth:7:40:2", 4, 3), ("synth:200:2:1", 8, 3), ("synth:33:9:3", 32, 2), ("synth:1:2:1", 32, 3)]
+         ("synth:7:40:2", 4, 3), ("synth:200:2:1", 8, 3), ("synth:33:9:3", 32, 2), ("synth:1:2:1", 32, 3),
+         ("synth:33:9:3@2.0", 32, 2), ("synth:65:9:3@-2.0", 8, 2)]
 
 
 @pytest.mark.parametrize("case,ty,zc", SYNTH)
