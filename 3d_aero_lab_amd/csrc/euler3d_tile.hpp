@@ -131,17 +131,19 @@ __device__ __forceinline__ void hllc_fast(const float *L, const float *R, float 
   const float S = left ? SL : SR, uK = left ? uL : uR, rK = left ? rL : rR, pK = left ? pL : pR;
   const float v0 = left ? L[1] : R[1], v1 = left ? L[2] : R[2], v2 = left ? L[3] : R[3];
   const float EK = hllc_energy(rK, pK, v0, v1, v2);
-  const float SmU = S - uK, iSS = frcp(S - SM);
-  const float f = rK * SmU * iSS, ru = rK * uK;
-  const float m = ru + S * (f - rK);         // mass flux rK uK + S (f - rK)
-  const float mn = ru * uK + pK + S * (f * SM - ru);   // normal momentum
+  // F + S (U* - U) with the star state U* = f (1, SM, v_t, E / r + (SM - u)(SM + p / (r (S - u)))),
+  // f = r (S - u) / (S - SM): every component of S (U* - U) reduces through
+  // g = S (SM - u) / (S - SM), giving the mass flux m = r (u + g), the normal
+  // momentum flux r u^2 + p + S r g and the energy flux (u + g)(E + p) + r g (S - u) SM
+  const float SmU = S - uK, g = S * (SM - uK) * frcp(S - SM);
+  const float ug = uK + g, rg = rK * g;
+  const float m = rK * ug;                   // mass flux
+  const float mn = rK * uK * uK + pK + S * rg;   // normal momentum
   out[0] = m;
   out[1] = d == 0 ? mn : m * v0;
   out[2] = d == 1 ? mn : m * v1;
   out[3] = d == 2 ? mn : m * v2;
-  // f (E/r + (SM - u)(SM + p / (r (S - u)))) with f = r (S - u) / (S - SM)
-  const float Es = EK * SmU * iSS + (SM - uK) * (f * SM + pK * iSS);
-  out[4] = uK * (EK + pK) + S * (Es - EK);
+  out[4] = ug * (EK + pK) + rg * SmU * SM;
 }
 
 // Face flux of axis d between left cell (PL, TL, slope sL) and right cell
