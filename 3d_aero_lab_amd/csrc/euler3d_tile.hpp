@@ -92,58 +92,59 @@ __device__ __forceinline__ void prim_fast(float a0, float a1, float a2, float a3
   o[4] = fmaxf(kGm1 * (a4 - 0.5f * r * (u * u + v * v + w * w)), 1e-7f);
 }
 
-// Total energy of a side of a face from its primitives (rho, p, velocity).
-__device__ __forceinline__ float hllc_energy(float r, float p, float v0, float v1, float v2) {
-  return p * 2.5f + 0.5f * r * (v0 * v0 + v1 * v1 + v2 * v2);
+// Total enthalpy per volume E + p of a side of a face from its primitives
+// (rho, p, velocity): 3.5 p + r |v|^2 / 2 for gamma = 1.4.
+__device__ __forceinline__ float hllc_enthalpy(float r, float p, float v0, float v1, float v2) {
+  return p * 3.5f + 0.5f * r * (v0 * v0 + v1 * v1 + v2 * v2);
 }
 
 // HLLC flux along axis d (src/air_cfd.js:179-195) with four hardware
 // transcendentals per face: two reciprocal square roots for the sound speeds,
-// the reciprocals of the contact-speed denominator and of S - SM (the star
-// energy is regrouped so that no 1/r or 1/(S - u) is needed). The momentum
-// fluxes are the mass flux times the upwind velocity components plus the
-// pressure (and, in the star region, S (f SM - r u)) on the normal one: the
-// reference's F + S (U* - U) with the common factor taken out.
+// the reciprocals of the contact-speed denominator and of S - SM. The
+// reference's F + S (U* - U) is regrouped through one common factor (below),
+// so that no 1/r or 1/(S - u) is needed; the tangential momentum fluxes are
+// the mass flux times the upwind side's velocity components.
 template <int d>
 __device__ __forceinline__ void hllc_fast(const float *L, const float *R, float *out) {
   const float rL = L[0], rR = R[0], pL = L[4], pR = R[4], uL = L[1 + d], uR = R[1 + d];
   const float gpL = kGamma * pL, gpR = kGamma * pR;   // a = sqrt(g p / r) = g p / sqrt(g p r)
   const float aL = gpL * frsq(gpL * rL), aR = gpR * frsq(gpR * rR);
   const float SL = fminf(uL - aL, uR - aR), SR = fmaxf(uL + aL, uR + aR);
-  // total energy of the chosen side only (hllc_energy), from that side's
-  // selected values: the same operations as for either side, so the same bits
+  // each branch forms the total enthalpy of the chosen side only (hllc_enthalpy)
   if (SL >= 0.f || SR <= 0.f) {              // supersonic: the upwind side's flux
     const bool left = SL >= 0.f;
     const float r = left ? rL : rR, u = left ? uL : uR, p = left ? pL : pR;
     const float v0 = left ? L[1] : R[1], v1 = left ? L[2] : R[2], v2 = left ? L[3] : R[3];
-    const float E = hllc_energy(r, p, v0, v1, v2);
+    const float H = hllc_enthalpy(r, p, v0, v1, v2);
     const float m = r * u;
     // each component stored once (no update of an element picked by d)
     out[0] = m;
     out[1] = d == 0 ? m * v0 + p : m * v0;
     out[2] = d == 1 ? m * v1 + p : m * v1;
     out[3] = d == 2 ? m * v2 + p : m * v2;
-    out[4] = u * (E + p);
+    out[4] = u * H;
     return;
   }
-  const float SM = (pR - pL + rL * uL * (SL - uL) - rR * uR * (SR - uR)) * frcp(rL * (SL - uL) - rR * (SR - uR));
+  const float AL = rL * (SL - uL), AR = rR * (SR - uR);   // contact speed from r (S - u) of both sides
+  const float SM = (pR - pL + uL * AL - uR * AR) * frcp(AL - AR);
   const bool left = SM >= 0.f;
   const float S = left ? SL : SR, uK = left ? uL : uR, rK = left ? rL : rR, pK = left ? pL : pR;
   const float v0 = left ? L[1] : R[1], v1 = left ? L[2] : R[2], v2 = left ? L[3] : R[3];
-  const float EK = hllc_energy(rK, pK, v0, v1, v2);
+  const float HK = hllc_enthalpy(rK, pK, v0, v1, v2);
   // F + S (U* - U) with the star state U* = f (1, SM, v_t, E / r + (SM - u)(SM + p / (r (S - u)))),
   // f = r (S - u) / (S - SM): every component of S (U* - U) reduces through
   // g = S (SM - u) / (S - SM), giving the mass flux m = r (u + g), the normal
-  // momentum flux r u^2 + p + S r g and the energy flux (u + g)(E + p) + r g (S - u) SM
+  // momentum flux r u^2 + p + S r g = m u + p + t and the energy flux
+  // (u + g)(E + p) + t SM, with t = r g (S - u)
   const float SmU = S - uK, g = S * (SM - uK) * frcp(S - SM);
-  const float ug = uK + g, rg = rK * g;
+  const float ug = uK + g, t = rK * g * SmU;
   const float m = rK * ug;                   // mass flux
-  const float mn = rK * uK * uK + pK + S * rg;   // normal momentum
+  const float mn = fmaf(m, uK, pK) + t;      // normal momentum
   out[0] = m;
   out[1] = d == 0 ? mn : m * v0;
   out[2] = d == 1 ? mn : m * v1;
   out[3] = d == 2 ? mn : m * v2;
-  out[4] = ug * (EK + pK) + rg * SmU * SM;
+  out[4] = fmaf(ug, HK, t * SM);
 }
 
 // Face flux of axis d between left cell (PL, TL, slope sL) and right cell
