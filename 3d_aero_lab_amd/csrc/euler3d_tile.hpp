@@ -597,7 +597,11 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
         const float dtp = cph * frcp(den * ph);
         float n[5];
 #pragma unroll
-        for (int c = 0; c < 5; c++) n[c] = wA * Bq[c] + wB * (Qc[c] + dtp * R[c]);
+        for (int c = 0; c < 5; c++) {
+          // stage 1 is Q + dt R itself (a product 0 * base would not be dropped by the compiler)
+          if constexpr (kS1) n[c] = Qc[c] + dtp * R[c];
+          else n[c] = wA * Bq[c] + wB * (Qc[c] + dtp * R[c]);
+        }
         n[0] = fmaxf(n[0], 1e-4f);
         const float ke = 0.5f * (n[1] * n[1] + n[2] * n[2] + n[3] * n[3]) * frcp(n[0]);
         if (kGm1 * (n[4] - ke) < 1e-5f) n[4] = 2.5e-5f + ke;
