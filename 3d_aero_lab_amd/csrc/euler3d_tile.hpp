@@ -147,26 +147,28 @@ __device__ __forceinline__ void hllc_fast(const float *L, const float *R, float 
   out[4] = fmaf(ug, HK, t * SM);
 }
 
-// Face flux of axis d between left cell (PL, TL, slope sL) and right cell
-// (PR, TR, slope sR) with aperture a; zero when the face is closed or touches
-// no fluid cell (no cell would use it).
+// Face flux of axis d between left cell (PL, slope sL) and right cell (PR,
+// slope sR) with aperture a; zero when the face is closed. A face between two
+// cells that are not fluid is computed like any other (no cell update uses
+// it): the reference closes every face that touches a solid cell, and the
+// tile gives the columns and rows outside the box aperture 0, so such faces
+// are fixed- or ghost-cell faces.
 // kPin (operands read from LDS): every operand is an input of one empty asm
 // statement before the face is tested, so all of its LDS reads are issued
 // together and waited for once; otherwise the reads are sunk below the test,
 // a second LDS round trip. Input operands only: read-write ones made the
 // compiler copy the values out of the paired LDS reads' registers.
 template <int d, bool kPin = false>
-__device__ __forceinline__ void face_flux(float *PL, float *sL, int TL, float *PR, float *sR, int TR, float a, float *F) {
+__device__ __forceinline__ void face_flux(float *PL, float *sL, float *PR, float *sR, float a, float *F) {
   if constexpr (kPin) {
 #ifdef __AMDGCN__
     // one statement, so the reads are waited for once (one wait per pinned value otherwise)
     asm volatile("" ::"v"(PL[0]), "v"(PL[1]), "v"(PL[2]), "v"(PL[3]), "v"(PL[4]), "v"(sL[0]), "v"(sL[1]), "v"(sL[2]),
                  "v"(sL[3]), "v"(sL[4]), "v"(PR[0]), "v"(PR[1]), "v"(PR[2]), "v"(PR[3]), "v"(PR[4]), "v"(sR[0]),
-                 "v"(sR[1]), "v"(sR[2]), "v"(sR[3]), "v"(sR[4]), "v"(TL), "v"(TR), "v"(a));
+                 "v"(sR[1]), "v"(sR[2]), "v"(sR[3]), "v"(sR[4]), "v"(a));
 #endif
   }
-  // one test without short-circuit: a single branch, so the zero flux is set once
-  if ((!(a > 0.f)) | ((TL != 0) & (TR != 0))) {
+  if (!(a > 0.f)) {
 #pragma unroll
     for (int v = 0; v < 5; v++) F[v] = 0.f;
     return;
@@ -271,7 +273,9 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
   const float ihx = own ? frcp(cv.dx[i]) : 0.f, ihy = own ? frcp(cv.dy[j]) : 0.f;
   const float *__restrict__ ap0 = cv.ap, *__restrict__ ap1 = cv.ap + cv.N, *__restrict__ ap2 = cv.ap + 2 * (size_t)cv.N,
                             *__restrict__ ap3 = cv.ap + 3 * (size_t)cv.N;
-  auto apz = [&](int k) -> float { return (k < 0 || k >= NZ || !own) ? 1.f : gld(ap2, (unsigned)(k * NXY) + qc); };
+  // z-face aperture of the own column; 1 past the box's bottom and top (open
+  // boundary faces), 0 for columns outside the box (their faces are never used)
+  auto apz = [&](int k) -> float { return !own ? 0.f : (k < 0 || k >= NZ) ? 1.f : gld(ap2, (unsigned)(k * NXY) + qc); };
 
   // halo ring (no corners: 4 rows of TX, 4 columns of TY), NHP cells per
   // thread; each cell's raw state is loaded one plane ahead (ha / ht) so the
@@ -318,8 +322,8 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
   float aX = 1.f, aY = 1.f, aE = 1.f, aS = 1.f;
   auto ap_load = [&](int kk) {
     const unsigned qq = (unsigned)(kk * NXY) + qc;
-    aX = own ? gld(ap0, qq) : 1.f;
-    aY = own ? gld(ap1, qq) : 1.f;
+    aX = own ? gld(ap0, qq) : 0.f;   // columns / rows outside the box: faces never used
+    aY = own ? gld(ap1, qq) : 0.f;
     if (tidv < 2 * TY) {
       const int r = tidv >> 1, ii = (tidv & 1) - 2, gi = i0 + ii, gj = j0 + r;
       aE = (gi < 0 || gi >= NX || gj >= NY) ? 1.f : gld(ap0, (unsigned)((kk * NY + gj) * NX + gi));
@@ -373,7 +377,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
   float sz0[5], sz1[5], Fzm[5];
   slope5(Pm2, Pz[0], Pz[1], o2 && Tm2 == 0 && Tz[0] == 0 && Tz[1] == 0 && Am1 == 1.f && A0z == 1.f, sz0);   // s(k0-1)
   slope5(Pz[0], Pz[1], Pz[2], o2 && Tz[0] == 0 && Tz[1] == 0 && Tz[2] == 0 && A0z == 1.f && A1z == 1.f, sz1); // s(k0)
-  face_flux<2>(Pz[0], sz0, Tz[0], Pz[1], sz1, Tz[1], A0z, Fzm);                                             // F(k0-1/2)
+  face_flux<2>(Pz[0], sz0, Pz[1], sz1, A0z, Fzm);                                             // F(k0-1/2)
 #pragma unroll
   for (int v = 0; v < 5; v++) sz0[v] = sz1[v];                                                              // sz0 = s(k)
   // queue: Pz[0] = k-1, Pz[1] = k, Pz[2] = k+1; Pz[3] from the raw state of
@@ -494,21 +498,20 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       float PL[5], PR[5], sL[5], sR[5], F[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) { PL[v] = sm.P[yj][xi][v]; PR[v] = sm.P[yj][xi + 1][v]; sL[v] = sm.Sx[ty][tx + 1][v]; sR[v] = sm.Sx[ty][tx + 2][v]; }
-      face_flux<0, true>(PL, sL, sm.T[yj][xi], PR, sR, sm.T[yj][xi + 1], sm.A0[ty][tx + 2], F);
+      face_flux<0, true>(PL, sL, PR, sR, sm.A0[ty][tx + 2], F);
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Fx[ty][tx + 1][v] = F[v];
       if constexpr (kEarlyY) {   // the y face's LDS reads go out before the z face (registers only) is computed
         float PRy[5], sLy[5], sRy[5];
 #pragma unroll
         for (int v = 0; v < 5; v++) { PRy[v] = sm.P[yj + 1][xi][v]; sLy[v] = sm.Sy[ty + 1][tx][v]; sRy[v] = sm.Sy[ty + 2][tx][v]; }
-        const int Ty0 = sm.T[yj][xi], Ty1 = sm.T[yj + 1][xi];
         const float Ay = sm.A1[ty + 2][tx];
-        face_flux<2>(Pz[1], sz0, Tz[1], Pz[2], sz1n, Tz[2], A1z, Fzp);
-        face_flux<1, true>(PL, sLy, Ty0, PRy, sRy, Ty1, Ay, F);
+        face_flux<2>(Pz[1], sz0, Pz[2], sz1n, A1z, Fzp);
+        face_flux<1, true>(PL, sLy, PRy, sRy, Ay, F);
       } else {
 #pragma unroll
         for (int v = 0; v < 5; v++) { PR[v] = sm.P[yj + 1][xi][v]; sL[v] = sm.Sy[ty + 1][tx][v]; sR[v] = sm.Sy[ty + 2][tx][v]; }
-        face_flux<1, true>(PL, sL, sm.T[yj][xi], PR, sR, sm.T[yj + 1][xi], sm.A1[ty + 2][tx], F);
+        face_flux<1, true>(PL, sL, PR, sR, sm.A1[ty + 2][tx], F);
       }
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Fy[ty + 1][tx][v] = F[v];
@@ -518,7 +521,7 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       float PL[5], PR[5], sL[5], sR[5], F[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) { PL[v] = sm.P[1][xi][v]; PR[v] = sm.P[2][xi][v]; sL[v] = sm.Sy[0][c][v]; sR[v] = sm.Sy[1][c][v]; }
-      face_flux<1, true>(PL, sL, sm.T[1][xi], PR, sR, sm.T[2][xi], sm.A1[1][c], F);
+      face_flux<1, true>(PL, sL, PR, sR, sm.A1[1][c], F);
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Fy[0][c][v] = F[v];
     } else if (tidv >= 64 && tidv < 64 + TY) {           // x face left of column 0, row r (wave 1)
@@ -526,11 +529,11 @@ __device__ __forceinline__ void tile_stage(const CaseView &cv, const StageConsts
       float PL[5], PR[5], sL[5], sR[5], F[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) { PL[v] = sm.P[yj][1][v]; PR[v] = sm.P[yj][2][v]; sL[v] = sm.Sx[r][0][v]; sR[v] = sm.Sx[r][1][v]; }
-      face_flux<0, true>(PL, sL, sm.T[yj][1], PR, sR, sm.T[yj][2], sm.A0[r][1], F);
+      face_flux<0, true>(PL, sL, PR, sR, sm.A0[r][1], F);
 #pragma unroll
       for (int v = 0; v < 5; v++) sm.Fx[r][0][v] = F[v];
     }
-    if constexpr (!kEarlyY) face_flux<2>(Pz[1], sz0, Tz[1], Pz[2], sz1n, Tz[2], A1z, Fzp);
+    if constexpr (!kEarlyY) face_flux<2>(Pz[1], sz0, Pz[2], sz1n, A1z, Fzp);
     // own apertures into registers (LDS apertures are rewritten by the next plane's load)
     const float ax_m = sm.A0[ty][tx + 1], ax_p = sm.A0[ty][tx + 2], ay_m = sm.A1[ty + 1][tx], ay_p = sm.A1[ty + 2][tx];
     // this plane's update operands (loaded one plane earlier), then the next plane's loads
